@@ -1,0 +1,165 @@
+/*
+ * rmh.h -- C ABI of librmh.so, the MI355X (gfx950) ray-casting simulation library.
+ *
+ * Every entry point is extern "C", takes plain pointers and sizes, and returns an int
+ * status (RMH_OK == 0).  On failure the thread-local message is available from
+ * rmh_last_error().  Device pointers ("_dev") are HIP device allocations; host
+ * pointers are ordinary memory.  Streams are hipStream_t passed as void* (NULL means the
+ * legacy default stream).
+ *
+ * The entry points replace the following pieces of Rmagine (uos/rmagine v2.4.2):
+ *   rmh_map_build        OptixScene::commit GAS build      src/rmagine_optix/src/map/optix/OptixScene.cpp:230-379
+ *                        EmbreeScene::commit               src/rmagine_embree/src/map/embree/EmbreeScene.cpp:187-191
+ *                        face normals                      src/rmagine_cuda/src/map/mesh_preprocessing.cu:6-43
+ *   rmh_sensor_create    Sphere/Pinhole/O1Dn SimulatorOptix::setModel (uploads the model)
+ *                                                          src/rmagine_optix/src/simulation/SphereSimulatorOptix.cpp:20-60
+ *                                                          src/rmagine_optix/src/simulation/O1DnSimulatorOptix.cpp:41-83
+ *   rmh_simulate         *SimulatorOptix::simulate(Memory<Transform,VRAM_CUDA>, Bundle&)
+ *                                                          src/rmagine_optix/include/rmagine/simulation/SphereSimulatorOptix.tcc:41-100
+ *                        with the Embree result contract   src/rmagine_embree/include/rmagine/simulation/SphereSimulatorEmbree.tcc:50-254
+ *   rmh_malloc/...       VRAM_CUDA memory tag              src/rmagine_cuda/src/types/MemoryCuda.cu:13-89
+ */
+#ifndef RMH_H
+#define RMH_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RMH_VERSION_MAJOR 0
+#define RMH_VERSION_MINOR 1
+
+/* status codes */
+#define RMH_OK              0
+#define RMH_ERR_INVALID     1  /* bad argument (null pointer, size, kind)         */
+#define RMH_ERR_HIP         2  /* a HIP runtime call failed                         */
+#define RMH_ERR_NOMAP       3  /* simulate without a map (reference: runtime_error) */
+#define RMH_ERR_ALLOC       4  /* device allocation failed                          */
+#define RMH_ERR_UNSUPPORTED 5  /* feature or size not supported                     */
+
+/* ---- POD layouts, byte-identical to Rmagine's types ------------------------------ */
+
+/* rmagine::Vector3 (src/rmagine_core/include/rmagine/math/types/Vector3.hpp:54-56) */
+typedef struct { float x, y, z; } rmh_vec3;
+/* rmagine::Quaternion (Quaternion.hpp:61-64) */
+typedef struct { float x, y, z, w; } rmh_quat;
+/* rmagine::Transform: 32 B {q, t, stamp} (Transform.hpp:63-68) */
+typedef struct { rmh_quat R; rmh_vec3 t; uint32_t stamp; } rmh_transform;
+/* rmagine::Interval (src/rmagine_core/include/rmagine/types/sensor_models.h:55-70) */
+typedef struct { float min, max; } rmh_interval;
+/* rmagine::DiscreteInterval (sensor_models.h:72-110) */
+typedef struct { float min, inc; uint32_t size; } rmh_discrete_interval;
+/* rmagine::SphericalModel, 32 B (sensor_models.h:122-214) */
+typedef struct { rmh_discrete_interval phi, theta; rmh_interval range; } rmh_spherical_model;
+/* rmagine::PinholeModel, 32 B (sensor_models.h:218-316) */
+typedef struct { uint32_t width, height; rmh_interval range; float f[2]; float c[2]; } rmh_pinhole_model;
+/* rmagine::O1DnModel_ flattened (sensor_models.h:338-418): dirs is a device array of
+ * width*height directions, copied by rmh_sensor_create. */
+typedef struct {
+  uint32_t width, height; rmh_interval range; rmh_vec3 orig; uint32_t _pad;
+  const rmh_vec3* dirs_dev;
+} rmh_o1dn_model;
+/* rmagine::OnDnModel_ flattened (sensor_models.h:422-503): origs and dirs are device
+ * arrays of width*height entries, copied by rmh_sensor_create. */
+typedef struct {
+  uint32_t width, height; rmh_interval range;
+  const rmh_vec3* origs_dev; const rmh_vec3* dirs_dev;
+} rmh_ondn_model;
+
+enum { RMH_MODEL_SPHERICAL = 0, RMH_MODEL_PINHOLE = 1, RMH_MODEL_O1DN = 2, RMH_MODEL_ONDN = 3 };
+
+/* Result buffers of one simulate call (src/rmagine_core/include/rmagine/simulation/
+ * SimulationResults.hpp:55-137).  Each is a device array of n_poses*width*height
+ * entries indexed glob_id = pose*W*H + row*W + col, or NULL when the attribute is not
+ * requested (the reference enables an attribute iff its buffer size > 0,
+ * src/rmagine_embree/include/rmagine/simulation/embree_common.h:79-139). */
+typedef struct {
+  uint8_t*  hits;
+  float*    ranges;
+  rmh_vec3* points;
+  rmh_vec3* normals;
+  uint32_t* face_ids;
+  uint32_t* geom_ids;
+  uint32_t* object_ids;
+} rmh_outputs;
+
+typedef struct rmh_map rmh_map;
+typedef struct rmh_sensor rmh_sensor;
+
+/* BVH build parameters (NULL = defaults). */
+typedef struct {
+  uint32_t max_leaf_size;   /* 1..8, default 4                          */
+  float    sah_cost_node;   /* SAH traversal cost, default 1.0          */
+  float    sah_cost_tri;    /* SAH intersection cost, default 1.5       */
+  uint32_t flags;           /* reserved, 0                              */
+} rmh_bvh_params;
+
+typedef struct {
+  uint32_t n_vertices, n_faces, n_meshes;
+  uint32_t n_nodes;         /* reachable interior nodes                   */
+  uint32_t n_leaves;        /* reachable leaves                           */
+  uint32_t max_depth;       /* deepest leaf (root = 0)                    */
+  uint64_t device_bytes;    /* nodes + triangles + normals                */
+  float    build_ms;        /* device time of the build                   */
+  float    sah_cost;        /* SAH cost of the tree (root-area relative)  */
+} rmh_map_info;
+
+/* ---- runtime ---------------------------------------------------------------------- */
+const char* rmh_last_error(void);
+int rmh_version(void);
+int rmh_get_device_count(int* count);
+int rmh_set_device(int device);
+int rmh_malloc(void** ptr_dev, size_t bytes);
+int rmh_free(void* ptr_dev);
+/* kind: 0 host->device, 1 device->host, 2 device->device, 3 host->host.  Blocking, like
+ * the reference's cudaMemcpy copies (MemoryCuda.cu:25-29). */
+int rmh_memcpy(void* dst, const void* src, size_t bytes, int kind);
+int rmh_memset(void* ptr_dev, int value, size_t bytes);
+int rmh_stream_create(void** stream);
+int rmh_stream_destroy(void* stream);
+int rmh_stream_synchronize(void* stream);
+int rmh_device_synchronize(void);
+
+/* ---- map (mesh + BVH in HBM) ------------------------------------------------------- */
+/* vertices_dev: n_vertices xyz float triples; faces_dev: n_faces uint32 index triples
+ * (indices into the concatenated vertex array).  mesh_face_offsets (host, n_meshes+1
+ * entries, or NULL for a single mesh) splits the faces into geometries: face f with
+ * offsets[g] <= f < offsets[g+1] reports geom_id = object_id = g and face_id =
+ * f - offsets[g], which is what Embree reports for meshes attached to one scene
+ * (EmbreeScene.cpp:54-71).  The map copies the mesh; the inputs may be freed after the
+ * call returns.  The build runs on `stream` and is synchronised before returning. */
+int rmh_map_build(const float* vertices_dev, uint32_t n_vertices,
+                  const uint32_t* faces_dev, uint32_t n_faces,
+                  const uint32_t* mesh_face_offsets, uint32_t n_meshes,
+                  const rmh_bvh_params* params, void* stream, rmh_map** out);
+int rmh_map_destroy(rmh_map* map);
+int rmh_map_get_info(const rmh_map* map, rmh_map_info* info);
+
+/* ---- sensors ------------------------------------------------------------------------ */
+/* model points at the host struct of `kind` (rmh_spherical_model, rmh_pinhole_model,
+ * rmh_o1dn_model, rmh_ondn_model).  The sensor keeps its own device copy of everything
+ * it needs (direction tables, O1Dn/OnDn rays). */
+int rmh_sensor_create(int kind, const void* model, rmh_sensor** out);
+int rmh_sensor_destroy(rmh_sensor* sensor);
+int rmh_sensor_dims(const rmh_sensor* sensor, uint32_t* width, uint32_t* height);
+
+/* ---- simulation --------------------------------------------------------------------- */
+/* Casts n_poses * W * H rays: for pose p the sensor pose is Tsm = Tbm_dev[p] * Tsb,
+ * rays start at Tsm applied to the model origin with direction Tsm.R * d_s, tnear = 0,
+ * tfar = range.max, closest hit over two-sided triangles.  Results follow the Embree
+ * contract (SphereSimulatorEmbree.tcc:104-249): hit = t >= range.min, range = t (also
+ * when t < range.min), points/normals in the sensor frame, normals flipped toward the
+ * sensor; misses write hits 0, range.max + 1, NaN points/normals, ids 0xFFFFFFFF.
+ * Tsb is a host transform (NULL = identity).  Asynchronous on `stream`. */
+int rmh_simulate(const rmh_map* map, const rmh_sensor* sensor,
+                 const rmh_transform* Tsb, const rmh_transform* Tbm_dev, uint32_t n_poses,
+                 const rmh_outputs* out, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RMH_H */

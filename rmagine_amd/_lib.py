@@ -1,0 +1,89 @@
+"""ctypes binding of librmh.so (include/rmh.h).
+
+The library is loaded from rmagine_amd/_lib/librmh.so.  There is no fallback: if the
+library is missing every entry point raises, so nothing silently runs on the CPU.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from pathlib import Path
+
+from . import types as T
+
+LIB_PATH = Path(__file__).resolve().parent / "_lib" / "librmh.so"
+
+_lib = None
+
+
+class RmhError(RuntimeError):
+    """Error reported by librmh (status code + rmh_last_error message)."""
+
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"librmh error {code}: {msg}")
+        self.code = code
+
+
+class NoMapError(RmhError):
+    """simulate() without a map (the reference throws std::runtime_error)."""
+
+
+c_void_p = ctypes.c_void_p
+c_u32 = ctypes.c_uint32
+c_int = ctypes.c_int
+c_size = ctypes.c_size_t
+
+_SIGNATURES = {
+    "rmh_last_error": (ctypes.c_char_p, []),
+    "rmh_version": (c_int, []),
+    "rmh_get_device_count": (c_int, [ctypes.POINTER(c_int)]),
+    "rmh_set_device": (c_int, [c_int]),
+    "rmh_malloc": (c_int, [ctypes.POINTER(c_void_p), c_size]),
+    "rmh_free": (c_int, [c_void_p]),
+    "rmh_memcpy": (c_int, [c_void_p, c_void_p, c_size, c_int]),
+    "rmh_memset": (c_int, [c_void_p, c_int, c_size]),
+    "rmh_stream_create": (c_int, [ctypes.POINTER(c_void_p)]),
+    "rmh_stream_destroy": (c_int, [c_void_p]),
+    "rmh_stream_synchronize": (c_int, [c_void_p]),
+    "rmh_device_synchronize": (c_int, []),
+    "rmh_map_build": (c_int, [c_void_p, c_u32, c_void_p, c_u32, ctypes.POINTER(c_u32), c_u32,
+                              ctypes.POINTER(T.rmh_bvh_params), c_void_p, ctypes.POINTER(c_void_p)]),
+    "rmh_map_destroy": (c_int, [c_void_p]),
+    "rmh_map_get_info": (c_int, [c_void_p, ctypes.POINTER(T.rmh_map_info)]),
+    "rmh_sensor_create": (c_int, [c_int, c_void_p, ctypes.POINTER(c_void_p)]),
+    "rmh_sensor_destroy": (c_int, [c_void_p]),
+    "rmh_sensor_dims": (c_int, [c_void_p, ctypes.POINTER(c_u32), ctypes.POINTER(c_u32)]),
+    "rmh_simulate": (c_int, [c_void_p, c_void_p, ctypes.POINTER(T.rmh_transform), c_void_p, c_u32,
+                             ctypes.POINTER(T.rmh_outputs), c_void_p]),
+}
+
+
+def exported_symbols() -> list[str]:
+    return list(_SIGNATURES)
+
+
+def lib() -> ctypes.CDLL:
+    """Loads librmh.so once; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m rmagine_amd.build` "
+                               "(there is no CPU fallback)")
+        _lib = ctypes.CDLL(str(LIB_PATH), mode=os.RTLD_NOW | os.RTLD_GLOBAL)
+        for name, (res, args) in _SIGNATURES.items():
+            fn = getattr(_lib, name)
+            fn.restype = res
+            fn.argtypes = args
+    return _lib
+
+
+def check(code: int) -> None:
+    if code != 0:
+        msg = lib().rmh_last_error().decode(errors="replace")
+        if code == 3:
+            raise NoMapError(code, msg)
+        raise RmhError(code, msg)
+
+
+def call(name: str, *args) -> None:
+    check(getattr(lib(), name)(*args))

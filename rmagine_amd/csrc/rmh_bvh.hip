@@ -1,0 +1,518 @@
+// GPU BVH builder of librmh: the role of OptixScene's GAS build
+// (src/rmagine_optix/src/map/optix/OptixScene.cpp:230-379) and EmbreeScene::commit
+// (src/rmagine_embree/src/map/embree/EmbreeScene.cpp:187-191).
+//
+// Steps, all on the device:
+//   1. per face: bounds, centroid, scene centroid bounds (wave64 shuffle + one atomic
+//      per block), face-index validation;
+//   2. 63-bit Morton codes of the centroids;
+//   3. radix sort of (code, face) pairs (rocPRIM);
+//   4. Karras 2012 radix tree over the sorted codes (index-augmented for duplicates);
+//   5. bottom-up pass with one atomic arrival counter per interior node: node boxes,
+//      primitive counts and the area-weighted SAH cost; a subtree of at most
+//      max_leaf_size triangles becomes a leaf when that is cheaper under SAH;
+//   6. layout: 64-byte binary nodes that hold both child boxes (padded outward), plus
+//      depth statistics of the reachable tree;
+//   7. gather of the triangles in Morton order (3 x float4 per triangle with the face,
+//      geometry and global ids in the w lanes) and of FP64-derived unit face normals.
+// The normals play the part of EmbreeMesh::computeFaceNormals
+// (src/rmagine_embree/src/map/embree/EmbreeMesh.cpp:275-295) and of
+// computeFaceNormals_kernel (src/rmagine_cuda/src/map/mesh_preprocessing.cu:6-20).
+#include "rmh_internal.hpp"
+
+#include <rocprim/device/device_radix_sort.hpp>
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <new>
+#include <vector>
+
+namespace {
+
+using namespace rmh;
+
+constexpr int kBlock = 256;
+constexpr uint32_t kLeafBit = 0x80000000u;
+
+__device__ inline void atomic_min_float(float* addr, float v) {
+  if (v >= 0.0f) atomicMin(reinterpret_cast<int*>(addr), __float_as_int(v));
+  else atomicMax(reinterpret_cast<unsigned int*>(addr), __float_as_uint(v));
+}
+
+__device__ inline void atomic_max_float(float* addr, float v) {
+  if (v >= 0.0f) atomicMax(reinterpret_cast<int*>(addr), __float_as_int(v));
+  else atomicMin(reinterpret_cast<unsigned int*>(addr), __float_as_uint(v));
+}
+
+__device__ inline float wave_min(float v) {
+  for (int off = 32; off > 0; off >>= 1) v = fminf(v, __shfl_xor(v, off, 64));
+  return v;
+}
+
+__device__ inline float wave_max(float v) {
+  for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, 64));
+  return v;
+}
+
+__global__ void k_init(float* bounds, uint32_t* err) {
+  if (threadIdx.x < 3) bounds[threadIdx.x] = FLT_MAX;
+  else if (threadIdx.x < 6) bounds[threadIdx.x] = -FLT_MAX;
+  else if (threadIdx.x == 6) *err = 0u;
+}
+
+// Step 1.
+__global__ __launch_bounds__(kBlock) void k_prep(const float* __restrict__ V, uint32_t nv,
+                                                 const uint32_t* __restrict__ F, uint32_t nf,
+                                                 float4* __restrict__ lo, float4* __restrict__ hi,
+                                                 float* bounds, uint32_t* err) {
+  const uint32_t f = blockIdx.x * kBlock + threadIdx.x;
+  float cmin[3] = {FLT_MAX, FLT_MAX, FLT_MAX};
+  float cmax[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+  if (f < nf) {
+    uint32_t idx[3] = {F[3 * f + 0], F[3 * f + 1], F[3 * f + 2]};
+    bool bad = false;
+    for (int k = 0; k < 3; k++) {
+      if (idx[k] >= nv) { bad = true; idx[k] = 0; }
+    }
+    if (bad) atomicOr(err, 1u);
+    float b0[3], b1[3];
+    for (int a = 0; a < 3; a++) {
+      const float x0 = V[3 * idx[0] + a], x1 = V[3 * idx[1] + a], x2 = V[3 * idx[2] + a];
+      b0[a] = fminf(x0, fminf(x1, x2));
+      b1[a] = fmaxf(x0, fmaxf(x1, x2));
+      if (!(isfinite(x0) && isfinite(x1) && isfinite(x2))) atomicOr(err, 2u);
+      const float c = 0.5f * b0[a] + 0.5f * b1[a];
+      cmin[a] = c;
+      cmax[a] = c;
+    }
+    lo[f] = make_float4(b0[0], b0[1], b0[2], 0.0f);
+    hi[f] = make_float4(b1[0], b1[1], b1[2], 0.0f);
+  }
+  __shared__ float red[2][3][kBlock / 64];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  for (int a = 0; a < 3; a++) {
+    const float mn = wave_min(cmin[a]), mx = wave_max(cmax[a]);
+    if (lane == 0) { red[0][a][wid] = mn; red[1][a][wid] = mx; }
+  }
+  __syncthreads();
+  if (threadIdx.x < 3) {
+    const int a = threadIdx.x;
+    float mn = red[0][a][0], mx = red[1][a][0];
+    for (int w = 1; w < kBlock / 64; w++) { mn = fminf(mn, red[0][a][w]); mx = fmaxf(mx, red[1][a][w]); }
+    atomic_min_float(&bounds[a], mn);
+    atomic_max_float(&bounds[3 + a], mx);
+  }
+}
+
+__device__ inline uint64_t split3_21(uint32_t x) {
+  uint64_t v = x & 0x1fffffu;
+  v = (v | (v << 32)) & 0x001f00000000ffffull;
+  v = (v | (v << 16)) & 0x001f0000ff0000ffull;
+  v = (v | (v << 8)) & 0x100f00f00f00f00full;
+  v = (v | (v << 4)) & 0x10c30c30c30c30c3ull;
+  v = (v | (v << 2)) & 0x1249249249249249ull;
+  return v;
+}
+
+// Step 2.
+__global__ __launch_bounds__(kBlock) void k_morton(const float4* __restrict__ lo, const float4* __restrict__ hi,
+                                                   uint32_t nf, const float* __restrict__ bounds,
+                                                   uint64_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+  const uint32_t f = blockIdx.x * kBlock + threadIdx.x;
+  if (f >= nf) return;
+  const float4 a = lo[f], b = hi[f];
+  const float c[3] = {0.5f * a.x + 0.5f * b.x, 0.5f * a.y + 0.5f * b.y, 0.5f * a.z + 0.5f * b.z};
+  uint32_t q[3];
+  for (int k = 0; k < 3; k++) {
+    const float ext = bounds[3 + k] - bounds[k];
+    float u = ext > 0.0f ? (c[k] - bounds[k]) / ext : 0.5f;
+    u = fminf(fmaxf(u, 0.0f), 1.0f);
+    q[k] = static_cast<uint32_t>(u * 2097151.0f);
+  }
+  keys[f] = (split3_21(q[0]) << 2) | (split3_21(q[1]) << 1) | split3_21(q[2]);
+  vals[f] = f;
+}
+
+__device__ inline int delta(const uint64_t* __restrict__ keys, int n, int i, int j) {
+  if (j < 0 || j >= n) return -1;
+  const uint64_t a = keys[i], b = keys[j];
+  if (a == b) return 64 + __clz(static_cast<uint32_t>(i ^ j));
+  return __clzll(a ^ b);
+}
+
+// Step 4: Karras, "Maximizing Parallelism in the Construction of BVHs, Octrees, and
+// k-d Trees", HPG 2012, Fig. 4.
+__global__ __launch_bounds__(kBlock) void k_karras(const uint64_t* __restrict__ keys, int n,
+                                                   uint32_t* __restrict__ child, uint32_t* __restrict__ range,
+                                                   int* __restrict__ parent_int, int* __restrict__ parent_leaf) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n - 1) return;
+  const int d = (delta(keys, n, i, i + 1) - delta(keys, n, i, i - 1)) >= 0 ? 1 : -1;
+  const int dmin = delta(keys, n, i, i - d);
+  int lmax = 2;
+  while (delta(keys, n, i, i + lmax * d) > dmin) lmax <<= 1;
+  int l = 0;
+  for (int t = lmax >> 1; t >= 1; t >>= 1)
+    if (delta(keys, n, i, i + (l + t) * d) > dmin) l += t;
+  const int j = i + l * d;
+  const int dnode = delta(keys, n, i, j);
+  int s = 0;
+  int t = l;
+  do {
+    t = (t + 1) >> 1;
+    if (delta(keys, n, i, i + (s + t) * d) > dnode) s += t;
+  } while (t > 1);
+  const int gamma = i + s * d + (d < 0 ? -1 : 0);
+  const int first = min(i, j), last = max(i, j);
+  uint32_t left, right;
+  if (first == gamma) { left = static_cast<uint32_t>(gamma) | kLeafBit; parent_leaf[gamma] = i; }
+  else { left = static_cast<uint32_t>(gamma); parent_int[gamma] = i; }
+  if (last == gamma + 1) { right = static_cast<uint32_t>(gamma + 1) | kLeafBit; parent_leaf[gamma + 1] = i; }
+  else { right = static_cast<uint32_t>(gamma + 1); parent_int[gamma + 1] = i; }
+  child[2 * i + 0] = left;
+  child[2 * i + 1] = right;
+  range[2 * i + 0] = static_cast<uint32_t>(first);
+  range[2 * i + 1] = static_cast<uint32_t>(last);
+  if (i == 0) parent_int[0] = -1;
+}
+
+__device__ inline float half_area(float4 a, float4 b) {
+  const float dx = b.x - a.x, dy = b.y - a.y, dz = b.z - a.z;
+  return dx * dy + dy * dz + dz * dx;
+}
+
+// Step 5.  node_lo.w = SAH cost (area weighted), node_hi.w = primitive count.
+__global__ __launch_bounds__(kBlock) void k_bottom_up(int n, const uint32_t* __restrict__ vals,
+                                                      const float4* __restrict__ flo, const float4* __restrict__ fhi,
+                                                      const uint32_t* __restrict__ child,
+                                                      const int* __restrict__ parent_int,
+                                                      const int* __restrict__ parent_leaf,
+                                                      float4* node_lo, float4* node_hi,
+                                                      uint32_t* arrive, uint8_t* collapse,
+                                                      uint32_t max_leaf, float c_node, float c_tri) {
+  const int k = blockIdx.x * kBlock + threadIdx.x;
+  if (k >= n) return;
+  int p = parent_leaf[k];
+  while (p >= 0) {
+    __threadfence();
+    const uint32_t old = __hip_atomic_fetch_add(&arrive[p], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    if (old == 0u) return;  // first arrival: the sibling's thread finishes this node
+    __threadfence();
+    float4 lo[2], hi[2];
+    float cost[2], cnt[2];
+    for (int c = 0; c < 2; c++) {
+      const uint32_t ch = child[2 * p + c];
+      if (ch & kLeafBit) {
+        const uint32_t f = vals[ch & ~kLeafBit];
+        lo[c] = flo[f];
+        hi[c] = fhi[f];
+        cnt[c] = 1.0f;
+        cost[c] = c_tri * half_area(lo[c], hi[c]);
+      } else {
+        // written by the sibling's thread before its release; read after our acquire
+        lo[c] = node_lo[ch];
+        hi[c] = node_hi[ch];
+        cost[c] = lo[c].w;
+        cnt[c] = hi[c].w;
+      }
+    }
+    float4 L = make_float4(fminf(lo[0].x, lo[1].x), fminf(lo[0].y, lo[1].y), fminf(lo[0].z, lo[1].z), 0.0f);
+    float4 H = make_float4(fmaxf(hi[0].x, hi[1].x), fmaxf(hi[0].y, hi[1].y), fmaxf(hi[0].z, hi[1].z), 0.0f);
+    const float count = cnt[0] + cnt[1];
+    const float area = half_area(L, H);
+    const float split_cost = c_node * area + cost[0] + cost[1];
+    float node_cost = split_cost;
+    uint8_t col = 0;
+    if (count <= static_cast<float>(max_leaf)) {
+      const float leaf_cost = c_tri * area * count;
+      if (leaf_cost <= split_cost) { node_cost = leaf_cost; col = 1; }
+    }
+    L.w = node_cost;
+    H.w = count;
+    node_lo[p] = L;
+    node_hi[p] = H;
+    collapse[p] = col;
+    p = parent_int[p];
+  }
+}
+
+// Outward padding so that the FP32 slab test stays conservative for boxes of zero
+// thickness and for values that are not exactly representable after subtraction.
+__device__ inline float pad_lo(float x) { return x - fabsf(x) * 2.4e-7f - 1e-30f; }
+__device__ inline float pad_hi(float x) { return x + fabsf(x) * 2.4e-7f + 1e-30f; }
+
+// Step 6.
+__global__ __launch_bounds__(kBlock) void k_layout(int n, const uint32_t* __restrict__ vals,
+                                                   const float4* __restrict__ flo, const float4* __restrict__ fhi,
+                                                   const uint32_t* __restrict__ child, const uint32_t* __restrict__ range,
+                                                   const int* __restrict__ parent_int,
+                                                   const float4* __restrict__ node_lo, const float4* __restrict__ node_hi,
+                                                   const uint8_t* __restrict__ collapse, Node2* __restrict__ nodes,
+                                                   uint32_t* stats /* [0]=reachable interior, [1]=max depth */) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n - 1) return;
+  float4 lo[2], hi[2];
+  int ref[2];
+  for (int c = 0; c < 2; c++) {
+    const uint32_t ch = child[2 * i + c];
+    if (ch & kLeafBit) {
+      const uint32_t k = ch & ~kLeafBit;
+      const uint32_t f = vals[k];
+      lo[c] = flo[f];
+      hi[c] = fhi[f];
+      ref[c] = make_leaf_ref(k, 1u);
+    } else {
+      lo[c] = node_lo[ch];
+      hi[c] = node_hi[ch];
+      if (collapse[ch]) {
+        const uint32_t first = range[2 * ch], last = range[2 * ch + 1];
+        ref[c] = make_leaf_ref(first, last - first + 1u);
+      } else {
+        ref[c] = static_cast<int>(ch);
+      }
+    }
+  }
+  Node2 nd;
+  nd.a = make_float4(pad_lo(lo[0].x), pad_hi(hi[0].x), pad_lo(lo[0].y), pad_hi(hi[0].y));
+  nd.b = make_float4(pad_lo(lo[1].x), pad_hi(hi[1].x), pad_lo(lo[1].y), pad_hi(hi[1].y));
+  nd.c = make_float4(pad_lo(lo[0].z), pad_hi(hi[0].z), pad_lo(lo[1].z), pad_hi(hi[1].z));
+  nd.d = make_int4(ref[0], ref[1], 0, 0);
+  nodes[i] = nd;
+  // reachable iff neither this node nor an ancestor became a leaf
+  bool reachable = collapse[i] == 0;
+  uint32_t depth = 0;
+  int p = i;
+  while (reachable && p != 0) {
+    p = parent_int[p];
+    depth++;
+    if (collapse[p]) reachable = false;
+  }
+  if (reachable) {
+    atomicAdd(&stats[0], 1u);
+    atomicMax(&stats[1], depth + 1u);
+  }
+}
+
+// Step 7.
+__global__ __launch_bounds__(kBlock) void k_gather(uint32_t nf, const uint32_t* __restrict__ vals,
+                                                   const float* __restrict__ V, const uint32_t* __restrict__ F,
+                                                   const uint32_t* __restrict__ mesh_off, uint32_t n_meshes,
+                                                   float4* __restrict__ tris, float4* __restrict__ normals) {
+  const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
+  if (k >= nf) return;
+  const uint32_t f = vals[k];
+  // geometry of face f: largest g with mesh_off[g] <= f
+  uint32_t lo = 0, hi = n_meshes;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (mesh_off[mid] <= f) lo = mid; else hi = mid;
+  }
+  const uint32_t g = lo, local = f - mesh_off[lo];
+  const uint32_t i0 = F[3 * f], i1 = F[3 * f + 1], i2 = F[3 * f + 2];  // validated in k_prep
+  const float x0 = V[3 * i0], y0 = V[3 * i0 + 1], z0 = V[3 * i0 + 2];
+  const float x1 = V[3 * i1], y1 = V[3 * i1 + 1], z1 = V[3 * i1 + 2];
+  const float x2 = V[3 * i2], y2 = V[3 * i2 + 1], z2 = V[3 * i2 + 2];
+  tris[3 * static_cast<size_t>(k) + 0] = make_float4(x0, y0, z0, __uint_as_float(local));
+  tris[3 * static_cast<size_t>(k) + 1] = make_float4(x1, y1, z1, __uint_as_float(g));
+  tris[3 * static_cast<size_t>(k) + 2] = make_float4(x2, y2, z2, __uint_as_float(f));
+  const double ax = (double)x1 - (double)x0, ay = (double)y1 - (double)y0, az = (double)z1 - (double)z0;
+  const double bx = (double)x2 - (double)x0, by = (double)y2 - (double)y0, bz = (double)z2 - (double)z0;
+  const double nx = ay * bz - az * by, ny = az * bx - ax * bz, nz = ax * by - ay * bx;
+  const double len = sqrt(nx * nx + ny * ny + nz * nz);
+  float4 nrm = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  if (len > 0.0) nrm = make_float4((float)(nx / len), (float)(ny / len), (float)(nz / len), 0.0f);
+  normals[k] = nrm;
+}
+
+struct DevBuf {
+  void* p = nullptr;
+  ~DevBuf() { if (p) (void)hipFree(p); }
+  template <class T> T* as() { return static_cast<T*>(p); }
+};
+
+inline uint32_t blocks_for(uint64_t n) { return static_cast<uint32_t>((n + kBlock - 1) / kBlock); }
+
+void map_free(rmh_map* m) {
+  if (!m) return;
+  if (m->vertices) (void)hipFree(m->vertices);
+  if (m->faces) (void)hipFree(m->faces);
+  if (m->tris) (void)hipFree(m->tris);
+  if (m->normals) (void)hipFree(m->normals);
+  if (m->nodes) (void)hipFree(m->nodes);
+  delete m;
+}
+
+#define BUILD_TRY(expr)                                             \
+  do {                                                              \
+    hipError_t _e = (expr);                                         \
+    if (_e != hipSuccess) { map_free(m); return hip_fail(_e, #expr); } \
+  } while (0)
+
+}  // namespace
+
+extern "C" int rmh_map_build(const float* vertices, uint32_t nv, const uint32_t* faces, uint32_t nf,
+                             const uint32_t* mesh_face_offsets, uint32_t n_meshes,
+                             const rmh_bvh_params* params, void* stream_, rmh_map** out) {
+  if (!out) return fail(RMH_ERR_INVALID, "rmh_map_build: out is NULL");
+  *out = nullptr;
+  if (nf > 0 && (!vertices || !faces || nv == 0))
+    return fail(RMH_ERR_INVALID, "rmh_map_build: faces given without vertices");
+  if (nf >= (1u << 28)) return fail(RMH_ERR_UNSUPPORTED, "rmh_map_build: at most 2^28 - 1 faces");
+  rmh_bvh_params prm{4u, 1.0f, 1.5f, 0u};
+  if (params) prm = *params;
+  if (prm.max_leaf_size < 1 || prm.max_leaf_size > static_cast<uint32_t>(kMaxLeaf))
+    return fail(RMH_ERR_INVALID, "rmh_map_build: max_leaf_size must be in [1, 8]");
+  std::vector<uint32_t> off;
+  if (mesh_face_offsets && n_meshes > 0) {
+    off.assign(mesh_face_offsets, mesh_face_offsets + n_meshes + 1);
+    if (off.front() != 0 || off.back() != nf)
+      return fail(RMH_ERR_INVALID, "rmh_map_build: mesh_face_offsets must start at 0 and end at n_faces");
+    for (uint32_t g = 0; g < n_meshes; g++)
+      if (off[g] > off[g + 1]) return fail(RMH_ERR_INVALID, "rmh_map_build: mesh_face_offsets not sorted");
+  } else {
+    n_meshes = 1;
+    off = {0u, nf};
+  }
+  hipStream_t stream = static_cast<hipStream_t>(stream_);
+  rmh_map* m = new (std::nothrow) rmh_map();
+  if (!m) return fail(RMH_ERR_ALLOC, "rmh_map_build: out of host memory");
+  m->n_vertices = nv;
+  m->n_faces = nf;
+  m->n_meshes = n_meshes;
+  (void)hipGetDevice(&m->device);
+  m->info.n_vertices = nv;
+  m->info.n_faces = nf;
+  m->info.n_meshes = n_meshes;
+  if (nf == 0) {  // empty map: every ray misses
+    m->root = 0;
+    *out = m;
+    return RMH_OK;
+  }
+
+  hipEvent_t ev0, ev1;
+  BUILD_TRY(hipEventCreate(&ev0));
+  BUILD_TRY(hipEventCreate(&ev1));
+  struct EvGuard { hipEvent_t a, b; ~EvGuard() { (void)hipEventDestroy(a); (void)hipEventDestroy(b); } } evg{ev0, ev1};
+
+  BUILD_TRY(hipMalloc(&m->vertices, sizeof(float) * 3 * nv));
+  BUILD_TRY(hipMalloc(&m->faces, sizeof(uint32_t) * 3 * nf));
+  BUILD_TRY(hipMemcpyAsync(m->vertices, vertices, sizeof(float) * 3 * nv, hipMemcpyDeviceToDevice, stream));
+  BUILD_TRY(hipMemcpyAsync(m->faces, faces, sizeof(uint32_t) * 3 * nf, hipMemcpyDeviceToDevice, stream));
+
+  const int n = static_cast<int>(nf);
+  const uint32_t ni = nf > 1 ? nf - 1 : 0;
+  DevBuf flo, fhi, bounds, err, keys, keys2, vals, vals2, child, range, pint, pleaf, nlo, nhi, arrive, coll,
+      stats, moff, tmp;
+  BUILD_TRY(hipMalloc(&flo.p, sizeof(float4) * nf));
+  BUILD_TRY(hipMalloc(&fhi.p, sizeof(float4) * nf));
+  BUILD_TRY(hipMalloc(&bounds.p, sizeof(float) * 8));
+  BUILD_TRY(hipMalloc(&err.p, sizeof(uint32_t) * 4));
+  BUILD_TRY(hipMalloc(&keys.p, sizeof(uint64_t) * nf));
+  BUILD_TRY(hipMalloc(&keys2.p, sizeof(uint64_t) * nf));
+  BUILD_TRY(hipMalloc(&vals.p, sizeof(uint32_t) * nf));
+  BUILD_TRY(hipMalloc(&vals2.p, sizeof(uint32_t) * nf));
+  BUILD_TRY(hipMalloc(&child.p, sizeof(uint32_t) * 2 * (ni + 1)));
+  BUILD_TRY(hipMalloc(&range.p, sizeof(uint32_t) * 2 * (ni + 1)));
+  BUILD_TRY(hipMalloc(&pint.p, sizeof(int) * (ni + 1)));
+  BUILD_TRY(hipMalloc(&pleaf.p, sizeof(int) * nf));
+  BUILD_TRY(hipMalloc(&nlo.p, sizeof(float4) * (ni + 1)));
+  BUILD_TRY(hipMalloc(&nhi.p, sizeof(float4) * (ni + 1)));
+  BUILD_TRY(hipMalloc(&arrive.p, sizeof(uint32_t) * (ni + 1)));
+  BUILD_TRY(hipMalloc(&coll.p, sizeof(uint8_t) * (ni + 1)));
+  BUILD_TRY(hipMalloc(&stats.p, sizeof(uint32_t) * 4));
+  BUILD_TRY(hipMalloc(&moff.p, sizeof(uint32_t) * (n_meshes + 1)));
+  BUILD_TRY(hipMalloc(&m->tris, sizeof(float4) * 3 * static_cast<size_t>(nf)));
+  BUILD_TRY(hipMalloc(&m->normals, sizeof(float4) * nf));
+  BUILD_TRY(hipMalloc(&m->nodes, sizeof(Node2) * (ni + 1)));
+  BUILD_TRY(hipMemcpy(moff.p, off.data(), sizeof(uint32_t) * (n_meshes + 1), hipMemcpyHostToDevice));
+
+  BUILD_TRY(hipEventRecord(ev0, stream));
+  k_init<<<1, 64, 0, stream>>>(bounds.as<float>(), err.as<uint32_t>());
+  k_prep<<<blocks_for(nf), kBlock, 0, stream>>>(m->vertices, nv, m->faces, nf, flo.as<float4>(), fhi.as<float4>(),
+                                                bounds.as<float>(), err.as<uint32_t>());
+  BUILD_TRY(hipGetLastError());
+  uint32_t herr = 0;
+  BUILD_TRY(hipMemcpyAsync(&herr, err.p, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+  BUILD_TRY(hipStreamSynchronize(stream));
+  if (herr & 1u) { map_free(m); return fail(RMH_ERR_INVALID, "rmh_map_build: face index >= n_vertices"); }
+  if (herr & 2u) { map_free(m); return fail(RMH_ERR_INVALID, "rmh_map_build: non-finite vertex coordinate"); }
+
+  k_morton<<<blocks_for(nf), kBlock, 0, stream>>>(flo.as<float4>(), fhi.as<float4>(), nf, bounds.as<float>(),
+                                                  keys.as<uint64_t>(), vals.as<uint32_t>());
+  BUILD_TRY(hipGetLastError());
+  size_t tmp_bytes = 0;
+  BUILD_TRY(rocprim::radix_sort_pairs(nullptr, tmp_bytes, keys.as<uint64_t>(), keys2.as<uint64_t>(),
+                                      vals.as<uint32_t>(), vals2.as<uint32_t>(), nf, 0u, 63u, stream));
+  BUILD_TRY(hipMalloc(&tmp.p, tmp_bytes));
+  BUILD_TRY(rocprim::radix_sort_pairs(tmp.p, tmp_bytes, keys.as<uint64_t>(), keys2.as<uint64_t>(),
+                                      vals.as<uint32_t>(), vals2.as<uint32_t>(), nf, 0u, 63u, stream));
+  uint64_t* skeys = keys2.as<uint64_t>();
+  uint32_t* svals = vals2.as<uint32_t>();
+
+  BUILD_TRY(hipMemsetAsync(stats.p, 0, sizeof(uint32_t) * 4, stream));
+  if (n >= 2) {
+    BUILD_TRY(hipMemsetAsync(arrive.p, 0, sizeof(uint32_t) * ni, stream));
+    k_karras<<<blocks_for(ni), kBlock, 0, stream>>>(skeys, n, child.as<uint32_t>(), range.as<uint32_t>(),
+                                                    pint.as<int>(), pleaf.as<int>());
+    BUILD_TRY(hipGetLastError());
+    k_bottom_up<<<blocks_for(nf), kBlock, 0, stream>>>(n, svals, flo.as<float4>(), fhi.as<float4>(), child.as<uint32_t>(),
+                                                       pint.as<int>(), pleaf.as<int>(), nlo.as<float4>(), nhi.as<float4>(),
+                                                       arrive.as<uint32_t>(), coll.as<uint8_t>(), prm.max_leaf_size,
+                                                       prm.sah_cost_node, prm.sah_cost_tri);
+    BUILD_TRY(hipGetLastError());
+    k_layout<<<blocks_for(ni), kBlock, 0, stream>>>(n, svals, flo.as<float4>(), fhi.as<float4>(), child.as<uint32_t>(),
+                                                    range.as<uint32_t>(), pint.as<int>(), nlo.as<float4>(),
+                                                    nhi.as<float4>(), coll.as<uint8_t>(), m->nodes, stats.as<uint32_t>());
+    BUILD_TRY(hipGetLastError());
+  }
+  k_gather<<<blocks_for(nf), kBlock, 0, stream>>>(nf, svals, m->vertices, m->faces, moff.as<uint32_t>(), n_meshes,
+                                                  m->tris, m->normals);
+  BUILD_TRY(hipGetLastError());
+  BUILD_TRY(hipEventRecord(ev1, stream));
+  BUILD_TRY(hipStreamSynchronize(stream));
+  float ms = 0.0f;
+  BUILD_TRY(hipEventElapsedTime(&ms, ev0, ev1));
+
+  uint32_t hstats[4] = {0, 0, 0, 0};
+  uint8_t root_collapsed = 0;
+  float4 root_lo = make_float4(0, 0, 0, 0), root_hi = make_float4(0, 0, 0, 0);
+  if (n >= 2) {
+    BUILD_TRY(hipMemcpy(hstats, stats.p, sizeof(hstats), hipMemcpyDeviceToHost));
+    BUILD_TRY(hipMemcpy(&root_collapsed, coll.p, 1, hipMemcpyDeviceToHost));
+    BUILD_TRY(hipMemcpy(&root_lo, nlo.p, sizeof(float4), hipMemcpyDeviceToHost));
+    BUILD_TRY(hipMemcpy(&root_hi, nhi.p, sizeof(float4), hipMemcpyDeviceToHost));
+  }
+  m->n_internal = ni;
+  if (n == 1 || root_collapsed) {
+    m->root = make_leaf_ref(0u, nf);
+    m->info.n_nodes = 0;
+    m->info.n_leaves = 1;
+    m->info.max_depth = 0;
+  } else {
+    m->root = 0;
+    m->info.n_nodes = hstats[0];
+    m->info.n_leaves = hstats[0] + 1;
+    m->info.max_depth = hstats[1];
+  }
+  {
+    const float dx = root_hi.x - root_lo.x, dy = root_hi.y - root_lo.y, dz = root_hi.z - root_lo.z;
+    const float area = dx * dy + dy * dz + dz * dx;
+    m->info.sah_cost = (n >= 2 && area > 0.0f) ? root_lo.w / area : 0.0f;
+  }
+  m->info.build_ms = ms;
+  m->info.device_bytes = sizeof(float4) * 4 * static_cast<uint64_t>(nf) + sizeof(Node2) * static_cast<uint64_t>(ni + 1);
+  *out = m;
+  return RMH_OK;
+}
+
+extern "C" int rmh_map_destroy(rmh_map* map) {
+  map_free(map);
+  return RMH_OK;
+}
+
+extern "C" int rmh_map_get_info(const rmh_map* map, rmh_map_info* info) {
+  if (!map || !info) return fail(RMH_ERR_INVALID, "rmh_map_get_info: NULL argument");
+  *info = map->info;
+  return RMH_OK;
+}

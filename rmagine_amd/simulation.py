@@ -1,0 +1,245 @@
+"""Python mirror of Rmagine's GPU Simulator API on top of librmh.
+
+Mirrors (uos/rmagine v2.4.2):
+  SimulatorOptix: setMap / setTsb              src/rmagine_optix/include/rmagine/simulation/SimulatorOptix.hpp:22-79
+  SphereSimulatorOptix: setModel / simulate     src/rmagine_optix/include/rmagine/simulation/SphereSimulatorOptix.hpp:122-191
+  O1DnSimulatorOptix::setModel (uploads dirs)   src/rmagine_optix/src/simulation/O1DnSimulatorOptix.cpp:41-83
+  OptixMap / import_optix_map                   src/rmagine_optix/include/rmagine/map/OptixMap.hpp:88-110
+  Bundle + resize_memory_bundle                 src/rmagine_core/include/rmagine/simulation/SimulationResults.hpp:176-216
+
+Device buffers are torch tensors on a HIP device (torch is used for device memory and
+streams only; every computation runs in librmh).  Transforms are float32 tensors of
+shape (N, 8).  Id attributes are int32 tensors holding the uint32 bit patterns
+(0xFFFFFFFF reads as -1).
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+from . import types as T
+
+IntAttrAll = T.ATTRIBUTES
+
+_TORCH_DTYPES = {
+    "hits": torch.uint8,
+    "ranges": torch.float32,
+    "points": torch.float32,
+    "normals": torch.float32,
+    "face_ids": torch.int32,
+    "geom_ids": torch.int32,
+    "object_ids": torch.int32,
+}
+
+
+def _ptr(t) -> ctypes.c_void_p:
+    return ctypes.c_void_p(0 if t is None else int(t.data_ptr()))
+
+
+def stream_handle(stream=None) -> ctypes.c_void_p:
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return ctypes.c_void_p(int(s.cuda_stream))
+
+
+def _device_tensor(x, dtype, device) -> torch.Tensor:
+    if isinstance(x, torch.Tensor):
+        return x.to(device=device, dtype=dtype).contiguous()
+    return torch.as_tensor(np.ascontiguousarray(x), dtype=dtype, device=device)
+
+
+class Bundle:
+    """Result buffers; an attribute is produced iff its tensor has size > 0."""
+
+    def __init__(self, **attrs):
+        for name in T.ATTRIBUTES:
+            setattr(self, name, attrs.get(name))
+
+    def enabled(self):
+        return [n for n in T.ATTRIBUTES if getattr(self, n) is not None and getattr(self, n).numel() > 0]
+
+    def to_numpy(self) -> dict:
+        out = {}
+        for n in self.enabled():
+            a = getattr(self, n).cpu().numpy()
+            if T.ATTR_DTYPES[n][0] == np.uint32:
+                a = a.view(np.uint32)
+            out[n] = a
+        return out
+
+
+def resize_memory_bundle(bundle: Bundle, W: int, H: int, N: int = 1, attrs=IntAttrAll, device="cuda") -> Bundle:
+    """Allocates W*H*N entries for each requested attribute (SimulationResults.hpp:176-216)."""
+    n = int(W) * int(H) * int(N)
+    for name in attrs:
+        comp = T.ATTR_DTYPES[name][1]
+        shape = (n,) if comp == 1 else (n, comp)
+        setattr(bundle, name, torch.empty(shape, dtype=_TORCH_DTYPES[name], device=device))
+    return bundle
+
+
+class HipMap:
+    """Mesh plus BVH resident in HBM; the role of OptixMap/EmbreeMap.
+
+    vertices: (V, 3) float32, faces: (F, 3) uint32 indices.  mesh_face_offsets (len
+    n_meshes+1) splits the faces into separate geometries (geom ids 0..n_meshes-1).
+    """
+
+    def __init__(self, vertices, faces, mesh_face_offsets=None, max_leaf_size: int = 4,
+                 sah_cost_node: float = 1.0, sah_cost_tri: float = 1.5, device="cuda", stream=None):
+        V = _device_tensor(np.asarray(vertices, dtype=np.float32).reshape(-1, 3), torch.float32, device)
+        F = _device_tensor(np.asarray(faces, dtype=np.uint32).reshape(-1, 3).view(np.int32), torch.int32, device)
+        self.device = torch.device(device)
+        self._handle = ctypes.c_void_p()
+        off_arr, n_meshes = None, 0
+        if mesh_face_offsets is not None:
+            off = np.ascontiguousarray(np.asarray(mesh_face_offsets, dtype=np.uint32))
+            n_meshes = len(off) - 1
+            off_arr = (ctypes.c_uint32 * len(off))(*off.tolist())
+        prm = T.rmh_bvh_params(max_leaf_size, sah_cost_node, sah_cost_tri, 0)
+        _lib.call("rmh_map_build", _ptr(V), V.shape[0], _ptr(F), F.shape[0], off_arr, n_meshes,
+                  ctypes.byref(prm), stream_handle(stream), ctypes.byref(self._handle))
+        self.n_vertices = V.shape[0]
+        self.n_faces = F.shape[0]
+
+    @property
+    def handle(self) -> ctypes.c_void_p:
+        return self._handle
+
+    def info(self) -> dict:
+        inf = T.rmh_map_info()
+        _lib.call("rmh_map_get_info", self._handle, ctypes.byref(inf))
+        return {k: getattr(inf, k) for k, _ in inf._fields_}
+
+    def __del__(self):
+        h = getattr(self, "_handle", None)
+        if h is not None and h.value:
+            try:
+                _lib.lib().rmh_map_destroy(h)
+            except Exception:
+                pass
+            self._handle = ctypes.c_void_p()
+
+
+class _SensorHandle:
+    def __init__(self, model, device):
+        self.model = model
+        keep = []
+        dirs_dev = origs_dev = 0
+        if isinstance(model, (T.O1DnModel, T.OnDnModel)):
+            d = _device_tensor(np.asarray(model.dirs, dtype=np.float32).reshape(-1, 3), torch.float32, device)
+            keep.append(d)
+            dirs_dev = int(d.data_ptr())
+            if isinstance(model, T.OnDnModel):
+                o = _device_tensor(np.asarray(model.origs, dtype=np.float32).reshape(-1, 3), torch.float32, device)
+                keep.append(o)
+                origs_dev = int(o.data_ptr())
+            if d.shape[0] != model.width * model.height:
+                raise ValueError("dirs must hold width*height directions")
+        kind, cmodel = T.model_to_c(model, dirs_dev, origs_dev)
+        self.handle = ctypes.c_void_p()
+        _lib.call("rmh_sensor_create", kind, ctypes.byref(cmodel), ctypes.byref(self.handle))
+        self.W, self.H = model.getWidth(), model.getHeight()
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h is not None and h.value:
+            try:
+                _lib.lib().rmh_sensor_destroy(h)
+            except Exception:
+                pass
+            self.handle = ctypes.c_void_p()
+
+
+class SimulatorHip:
+    """Common part of the *SimulatorHip classes (SimulatorOptix.hpp:22-79)."""
+
+    _model_type = None
+
+    def __init__(self, map: HipMap | None = None, device="cuda"):
+        self.device = torch.device(device)
+        self._map = map
+        self._sensor = None
+        self._Tsb = T.identity_transforms(1)[0]
+
+    def setMap(self, map: HipMap) -> None:
+        self._map = map
+
+    def map(self):
+        return self._map
+
+    def setTsb(self, Tsb) -> None:
+        self._Tsb = np.asarray(Tsb, dtype=np.float32).reshape(8).copy()
+
+    def setModel(self, model) -> None:
+        if self._model_type is not None and not isinstance(model, self._model_type):
+            raise TypeError(f"{type(self).__name__} expects {self._model_type.__name__}")
+        self._sensor = _SensorHandle(model, self.device)
+
+    @property
+    def model(self):
+        return None if self._sensor is None else self._sensor.model
+
+    def simulate(self, Tbm, bundle: Bundle | None = None, attrs=IntAttrAll, stream=None) -> Bundle:
+        """Casts every ray of the model from every pose in Tbm ((N,8) float32 tensor).
+
+        If bundle is None a new one is allocated with `attrs` (resize_memory_bundle);
+        otherwise its non-empty tensors receive the results.  Asynchronous on `stream`.
+        """
+        if self._map is None:
+            raise _lib.NoMapError(3, "simulate: no map set")  # SphereSimulatorOptix.tcc:49-53
+        if self._sensor is None:
+            raise RuntimeError("simulate: no sensor model set")
+        if not isinstance(Tbm, torch.Tensor):
+            Tbm = torch.as_tensor(np.asarray(Tbm, dtype=np.float32).reshape(-1, 8), device=self.device)
+        if Tbm.dtype != torch.float32 or Tbm.dim() != 2 or Tbm.shape[1] != 8 or not Tbm.is_contiguous():
+            raise ValueError("Tbm must be a contiguous float32 tensor of shape (N, 8)")
+        if Tbm.device.type != "cuda":
+            raise ValueError("Tbm must be a device tensor")
+        N = Tbm.shape[0]
+        W, H = self._sensor.W, self._sensor.H
+        if bundle is None:
+            bundle = resize_memory_bundle(Bundle(), W, H, N, attrs, device=self.device)
+        need = W * H * N
+        outs = T.rmh_outputs()
+        for name in T.ATTRIBUTES:
+            t = getattr(bundle, name)
+            if t is None or t.numel() == 0:
+                continue
+            comp = T.ATTR_DTYPES[name][1]
+            if t.numel() != need * comp or not t.is_contiguous() or t.device.type != "cuda":
+                raise ValueError(f"bundle.{name} must be a contiguous device tensor of {need}x{comp} elements")
+            if t.dtype != _TORCH_DTYPES[name]:
+                raise ValueError(f"bundle.{name} must be {_TORCH_DTYPES[name]}")
+            setattr(outs, name, int(t.data_ptr()))
+        Tsb = T.transform_to_c(self._Tsb)
+        _lib.call("rmh_simulate", self._map.handle, self._sensor.handle, ctypes.byref(Tsb), _ptr(Tbm), N,
+                  ctypes.byref(outs), stream_handle(stream))
+        return bundle
+
+
+class SphereSimulatorHip(SimulatorHip):
+    _model_type = T.SphericalModel
+
+
+class PinholeSimulatorHip(SimulatorHip):
+    _model_type = T.PinholeModel
+
+
+class O1DnSimulatorHip(SimulatorHip):
+    _model_type = T.O1DnModel
+
+
+class OnDnSimulatorHip(SimulatorHip):
+    _model_type = T.OnDnModel
+
+
+def simulator_for(model, map: HipMap | None = None, device="cuda") -> SimulatorHip:
+    """make_simulator analogue (src/rmagine_core/include/rmagine/simulation/Simulator.hpp:52-78)."""
+    cls = {T.SphericalModel: SphereSimulatorHip, T.PinholeModel: PinholeSimulatorHip,
+           T.O1DnModel: O1DnSimulatorHip, T.OnDnModel: OnDnSimulatorHip}[type(model)]
+    sim = cls(map, device=device)
+    sim.setModel(model)
+    return sim

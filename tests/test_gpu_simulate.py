@@ -1,0 +1,206 @@
+"""GPU parity of librmh's simulate() against the oracle (marked gpu).
+
+Bar: hits, face/geom/object ids and ranges bit-exact (the kernel and the oracle evaluate
+the same FP64 watertight predicate from the same FP32 rays); points and normals within
+1e-5 (they are FP32 products of the same inputs and in practice bit-exact too).  Rays the
+oracle flags as ambiguous for an FP32 intersector are counted and reported; they must
+still match here.
+"""
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from rmagine_amd import scenes
+from rmagine_amd import types as T
+from rmagine_amd.simulation import (Bundle, HipMap, O1DnSimulatorHip, OnDnSimulatorHip, PinholeSimulatorHip,
+                                    SphereSimulatorHip, resize_memory_bundle, simulator_for)
+from rmagine_amd._lib import NoMapError
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5  # m for ranges/points, rad-equivalent for unit normals
+
+
+def run_gpu(map_, model, Tbm, attrs=T.ATTRIBUTES, Tsb=None):
+    sim = simulator_for(model, map_)
+    if Tsb is not None:
+        sim.setTsb(Tsb)
+    res = sim.simulate(torch.as_tensor(Tbm, device="cuda"), attrs=attrs)
+    torch.cuda.synchronize()
+    return res.to_numpy()
+
+
+def compare(g, o, label=""):
+    flags = o["flags"]
+    report = {"rays": len(flags), "ambiguous": int((flags != 0).sum())}
+    for k in ("hits", "face_ids", "geom_ids", "object_ids"):
+        if k in g:
+            bad = g[k] != o[k]
+            report[k + "_mismatch"] = int(bad.sum())
+            assert not bad.any(), f"{label}: {k} differs on {bad.sum()} rays ({(bad & (flags != 0)).sum()} flagged)"
+    if "ranges" in g:
+        d = np.abs(g["ranges"].astype(np.float64) - o["ranges"].astype(np.float64))
+        report["range_max_err"] = float(d.max()) if len(d) else 0.0
+        assert report["range_max_err"] <= TOL, f"{label}: range error {report['range_max_err']}"
+    for k in ("points", "normals"):
+        if k in g:
+            np.testing.assert_array_equal(np.isnan(g[k]), np.isnan(o[k]), err_msg=f"{label}: {k} NaN pattern")
+            ok = ~np.isnan(o[k])
+            d = np.abs(g[k][ok].astype(np.float64) - o[k][ok]) if ok.any() else np.zeros(1)
+            report[k + "_max_err"] = float(d.max())
+            assert report[k + "_max_err"] <= TOL, f"{label}: {k} error {report[k + '_max_err']}"
+    print(label, report)
+    return report
+
+
+@pytest.fixture(scope="module")
+def cube():
+    V, F = scenes.genCube()
+    return V, F, HipMap(V, F), oracle.OracleMap(V, F)
+
+
+def test_kat_spherical(cube):
+    _, _, hm, om = cube
+    m = T.example_spherical()
+    Tbm = T.identity_transforms(100)
+    g = run_gpu(hm, m, Tbm)
+    r = g["ranges"][99 * m.size():][m.getBufferId(m.phi.size // 2, 0)]
+    assert abs(r - 0.500076) < 1e-4  # tests/embree/simulation_spherical.cpp:57-66
+    compare(g, oracle.simulate(om, m, Tbm), "cube/spherical")
+
+
+def test_kat_pinhole_o1dn_ondn(cube):
+    _, _, hm, om = cube
+    Tbm = T.identity_transforms(100)
+    for m in (T.example_pinhole(), T.example_o1dn(), T.example_ondn()):
+        g = run_gpu(hm, m, Tbm)
+        compare(g, oracle.simulate(om, m, Tbm), f"cube/{type(m).__name__}")
+    g = run_gpu(hm, T.example_pinhole(), Tbm)
+    assert np.abs(g["points"][0] - [0.5, 0.5, 0.375]).sum() < 1e-4   # simulation_pinhole.cpp:61-73
+    g = run_gpu(hm, T.example_o1dn(), Tbm)
+    assert np.abs(g["points"][0] - [0.256732, 0.5, -0.1]).sum() < 1e-4  # simulation_o1dn.cpp:59-72
+    g = run_gpu(hm, T.example_ondn(), Tbm)
+    assert abs(g["ranges"][0] - 0.5) < 1e-4  # simulation_ondn.cpp:63-70
+
+
+@pytest.fixture(scope="module")
+def room():
+    V, F, Tbm = scenes.room_workload(n_sensors=100)
+    return V, F, Tbm, HipMap(V, F), oracle.OracleMap(V, F)
+
+
+def test_room_vlp16_all_attributes(room):
+    """Room with 200 boxes (2,412 tris), 100 VLP-16 poses: 1.44M rays, every attribute."""
+    V, F, Tbm, hm, om = room
+    m = scenes.vlp16_workload_model()
+    g = run_gpu(hm, m, Tbm)
+    o = oracle.simulate(om, m, Tbm)
+    rep = compare(g, o, "room/vlp16")
+    assert rep["rays"] == 1_440_000
+    assert g["hits"].mean() > 0.5
+
+
+def test_room_other_models_and_tsb(room):
+    V, F, Tbm, hm, om = room
+    rng = np.random.default_rng(3)
+    dirs = rng.normal(size=(3000, 3))
+    dirs = (dirs / np.linalg.norm(dirs, axis=1, keepdims=True)).astype(np.float32)
+    origs = rng.uniform(-0.2, 0.2, size=(3000, 3)).astype(np.float32)
+    models = [
+        T.PinholeModel(160, 120, T.Interval(0.05, 80.0), (80.0, 80.0), (79.5, 59.5)),
+        T.O1DnModel(3000, 1, T.Interval(0.1, 60.0), (0.05, -0.02, 0.1), dirs),
+        T.OnDnModel(1000, 3, T.Interval(0.1, 60.0), origs, dirs),
+    ]
+    Tsb = np.concatenate([T.quat_from_euler(0.02, -0.05, 0.3), [0.1, 0.0, 0.2, 0.0]]).astype(np.float32)
+    for m in models:
+        g = run_gpu(hm, m, Tbm[:20], Tsb=Tsb)
+        o = oracle.simulate(om, m, Tbm[:20], Tsb=Tsb)
+        compare(g, o, f"room/{type(m).__name__}")
+
+
+def test_attribute_subsets(room):
+    V, F, Tbm, hm, om = room
+    m = scenes.vlp16_workload_model()
+    o = oracle.simulate(om, m, Tbm[:5])
+    for attrs in (("ranges",), ("hits", "ranges"), ("normals", "face_ids"), ("points",)):
+        g = run_gpu(hm, m, Tbm[:5], attrs=attrs)
+        assert set(g) == set(attrs)
+        compare(g, o, f"subset {attrs}")
+
+
+def test_map_info_and_leaf_sizes(room):
+    V, F, Tbm, _, om = room
+    m = scenes.vlp16_workload_model()
+    o = oracle.simulate(om, m, Tbm[:4])
+    for leaf in (1, 2, 8):
+        hm = HipMap(V, F, max_leaf_size=leaf)
+        info = hm.info()
+        assert info["n_faces"] == len(F) and 0 < info["max_depth"] <= 64
+        compare(run_gpu(hm, m, Tbm[:4]), o, f"leaf{leaf}")
+
+
+def test_terrain_small_parity():
+    """Reduced terrain (128^2 grid, 32k tris), Ouster-style 128x1024 sensors, all attributes."""
+    V, F, Tbm = scenes.terrain_workload(n_grid=128, n_sensors=8)
+    hm = HipMap(V, F)
+    om = oracle.OracleMap(V, F)
+    m = scenes.ouster_workload_model()
+    compare(run_gpu(hm, m, Tbm), oracle.simulate(om, m, Tbm), "terrain128")
+
+
+def test_multi_mesh_geom_ids():
+    V1, F1 = scenes.genCube()
+    V2, F2 = scenes.genSphere(20, 40)
+    V2 = V2 * np.float32(0.3)
+    V = np.concatenate([V1, V2])
+    F = np.concatenate([F1, F2 + len(V1)])
+    off = [0, len(F1), len(F1) + len(F2)]
+    hm, om = HipMap(V, F, off), oracle.OracleMap(V, F, off)
+    m = T.example_spherical()
+    Tbm = T.make_transforms([[0.35, 0.1, 0.0], [-0.4, -0.2, 0.1]], [[0, 0, 0, 1], [0, 0, 0.2, 0.98]])
+    g = run_gpu(hm, m, Tbm)
+    compare(g, oracle.simulate(om, m, Tbm), "multimesh")
+    assert set(np.unique(g["geom_ids"])) == {0, 1}
+
+
+def test_empty_map_and_degenerate():
+    # a map without faces: every ray misses (the reference returns early, leaving outputs unwritten)
+    hm = HipMap(np.zeros((0, 3), np.float32), np.zeros((0, 3), np.uint32))
+    g = run_gpu(hm, T.example_spherical(), T.identity_transforms(2))
+    assert (g["hits"] == 0).all() and (g["ranges"] == np.float32(101.0)).all() and (g["face_ids"] == T.UINT_MAX).all()
+    # single triangle, and a degenerate (zero area) triangle next to it
+    V = np.array([[1, -1, -1], [1, 1, -1], [1, 0, 1], [2, 0, 0], [2, 0, 0], [2, 1, 0]], np.float32)
+    F = np.array([[0, 1, 2], [3, 4, 5]], np.uint32)
+    hm1, om1 = HipMap(V[:3], F[:1]), oracle.OracleMap(V[:3], F[:1])
+    hm2, om2 = HipMap(V, F), oracle.OracleMap(V, F)
+    m = T.example_spherical()
+    Tbm = T.identity_transforms(3)
+    compare(run_gpu(hm1, m, Tbm), oracle.simulate(om1, m, Tbm), "single tri")
+    compare(run_gpu(hm2, m, Tbm), oracle.simulate(om2, m, Tbm), "degenerate tri")
+
+
+def test_zero_poses_and_errors(cube):
+    _, _, hm, _ = cube
+    sim = SphereSimulatorHip(hm)
+    sim.setModel(T.example_spherical())
+    res = sim.simulate(torch.zeros((0, 8), dtype=torch.float32, device="cuda"))
+    assert res.ranges.numel() == 0
+    nomap = SphereSimulatorHip()
+    nomap.setModel(T.example_spherical())
+    with pytest.raises(NoMapError):
+        nomap.simulate(torch.as_tensor(T.identity_transforms(1), device="cuda"))
+    with pytest.raises(ValueError):
+        b = resize_memory_bundle(Bundle(), 10, 10, 1, ("ranges",))
+        sim.simulate(torch.as_tensor(T.identity_transforms(1), device="cuda"), b)
+    with pytest.raises(Exception):
+        HipMap(np.zeros((3, 3), np.float32), np.array([[0, 1, 7]], np.uint32))  # index out of range
+
+
+def test_repeat_is_deterministic(room):
+    V, F, Tbm, hm, _ = room
+    m = scenes.vlp16_workload_model()
+    a = run_gpu(hm, m, Tbm[:10])
+    b = run_gpu(hm, m, Tbm[:10])
+    for k in a:
+        np.testing.assert_array_equal(a[k].view(np.uint8), b[k].view(np.uint8))
