@@ -40,6 +40,8 @@ def parse():
     ap.add_argument("--sensors", type=int, default=1000)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain 1 GiB device copy 5 times (scale factor for HBM counters)")
     return ap.parse_args()
 
 
@@ -85,6 +87,16 @@ def main():
                           "n_gpus": args.gpus}))
         sys.exit(2)
     import torch
+
+    if args.copy_calibration:
+        n = 1 << 30
+        src = torch.ones(n, dtype=torch.uint8, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes": n, "launches": 5}))
+        return
 
     from rmagine_amd import scenes
     from rmagine_amd.simulation import HipMap, SphereSimulatorHip

@@ -1,0 +1,218 @@
+// The *SimulatorHip classes: Rmagine's GPU Simulator API on librmh.
+//
+// API shape of SimulatorOptix / SphereSimulatorOptix / PinholeSimulatorOptix /
+// O1DnSimulatorOptix / OnDnSimulatorOptix
+// (src/rmagine_optix/include/rmagine/simulation/SimulatorOptix.hpp:22-79,
+//  SphereSimulatorOptix.hpp:122-191, SphereSimulatorOptix.tcc:11-100,
+//  O1DnSimulatorOptix.cpp:41-102): setMap / setTsb / setModel / simulate<BundleT>(Tbm[, res])
+// with poses and results in device memory (VRAM_HIP), asynchronous on the map's stream,
+// and the results of the Embree simulators
+// (src/rmagine_embree/include/rmagine/simulation/SphereSimulatorEmbree.tcc:50-254).
+// An attribute is produced iff its buffer in the bundle has size > 0
+// (embree_common.h:79-139, optix/common.h:155-232).
+#pragma once
+
+#include <rmh.h>
+
+#include <memory>
+#include <stdexcept>
+
+#include <rmagine/map/HipMap.hpp>
+#include <rmagine/math/types.h>
+#include <rmagine/simulation/SimulationResults.hpp>
+#include <rmagine/types/MemoryHip.hpp>
+#include <rmagine/types/sensor_models.h>
+
+namespace rmagine {
+
+namespace detail {
+
+template <typename BundleT>
+rmh_outputs outputs_of(BundleT& res, size_t n) {
+  rmh_outputs o{};
+  auto take = [n](auto& mem) -> void* {
+    if (mem.size() == 0) return nullptr;
+    if (mem.size() != n) throw std::runtime_error("[SimulatorHip] result buffer size does not match W*H*N");
+    return mem.raw();
+  };
+  if constexpr (BundleT::template has<Hits<VRAM_HIP>>()) o.hits = static_cast<uint8_t*>(take(res.Hits<VRAM_HIP>::hits));
+  if constexpr (BundleT::template has<Ranges<VRAM_HIP>>()) o.ranges = static_cast<float*>(take(res.Ranges<VRAM_HIP>::ranges));
+  if constexpr (BundleT::template has<Points<VRAM_HIP>>()) o.points = static_cast<rmh_vec3*>(take(res.Points<VRAM_HIP>::points));
+  if constexpr (BundleT::template has<Normals<VRAM_HIP>>()) o.normals = static_cast<rmh_vec3*>(take(res.Normals<VRAM_HIP>::normals));
+  if constexpr (BundleT::template has<FaceIds<VRAM_HIP>>()) o.face_ids = static_cast<uint32_t*>(take(res.FaceIds<VRAM_HIP>::face_ids));
+  if constexpr (BundleT::template has<GeomIds<VRAM_HIP>>()) o.geom_ids = static_cast<uint32_t*>(take(res.GeomIds<VRAM_HIP>::geom_ids));
+  if constexpr (BundleT::template has<ObjectIds<VRAM_HIP>>()) o.object_ids = static_cast<uint32_t*>(take(res.ObjectIds<VRAM_HIP>::object_ids));
+  return o;
+}
+
+struct SensorDeleter {
+  void operator()(rmh_sensor* s) const { (void)rmh_sensor_destroy(s); }
+};
+
+inline rmh_transform to_rmh(const Transform& T) {
+  rmh_transform r;
+  r.R = {T.R.x, T.R.y, T.R.z, T.R.w};
+  r.t = {T.t.x, T.t.y, T.t.z};
+  r.stamp = T.stamp;
+  return r;
+}
+
+}  // namespace detail
+
+class SimulatorHip {
+ public:
+  SimulatorHip() { m_Tsb.setIdentity(); }
+  explicit SimulatorHip(HipMapPtr map) : m_map(std::move(map)) { m_Tsb.setIdentity(); }
+  virtual ~SimulatorHip() = default;
+
+  void setMap(HipMapPtr map) { m_map = std::move(map); }
+  HipMapPtr map() const { return m_map; }
+
+  void setTsb(const MemoryView<Transform, RAM>& Tsb) { m_Tsb = Tsb[0]; }
+  void setTsb(const Transform& Tsb) { m_Tsb = Tsb; }
+
+  uint32_t getWidth() const { return m_width; }
+  uint32_t getHeight() const { return m_height; }
+
+  // SimulatorOptix::simulate<BundleT>(Tbm, res)
+  template <typename BundleT>
+  void simulate(const MemoryView<Transform, VRAM_HIP>& Tbm, BundleT& res) const {
+    if (!m_map) throw std::runtime_error("[SimulatorHip] simulate(): no map set");
+    if (!m_sensor) throw std::runtime_error("[SimulatorHip] simulate(): no sensor model set");
+    const size_t n = static_cast<size_t>(m_width) * m_height * Tbm.size();
+    const rmh_outputs out = detail::outputs_of(res, n);
+    const rmh_transform Tsb = detail::to_rmh(m_Tsb);
+    RM_RMH_CHECK(rmh_simulate(m_map->handle(), m_sensor.get(), &Tsb, reinterpret_cast<const rmh_transform*>(Tbm.raw()),
+                              static_cast<uint32_t>(Tbm.size()), &out, m_map->stream()));
+  }
+
+  template <typename BundleT>
+  BundleT simulate(const MemoryView<Transform, VRAM_HIP>& Tbm) const {
+    BundleT res;
+    resize_memory_bundle<VRAM_HIP>(res, m_width, m_height, static_cast<unsigned int>(Tbm.size()));
+    simulate(Tbm, res);
+    return res;
+  }
+
+  // single pose from the host (SimulatorOptix.tcc:6-24 uploads it)
+  template <typename BundleT>
+  void simulate(const Transform& Tbm, BundleT& res) const {
+    Memory<Transform, VRAM_HIP> T(1);
+    T = make_view(const_cast<Transform&>(Tbm));
+    simulate(T, res);
+  }
+
+  template <typename BundleT>
+  BundleT simulate(const Transform& Tbm) const {
+    BundleT res;
+    resize_memory_bundle<VRAM_HIP>(res, m_width, m_height, 1);
+    simulate(Tbm, res);
+    return res;
+  }
+
+  // OptiX builds a pipeline per attribute set here; the HIP kernels are specialised at
+  // build time, so this only checks that a map and a model are set.
+  template <typename BundleT>
+  void preBuildProgram() {
+    if (!m_sensor) throw std::runtime_error("[SimulatorHip] preBuildProgram(): no sensor model set");
+  }
+
+  // wait for all launches of this simulator's map stream
+  void synchronize() const {
+    if (m_map) RM_RMH_CHECK(rmh_stream_synchronize(m_map->stream()));
+  }
+
+ protected:
+  void setSensor(int kind, const void* model) {
+    rmh_sensor* s = nullptr;
+    RM_RMH_CHECK(rmh_sensor_create(kind, model, &s));
+    m_sensor.reset(s, detail::SensorDeleter());
+    RM_RMH_CHECK(rmh_sensor_dims(s, &m_width, &m_height));
+  }
+
+  HipMapPtr m_map;
+  Transform m_Tsb;
+  std::shared_ptr<rmh_sensor> m_sensor;
+  uint32_t m_width = 0, m_height = 0;
+};
+
+class SphereSimulatorHip : public SimulatorHip {
+ public:
+  using SimulatorHip::SimulatorHip;
+  void setModel(const SphericalModel& model) {
+    m_model = model;
+    setSensor(RMH_MODEL_SPHERICAL, &model);
+  }
+  void setModel(const MemoryView<SphericalModel, RAM>& model) { setModel(model[0]); }
+  const SphericalModel& model() const { return m_model; }
+
+ private:
+  SphericalModel m_model{};
+};
+
+class PinholeSimulatorHip : public SimulatorHip {
+ public:
+  using SimulatorHip::SimulatorHip;
+  void setModel(const PinholeModel& model) {
+    m_model = model;
+    setSensor(RMH_MODEL_PINHOLE, &model);
+  }
+  void setModel(const MemoryView<PinholeModel, RAM>& model) { setModel(model[0]); }
+  const PinholeModel& model() const { return m_model; }
+
+ private:
+  PinholeModel m_model{};
+};
+
+class O1DnSimulatorHip : public SimulatorHip {
+ public:
+  using SimulatorHip::SimulatorHip;
+  // uploads the directions, like O1DnSimulatorOptix::setModel (O1DnSimulatorOptix.cpp:41-83)
+  void setModel(const O1DnModel_<RAM>& model) {
+    Memory<Vector, VRAM_HIP> dirs(model.dirs);
+    setDev(model, dirs);
+  }
+  void setModel(const O1DnModel_<VRAM_HIP>& model) { setDev(model, model.dirs); }
+
+ private:
+  template <typename M>
+  void setDev(const M& model, const MemoryView<Vector, VRAM_HIP>& dirs) {
+    rmh_o1dn_model m{};
+    m.width = model.width;
+    m.height = model.height;
+    m.range = {model.range.min, model.range.max};
+    m.orig = {model.orig.x, model.orig.y, model.orig.z};
+    m.dirs_dev = reinterpret_cast<const rmh_vec3*>(dirs.raw());
+    setSensor(RMH_MODEL_O1DN, &m);
+  }
+};
+
+class OnDnSimulatorHip : public SimulatorHip {
+ public:
+  using SimulatorHip::SimulatorHip;
+  void setModel(const OnDnModel_<RAM>& model) {
+    Memory<Vector, VRAM_HIP> dirs(model.dirs), origs(model.origs);
+    setDev(model, origs, dirs);
+  }
+  void setModel(const OnDnModel_<VRAM_HIP>& model) { setDev(model, model.origs, model.dirs); }
+
+ private:
+  template <typename M>
+  void setDev(const M& model, const MemoryView<Vector, VRAM_HIP>& origs, const MemoryView<Vector, VRAM_HIP>& dirs) {
+    rmh_ondn_model m{};
+    m.width = model.width;
+    m.height = model.height;
+    m.range = {model.range.min, model.range.max};
+    m.origs_dev = reinterpret_cast<const rmh_vec3*>(origs.raw());
+    m.dirs_dev = reinterpret_cast<const rmh_vec3*>(dirs.raw());
+    setSensor(RMH_MODEL_ONDN, &m);
+  }
+};
+
+using SimulatorHipPtr = std::shared_ptr<SimulatorHip>;
+using SphereSimulatorHipPtr = std::shared_ptr<SphereSimulatorHip>;
+using PinholeSimulatorHipPtr = std::shared_ptr<PinholeSimulatorHip>;
+using O1DnSimulatorHipPtr = std::shared_ptr<O1DnSimulatorHip>;
+using OnDnSimulatorHipPtr = std::shared_ptr<OnDnSimulatorHip>;
+
+}  // namespace rmagine

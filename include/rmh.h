@@ -158,6 +158,81 @@ int rmh_simulate(const rmh_map* map, const rmh_sensor* sensor,
                  const rmh_transform* Tsb, const rmh_transform* Tbm_dev, uint32_t n_poses,
                  const rmh_outputs* out, void* stream);
 
+/* ---- math layer (rmagine_cuda's math, src/rmagine_cuda/src/math/) --------------------- */
+
+/* C[i] = A[nA == 1 ? 0 : i] * B[nB == 1 ? 0 : i], i < n  (multNxN / multNx1 / mult1xN of
+ * Transforms, src/rmagine_cuda/src/math/memory_math.cu:16-73,471-1024).  Device arrays. */
+int rmh_transform_mul(const rmh_transform* A_dev, uint32_t nA, const rmh_transform* B_dev, uint32_t nB,
+                      rmh_transform* C_dev, uint32_t n, void* stream);
+/* C[i] = A[i]^-1  (invert, memory_math.cu:121-160, Transform.tcc:24-31) */
+int rmh_transform_inv(const rmh_transform* A_dev, rmh_transform* C_dev, uint32_t n, void* stream);
+/* out[i] = T[nT == 1 ? 0 : i] * p[i]  (multNxN / mult1xN Transform x Vector) */
+int rmh_transform_points(const rmh_transform* T_dev, uint32_t nT, const rmh_vec3* p_dev, rmh_vec3* out_dev,
+                         uint32_t n, void* stream);
+/* normalizeInplace of the rotation quaternions (memory_math.cu:402-413) */
+int rmh_quat_normalize(rmh_transform* T_dev, uint32_t n, void* stream);
+
+/* Workspace (device bytes) that rmh_vec3_moments / rmh_statistics need for this shape when
+ * a segment is split over several blocks; 0 when none is needed. */
+int rmh_reduction_workspace_size(uint64_t segment_size, uint32_t n_segments, size_t* bytes);
+
+/* Segmented FP64 moments of n_segments consecutive segments of segment_size vectors:
+ * sum_a (3/segment), mean_a (3/segment), cov = (1/n) sum b a^T (9/segment, column-major,
+ * b = a when b_dev is NULL) and count, over the elements whose mask is nonzero (mask NULL:
+ * all).  Replaces sum/mean/cov (memory_math.cu:415-469,1403-1549) and
+ * sumBatched/covBatched (math_batched.cu:30-546).  Outputs are nullable device arrays.
+ * workspace: device memory of rmh_reduction_workspace_size bytes, or NULL (then the call
+ * allocates and synchronises internally). */
+int rmh_vec3_moments(const rmh_vec3* a_dev, const rmh_vec3* b_dev, const uint8_t* mask_dev,
+                     uint64_t segment_size, uint32_t n_segments, double* sum_a_dev, double* mean_a_dev,
+                     double* cov_dev, uint32_t* count_dev, void* workspace, size_t workspace_bytes,
+                     void* stream);
+
+/* A point cloud view (PointCloudView_, src/rmagine_core/include/rmagine/types/PointCloud.hpp:62-70)
+ * over device arrays; mask, normals and ids are nullable.  Element i of segment s is at
+ * s * segment_stride + i (segment_stride 0: every segment shares the same elements). */
+typedef struct {
+  const rmh_vec3* points;
+  const uint8_t*  mask;
+  const rmh_vec3* normals;
+  const uint32_t* ids;
+  uint64_t        segment_stride;
+} rmh_cloud;
+
+/* rmagine::UmeyamaReductionConstraints (UmeyamaReductionConstraints.hpp:46-59) */
+typedef struct { float max_dist; uint32_t dataset_id; uint32_t model_id; } rmh_umeyama_constraints;
+
+/* FP64 CrossStatistics (CrossStatistics.hpp:50-60): means, covariance
+ * (1/n) sum (m - m_mean)(d - d_mean)^T column-major, count.  128 bytes. */
+typedef struct {
+  double   dataset_mean[3];
+  double   model_mean[3];
+  double   covariance[9];
+  uint32_t n_meas;
+  uint32_t _pad;
+} rmh_cross_stats;
+
+enum { RMH_STATS_P2P = 0, RMH_STATS_P2L = 1 };
+
+/* Correspondence statistics per segment (statistics_p2p / statistics_p2l,
+ * src/rmagine_cuda/src/math/statistics.cu:58-160,190-276,374-420; CPU
+ * src/rmagine_core/src/math/statistics.cpp:20-126), in FP64: element i passes the mask
+ * and id filters, D = pre_transform[s] * dataset point, then
+ *   p2p: M = model point, kept if |M - D| < max_dist;
+ *   p2l: M = D + N ((I - D) . N) with I, N the model point and normal, kept if
+ *        |(I - D) . N| < max_dist.
+ * pre_transforms_dev holds n_pre = 1 (shared) or n_segments transforms. */
+int rmh_statistics(int kind, const rmh_transform* pre_transforms_dev, uint32_t n_pre,
+                   const rmh_cloud* dataset, const rmh_cloud* model, uint64_t segment_size,
+                   uint32_t n_segments, const rmh_umeyama_constraints* params,
+                   rmh_cross_stats* stats_dev, void* workspace, size_t workspace_bytes, void* stream);
+
+/* Umeyama per statistics entry (umeyama_transform, src/rmagine_core/src/math/optimization.cpp:12-55;
+ * src/rmagine_cuda/src/math/linalg.cu:1150-1190) in FP64: out_dev receives FP32 transforms,
+ * out64_dev (nullable) 7 doubles per entry (qx, qy, qz, qw, tx, ty, tz). */
+int rmh_umeyama(const rmh_cross_stats* stats_dev, uint32_t n, rmh_transform* out_dev, double* out64_dev,
+                void* stream);
+
 #ifdef __cplusplus
 }
 #endif

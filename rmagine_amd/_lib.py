@@ -55,6 +55,17 @@ _SIGNATURES = {
     "rmh_sensor_dims": (c_int, [c_void_p, ctypes.POINTER(c_u32), ctypes.POINTER(c_u32)]),
     "rmh_simulate": (c_int, [c_void_p, c_void_p, ctypes.POINTER(T.rmh_transform), c_void_p, c_u32,
                              ctypes.POINTER(T.rmh_outputs), c_void_p]),
+    "rmh_transform_mul": (c_int, [c_void_p, c_u32, c_void_p, c_u32, c_void_p, c_u32, c_void_p]),
+    "rmh_transform_inv": (c_int, [c_void_p, c_void_p, c_u32, c_void_p]),
+    "rmh_transform_points": (c_int, [c_void_p, c_u32, c_void_p, c_void_p, c_u32, c_void_p]),
+    "rmh_quat_normalize": (c_int, [c_void_p, c_u32, c_void_p]),
+    "rmh_reduction_workspace_size": (c_int, [ctypes.c_uint64, c_u32, ctypes.POINTER(c_size)]),
+    "rmh_vec3_moments": (c_int, [c_void_p, c_void_p, c_void_p, ctypes.c_uint64, c_u32, c_void_p, c_void_p,
+                                 c_void_p, c_void_p, c_void_p, c_size, c_void_p]),
+    "rmh_statistics": (c_int, [c_int, c_void_p, c_u32, ctypes.POINTER(T.rmh_cloud), ctypes.POINTER(T.rmh_cloud),
+                               ctypes.c_uint64, c_u32, ctypes.POINTER(T.rmh_umeyama_constraints), c_void_p,
+                               c_void_p, c_size, c_void_p]),
+    "rmh_umeyama": (c_int, [c_void_p, c_u32, c_void_p, c_void_p, c_void_p]),
 }
 
 

@@ -1,0 +1,583 @@
+// Math layer of librmh: the MI355X counterpart of rmagine_cuda's math
+// (src/rmagine_cuda/src/math/memory_math.cu, math_batched.cu, statistics.cu, linalg.cu).
+//
+//   rmh_transform_mul / _inv / _points / rmh_quat_normalize
+//       element-wise pose algebra with N x N, N x 1 and 1 x N broadcasting
+//       (memory_math.cu:16-160 multNxN/multNx1/mult1xN/invert, :402-413 normalizeInplace)
+//   rmh_vec3_moments
+//       segmented sum / mean / cross second moment of vector pairs with an optional
+//       mask (memory_math.cu:415-469 cov_kernel, :1403-1549 sum/mean/cov;
+//       math_batched.cu:30-546 sumBatched/covBatched), FP64 accumulation
+//   rmh_statistics
+//       point-to-point / point-to-plane correspondence statistics, segmented (one
+//       segment per pose hypothesis), FP64 (statistics.cu:58-160 p2p, :190-276 p2l,
+//       :278-371 objectwise; CPU statistics.cpp:20-126)
+//   rmh_umeyama
+//       per-segment FP64 3x3 SVD (one-sided Jacobi) and the Umeyama rotation with the
+//       reflection fix (optimization.cpp:12-55, linalg.cu:1150-1190)
+//
+// Reductions are wave64 shuffles, then one LDS step across the waves of a block, then
+// (for segments split over several blocks) a second pass over the per-block partials in a
+// fixed order, so results are deterministic run to run.
+#include "rmh_internal.hpp"
+
+#include <cmath>
+#include <vector>
+
+namespace {
+
+using namespace rmh;
+
+constexpr int kBlk = 256;
+constexpr int kWaves = kBlk / 64;
+
+inline uint32_t grid1(uint64_t n) { return static_cast<uint32_t>((n + kBlk - 1) / kBlk); }
+
+// ---- pose algebra -----------------------------------------------------------------------
+
+__device__ __forceinline__ rmh_transform store_tf(const Tf& T, uint32_t stamp) {
+  rmh_transform r;
+  r.R.x = T.R.x; r.R.y = T.R.y; r.R.z = T.R.z; r.R.w = T.R.w;
+  r.t.x = T.t.x; r.t.y = T.t.y; r.t.z = T.t.z;
+  r.stamp = stamp;
+  return r;
+}
+
+__global__ void k_tf_mul(const rmh_transform* A, uint32_t nA, const rmh_transform* B, uint32_t nB, rmh_transform* C,
+                         uint32_t n) {
+  const uint32_t i = blockIdx.x * kBlk + threadIdx.x;
+  if (i >= n) return;
+  const rmh_transform a = A[nA == 1 ? 0 : i];
+  const rmh_transform b = B[nB == 1 ? 0 : i];
+  C[i] = store_tf(tmul(load_tf(a), load_tf(b)), a.stamp);
+}
+
+__global__ void k_tf_inv(const rmh_transform* A, rmh_transform* C, uint32_t n) {
+  const uint32_t i = blockIdx.x * kBlk + threadIdx.x;
+  if (i >= n) return;
+  const rmh_transform a = A[i];
+  const Tf T = load_tf(a);
+  Tf R;
+  R.R = qconj(T.R);                 // Transform.tcc:24-31
+  const V3 t = qrot(R.R, T.t);
+  R.t = v3(-t.x, -t.y, -t.z);
+  C[i] = store_tf(R, a.stamp);
+}
+
+__global__ void k_tf_points(const rmh_transform* Tr, uint32_t nT, const rmh_vec3* p, rmh_vec3* out, uint32_t n) {
+  const uint32_t i = blockIdx.x * kBlk + threadIdx.x;
+  if (i >= n) return;
+  const Tf T = load_tf(Tr[nT == 1 ? 0 : i]);
+  const rmh_vec3 a = p[i];
+  const V3 r = tapply(T, v3(a.x, a.y, a.z));
+  rmh_vec3 o; o.x = r.x; o.y = r.y; o.z = r.z;
+  out[i] = o;
+}
+
+__global__ void k_quat_normalize(rmh_transform* T, uint32_t n) {
+  const uint32_t i = blockIdx.x * kBlk + threadIdx.x;
+  if (i >= n) return;
+  rmh_quat q = T[i].R;
+  const float l = sqrtf(q.x * q.x + q.y * q.y + q.z * q.z + q.w * q.w);  // Quaternion::l2norm
+  q.x /= l; q.y /= l; q.z /= l; q.w /= l;
+  T[i].R = q;
+}
+
+// ---- reductions -------------------------------------------------------------------------
+
+template <int N>
+struct Acc {
+  double v[N];
+};
+
+template <int N>
+__device__ __forceinline__ void wave_sum(Acc<N>& a) {
+#pragma unroll
+  for (int k = 0; k < N; k++) {
+    double x = a.v[k];
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+    a.v[k] = x;
+  }
+}
+
+// block-wide sum, result valid in thread 0
+template <int N>
+__device__ __forceinline__ void block_sum(Acc<N>& a, double (*lds)[N]) {
+  wave_sum(a);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0)
+    for (int k = 0; k < N; k++) lds[w][k] = a.v[k];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int k = 0; k < N; k++) {
+      double s = lds[0][k];
+      for (int j = 1; j < kWaves; j++) s += lds[j][k];
+      a.v[k] = s;
+    }
+  }
+}
+
+// moments: [0] count, [1..3] sum a, [4..6] sum b, [7..15] sum b_r a_c (column-major, c*3+r)
+constexpr int kMom = 16;
+
+__device__ __forceinline__ void acc_pair(Acc<kMom>& m, double ax, double ay, double az, double bx, double by, double bz) {
+  m.v[0] += 1.0;
+  m.v[1] += ax; m.v[2] += ay; m.v[3] += az;
+  m.v[4] += bx; m.v[5] += by; m.v[6] += bz;
+  m.v[7] += bx * ax;  m.v[8] += by * ax;  m.v[9] += bz * ax;
+  m.v[10] += bx * ay; m.v[11] += by * ay; m.v[12] += bz * ay;
+  m.v[13] += bx * az; m.v[14] += by * az; m.v[15] += bz * az;
+}
+
+__device__ __forceinline__ void zero(Acc<kMom>& m) {
+#pragma unroll
+  for (int k = 0; k < kMom; k++) m.v[k] = 0.0;
+}
+
+// ---- segmented vector moments (sum / mean / cov) ---------------------------------------
+
+struct MomArgs {
+  const rmh_vec3* a;
+  const rmh_vec3* b;        // NULL: b = a
+  const uint8_t* mask;      // NULL: all
+  uint64_t seg;             // elements per segment
+  uint32_t nseg;
+  uint32_t bps;             // blocks per segment
+  double* partial;          // bps > 1: nseg*bps*kMom
+  double* sum_a;            // 3 per segment (nullable)
+  double* mean_a;           // 3 per segment (nullable)
+  double* cov;              // 9 per segment: (1/n) sum b a^T, column-major (nullable)
+  uint32_t* count;          // nullable
+};
+
+__device__ void finalize_moments(const MomArgs& a, uint32_t s, const Acc<kMom>& m) {
+  const double n = m.v[0];
+  if (a.count) a.count[s] = static_cast<uint32_t>(n);
+  if (a.sum_a)
+    for (int k = 0; k < 3; k++) a.sum_a[3 * s + k] = m.v[1 + k];
+  if (a.mean_a)
+    for (int k = 0; k < 3; k++) a.mean_a[3 * s + k] = n > 0 ? m.v[1 + k] / n : 0.0;
+  if (a.cov)
+    for (int k = 0; k < 9; k++) a.cov[9 * s + k] = n > 0 ? m.v[7 + k] / n : 0.0;
+}
+
+__global__ __launch_bounds__(kBlk) void k_moments(const MomArgs a) {
+  __shared__ double lds[kWaves][kMom];
+  const uint32_t s = blockIdx.y, bb = blockIdx.x;
+  Acc<kMom> m;
+  zero(m);
+  const uint64_t base = static_cast<uint64_t>(s) * a.seg;
+  const uint64_t stride = static_cast<uint64_t>(a.bps) * kBlk;
+  for (uint64_t i = static_cast<uint64_t>(bb) * kBlk + threadIdx.x; i < a.seg; i += stride) {
+    const uint64_t g = base + i;
+    if (a.mask && a.mask[g] == 0) continue;
+    const rmh_vec3 x = a.a[g];
+    const rmh_vec3 y = a.b ? a.b[g] : x;
+    acc_pair(m, x.x, x.y, x.z, y.x, y.y, y.z);
+  }
+  block_sum(m, lds);
+  if (threadIdx.x != 0) return;
+  if (a.bps == 1) finalize_moments(a, s, m);
+  else
+    for (int k = 0; k < kMom; k++) a.partial[(static_cast<uint64_t>(s) * a.bps + bb) * kMom + k] = m.v[k];
+}
+
+__global__ void k_moments_combine(const MomArgs a) {
+  const uint32_t s = blockIdx.x * kBlk + threadIdx.x;
+  if (s >= a.nseg) return;
+  Acc<kMom> m;
+  zero(m);
+  for (uint32_t b = 0; b < a.bps; b++)
+    for (int k = 0; k < kMom; k++) m.v[k] += a.partial[(static_cast<uint64_t>(s) * a.bps + b) * kMom + k];
+  finalize_moments(a, s, m);
+}
+
+// ---- correspondence statistics ------------------------------------------------------------
+
+struct StatArgs {
+  int kind;                       // 0 p2p, 1 p2l
+  const rmh_transform* pre;
+  uint32_t npre;
+  rmh_cloud d, m;
+  uint64_t seg;
+  uint32_t nseg, bps;
+  rmh_umeyama_constraints prm;
+  double* partial;
+  rmh_cross_stats* out;
+};
+
+struct M3d {
+  double m[9];  // row-major rotation of q p q^-1 (valid for any q, |q|^2 scaled)
+};
+
+__device__ __forceinline__ M3d quat_matrix(const rmh_quat& qf) {
+  const double x = qf.x, y = qf.y, z = qf.z, w = qf.w;
+  M3d r;
+  r.m[0] = w * w + x * x - y * y - z * z; r.m[1] = 2.0 * (x * y - w * z);          r.m[2] = 2.0 * (x * z + w * y);
+  r.m[3] = 2.0 * (x * y + w * z);         r.m[4] = w * w - x * x + y * y - z * z;  r.m[5] = 2.0 * (y * z - w * x);
+  r.m[6] = 2.0 * (x * z - w * y);         r.m[7] = 2.0 * (y * z + w * x);          r.m[8] = w * w - x * x - y * y + z * z;
+  return r;
+}
+
+__device__ void finalize_stats(rmh_cross_stats* out, const Acc<kMom>& m, const double c[3]) {
+  rmh_cross_stats r;
+  const double n = m.v[0];
+  r.n_meas = static_cast<uint32_t>(n);
+  r._pad = 0;
+  if (n > 0) {
+    double dm[3], mm[3];
+    for (int k = 0; k < 3; k++) { dm[k] = m.v[1 + k] / n; mm[k] = m.v[4 + k] / n; }
+    for (int col = 0; col < 3; col++)
+      for (int row = 0; row < 3; row++)
+        r.covariance[col * 3 + row] = m.v[7 + col * 3 + row] / n - mm[row] * dm[col];
+    for (int k = 0; k < 3; k++) { r.dataset_mean[k] = dm[k] + c[k]; r.model_mean[k] = mm[k] + c[k]; }
+  } else {
+    for (int k = 0; k < 3; k++) { r.dataset_mean[k] = 0.0; r.model_mean[k] = 0.0; }
+    for (int k = 0; k < 9; k++) r.covariance[k] = 0.0;
+  }
+  *out = r;
+}
+
+__global__ __launch_bounds__(kBlk) void k_stats(const StatArgs a) {
+  __shared__ double lds[kWaves][kMom];
+  const uint32_t s = blockIdx.y, bb = blockIdx.x;
+  const rmh_transform P = a.pre[a.npre == 1 ? 0 : s];
+  const M3d R = quat_matrix(P.R);
+  // moments are taken relative to the pre-transform's translation (exact shift, keeps the
+  // cancellation in sum(m d^T)/n - mean_m mean_d^T small)
+  const double c[3] = {(double)P.t.x, (double)P.t.y, (double)P.t.z};
+  Acc<kMom> m;
+  zero(m);
+  const uint64_t db = static_cast<uint64_t>(s) * a.d.segment_stride;
+  const uint64_t mb = static_cast<uint64_t>(s) * a.m.segment_stride;
+  const uint64_t stride = static_cast<uint64_t>(a.bps) * kBlk;
+  const double maxd = a.prm.max_dist;
+  for (uint64_t i = static_cast<uint64_t>(bb) * kBlk + threadIdx.x; i < a.seg; i += stride) {
+    const uint64_t di = db + i, mi = mb + i;
+    // statistics.cu:230-235 / statistics.cpp:99-103: mask and id filters
+    if (a.d.mask && a.d.mask[di] == 0) continue;
+    if (a.m.mask && a.m.mask[mi] == 0) continue;
+    if (a.d.ids && a.d.ids[di] != a.prm.dataset_id) continue;
+    if (a.m.ids && a.m.ids[mi] != a.prm.model_id) continue;
+    const rmh_vec3 dp = a.d.points[di];
+    const rmh_vec3 mp = a.m.points[mi];
+    // Di = pre_transform * dataset point (FP64)
+    const double dx = R.m[0] * dp.x + R.m[1] * dp.y + R.m[2] * dp.z + (double)P.t.x;
+    const double dy = R.m[3] * dp.x + R.m[4] * dp.y + R.m[5] * dp.z + (double)P.t.y;
+    const double dz = R.m[6] * dp.x + R.m[7] * dp.y + R.m[8] * dp.z + (double)P.t.z;
+    double mx = mp.x, my = mp.y, mz = mp.z;
+    if (a.kind == 1) {
+      // point to plane: project Di onto the model plane (statistics.cpp:105-119)
+      const rmh_vec3 nn = a.m.normals[mi];
+      const double spd = (mx - dx) * nn.x + (my - dy) * nn.y + (mz - dz) * nn.z;
+      if (!(fabs(spd) < maxd)) continue;
+      mx = dx + nn.x * spd;
+      my = dy + nn.y * spd;
+      mz = dz + nn.z * spd;
+    } else {
+      // point to point (statistics.cpp:40-50)
+      const double ex = mx - dx, ey = my - dy, ez = mz - dz;
+      if (!(sqrt(ex * ex + ey * ey + ez * ez) < maxd)) continue;
+    }
+    acc_pair(m, dx - c[0], dy - c[1], dz - c[2], mx - c[0], my - c[1], mz - c[2]);
+  }
+  block_sum(m, lds);
+  if (threadIdx.x != 0) return;
+  if (a.bps == 1) finalize_stats(&a.out[s], m, c);
+  else
+    for (int k = 0; k < kMom; k++) a.partial[(static_cast<uint64_t>(s) * a.bps + bb) * kMom + k] = m.v[k];
+}
+
+__global__ void k_stats_combine(const StatArgs a) {
+  const uint32_t s = blockIdx.x * kBlk + threadIdx.x;
+  if (s >= a.nseg) return;
+  const rmh_transform P = a.pre[a.npre == 1 ? 0 : s];
+  const double c[3] = {(double)P.t.x, (double)P.t.y, (double)P.t.z};
+  Acc<kMom> m;
+  zero(m);
+  for (uint32_t b = 0; b < a.bps; b++)
+    for (int k = 0; k < kMom; k++) m.v[k] += a.partial[(static_cast<uint64_t>(s) * a.bps + b) * kMom + k];
+  finalize_stats(&a.out[s], m, c);
+}
+
+// ---- Umeyama ------------------------------------------------------------------------------
+
+__device__ __forceinline__ double det3(const double* M) {  // row-major
+  return M[0] * (M[4] * M[8] - M[5] * M[7]) - M[1] * (M[3] * M[8] - M[5] * M[6]) + M[2] * (M[3] * M[7] - M[4] * M[6]);
+}
+
+// One-sided Jacobi SVD of a 3x3 matrix A (row-major): A = U diag(s) V^T, s descending,
+// U and V orthonormal (U completed by cross products for rank-deficient A).
+__device__ void svd3(const double* Ain, double* U, double* s, double* V) {
+  double A[9];
+  for (int k = 0; k < 9; k++) A[k] = Ain[k];
+  for (int k = 0; k < 9; k++) V[k] = (k % 4 == 0) ? 1.0 : 0.0;
+  for (int sweep = 0; sweep < 40; sweep++) {
+    bool rotated = false;
+    for (int pq = 0; pq < 3; pq++) {
+      const int p = pq == 2 ? 1 : 0, q = pq == 0 ? 1 : 2;
+      double al = 0, be = 0, ga = 0;
+      for (int r = 0; r < 3; r++) {
+        al += A[r * 3 + p] * A[r * 3 + p];
+        be += A[r * 3 + q] * A[r * 3 + q];
+        ga += A[r * 3 + p] * A[r * 3 + q];
+      }
+      if (fabs(ga) <= 1e-15 * sqrt(al * be) || ga == 0.0) continue;
+      rotated = true;
+      const double zeta = (be - al) / (2.0 * ga);
+      const double t = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+      const double cs = 1.0 / sqrt(1.0 + t * t), sn = cs * t;
+      for (int r = 0; r < 3; r++) {
+        const double ap = A[r * 3 + p], aq = A[r * 3 + q];
+        A[r * 3 + p] = cs * ap - sn * aq;
+        A[r * 3 + q] = sn * ap + cs * aq;
+        const double vp = V[r * 3 + p], vq = V[r * 3 + q];
+        V[r * 3 + p] = cs * vp - sn * vq;
+        V[r * 3 + q] = sn * vp + cs * vq;
+      }
+    }
+    if (!rotated) break;
+  }
+  double sv[3];
+  for (int c = 0; c < 3; c++) sv[c] = sqrt(A[c] * A[c] + A[3 + c] * A[3 + c] + A[6 + c] * A[6 + c]);
+  int ord[3] = {0, 1, 2};
+  for (int i = 0; i < 2; i++)
+    for (int j = 0; j < 2 - i; j++)
+      if (sv[ord[j]] < sv[ord[j + 1]]) { const int t = ord[j]; ord[j] = ord[j + 1]; ord[j + 1] = t; }
+  double Vs[9];
+  for (int c = 0; c < 3; c++) {
+    s[c] = sv[ord[c]];
+    for (int r = 0; r < 3; r++) {
+      Vs[r * 3 + c] = V[r * 3 + ord[c]];
+      U[r * 3 + c] = s[c] > 0 ? A[r * 3 + ord[c]] / s[c] : 0.0;
+    }
+  }
+  for (int k = 0; k < 9; k++) V[k] = Vs[k];
+  const double tiny = 1e-13 * s[0];
+  if (!(s[1] > tiny)) {
+    // rank <= 1: complete U[:,1] orthogonal to U[:,0]
+    const double ux = U[0], uy = U[3], uz = U[6];
+    double ax = 1, ay = 0, az = 0;
+    if (fabs(ux) > 0.9) { ax = 0; ay = 1; }
+    double cx = uy * az - uz * ay, cy = uz * ax - ux * az, cz = ux * ay - uy * ax;
+    const double l = sqrt(cx * cx + cy * cy + cz * cz);
+    U[1] = cx / l; U[4] = cy / l; U[7] = cz / l;
+  }
+  if (!(s[2] > tiny)) {
+    U[2] = U[3] * U[7] - U[6] * U[4];
+    U[5] = U[6] * U[1] - U[0] * U[7];
+    U[8] = U[0] * U[4] - U[3] * U[1];
+  }
+}
+
+// Umeyama (optimization.cpp:12-49): C = U S V^T, R = U diag(1, 1, sign(det U det V)) V^T,
+// converted to a quaternion with the reference's matrix -> quaternion formula
+// (src/rmagine_core/include/rmagine/math/types/MatrixOps.tcc:1492-1529), normalised;
+// a non-finite rotation falls back to identity; t = m - R d; n_meas == 0 -> identity.
+__global__ void k_umeyama(const rmh_cross_stats* st, uint32_t n, rmh_transform* out, double* out64) {
+  const uint32_t i = blockIdx.x * kBlk + threadIdx.x;
+  if (i >= n) return;
+  const rmh_cross_stats S = st[i];
+  double q[4] = {0.0, 0.0, 0.0, 1.0};
+  double t[3] = {0.0, 0.0, 0.0};
+  if (S.n_meas > 0) {
+    double C[9];
+    for (int r = 0; r < 3; r++)
+      for (int c = 0; c < 3; c++) C[r * 3 + c] = S.covariance[c * 3 + r];
+    double U[9], sv[3], V[9];
+    svd3(C, U, sv, V);
+    bool ok = sv[0] > 0.0;
+    if (ok) {
+      const double sd = (det3(U) * det3(V) < 0.0) ? -1.0 : 1.0;
+      double R[9];
+      for (int r = 0; r < 3; r++)
+        for (int c = 0; c < 3; c++)
+          R[r * 3 + c] = U[r * 3 + 0] * V[c * 3 + 0] + U[r * 3 + 1] * V[c * 3 + 1] + sd * U[r * 3 + 2] * V[c * 3 + 2];
+      const double tr = R[0] + R[4] + R[8];
+      double x, y, z, w;
+      if (tr > 0) {
+        const double s4 = sqrt(tr + 1.0) * 2.0;
+        w = 0.25 * s4; x = (R[7] - R[5]) / s4; y = (R[2] - R[6]) / s4; z = (R[3] - R[1]) / s4;
+      } else if (R[0] > R[4] && R[0] > R[8]) {
+        const double s4 = sqrt(1.0 + R[0] - R[4] - R[8]) * 2.0;
+        w = (R[7] - R[5]) / s4; x = 0.25 * s4; y = (R[1] + R[3]) / s4; z = (R[2] + R[6]) / s4;
+      } else if (R[4] > R[8]) {
+        const double s4 = sqrt(1.0 + R[4] - R[0] - R[8]) * 2.0;
+        w = (R[2] - R[6]) / s4; x = (R[1] + R[3]) / s4; y = 0.25 * s4; z = (R[5] + R[7]) / s4;
+      } else {
+        const double s4 = sqrt(1.0 + R[8] - R[0] - R[4]) * 2.0;
+        w = (R[3] - R[1]) / s4; x = (R[2] + R[6]) / s4; y = (R[5] + R[7]) / s4; z = 0.25 * s4;
+      }
+      const double l = sqrt(x * x + y * y + z * z + w * w);
+      x /= l; y /= l; z /= l; w /= l;
+      ok = isfinite(x) && isfinite(y) && isfinite(z) && isfinite(w);
+      if (ok) { q[0] = x; q[1] = y; q[2] = z; q[3] = w; }
+    }
+    // t = m - R * d, R applied as the quaternion sandwich
+    const double x = q[0], y = q[1], z = q[2], w = q[3];
+    const double d0 = S.dataset_mean[0], d1 = S.dataset_mean[1], d2 = S.dataset_mean[2];
+    const double r0 = (w * w + x * x - y * y - z * z) * d0 + 2.0 * (x * y - w * z) * d1 + 2.0 * (x * z + w * y) * d2;
+    const double r1 = 2.0 * (x * y + w * z) * d0 + (w * w - x * x + y * y - z * z) * d1 + 2.0 * (y * z - w * x) * d2;
+    const double r2 = 2.0 * (x * z - w * y) * d0 + 2.0 * (y * z + w * x) * d1 + (w * w - x * x - y * y + z * z) * d2;
+    t[0] = S.model_mean[0] - r0;
+    t[1] = S.model_mean[1] - r1;
+    t[2] = S.model_mean[2] - r2;
+  }
+  rmh_transform T;
+  T.R.x = (float)q[0]; T.R.y = (float)q[1]; T.R.z = (float)q[2]; T.R.w = (float)q[3];
+  T.t.x = (float)t[0]; T.t.y = (float)t[1]; T.t.z = (float)t[2];
+  T.stamp = 0;
+  if (out) out[i] = T;
+  if (out64) {
+    for (int k = 0; k < 4; k++) out64[7 * i + k] = q[k];
+    for (int k = 0; k < 3; k++) out64[7 * i + 4 + k] = t[k];
+  }
+}
+
+uint32_t blocks_per_segment(uint64_t seg, uint32_t nseg) {
+  // enough blocks in total to fill the chip (>= 1024), at least 2048 elements per block
+  const uint64_t by_size = (seg + 2047) / 2048;
+  const uint64_t want = (1024 + nseg - 1) / nseg;
+  uint64_t b = want < by_size ? want : by_size;
+  if (b < 1) b = 1;
+  if (b > 256) b = 256;
+  return static_cast<uint32_t>(b);
+}
+
+struct Scratch {
+  void* p = nullptr;
+  ~Scratch() { if (p) (void)hipFree(p); }
+};
+
+}  // namespace
+
+extern "C" {
+
+int rmh_transform_mul(const rmh_transform* A, uint32_t nA, const rmh_transform* B, uint32_t nB, rmh_transform* C,
+                      uint32_t n, void* stream) {
+  if (n == 0) return RMH_OK;
+  if (!A || !B || !C) return fail(RMH_ERR_INVALID, "rmh_transform_mul: NULL pointer");
+  if ((nA != 1 && nA != n) || (nB != 1 && nB != n)) return fail(RMH_ERR_INVALID, "rmh_transform_mul: sizes must be 1 or n");
+  k_tf_mul<<<grid1(n), kBlk, 0, static_cast<hipStream_t>(stream)>>>(A, nA, B, nB, C, n);
+  RMH_HIP_TRY(hipGetLastError());
+  return RMH_OK;
+}
+
+int rmh_transform_inv(const rmh_transform* A, rmh_transform* C, uint32_t n, void* stream) {
+  if (n == 0) return RMH_OK;
+  if (!A || !C) return fail(RMH_ERR_INVALID, "rmh_transform_inv: NULL pointer");
+  k_tf_inv<<<grid1(n), kBlk, 0, static_cast<hipStream_t>(stream)>>>(A, C, n);
+  RMH_HIP_TRY(hipGetLastError());
+  return RMH_OK;
+}
+
+int rmh_transform_points(const rmh_transform* T, uint32_t nT, const rmh_vec3* p, rmh_vec3* out, uint32_t n, void* stream) {
+  if (n == 0) return RMH_OK;
+  if (!T || !p || !out) return fail(RMH_ERR_INVALID, "rmh_transform_points: NULL pointer");
+  if (nT != 1 && nT != n) return fail(RMH_ERR_INVALID, "rmh_transform_points: nT must be 1 or n");
+  k_tf_points<<<grid1(n), kBlk, 0, static_cast<hipStream_t>(stream)>>>(T, nT, p, out, n);
+  RMH_HIP_TRY(hipGetLastError());
+  return RMH_OK;
+}
+
+int rmh_quat_normalize(rmh_transform* T, uint32_t n, void* stream) {
+  if (n == 0) return RMH_OK;
+  if (!T) return fail(RMH_ERR_INVALID, "rmh_quat_normalize: NULL pointer");
+  k_quat_normalize<<<grid1(n), kBlk, 0, static_cast<hipStream_t>(stream)>>>(T, n);
+  RMH_HIP_TRY(hipGetLastError());
+  return RMH_OK;
+}
+
+int rmh_reduction_workspace_size(uint64_t segment_size, uint32_t n_segments, size_t* bytes) {
+  if (!bytes) return fail(RMH_ERR_INVALID, "rmh_reduction_workspace_size: NULL");
+  const uint32_t b = blocks_per_segment(segment_size, n_segments ? n_segments : 1);
+  *bytes = b > 1 ? sizeof(double) * kMom * static_cast<size_t>(b) * n_segments : 0;
+  return RMH_OK;
+}
+
+int rmh_vec3_moments(const rmh_vec3* a, const rmh_vec3* b, const uint8_t* mask, uint64_t segment_size,
+                     uint32_t n_segments, double* sum_a, double* mean_a, double* cov, uint32_t* count,
+                     void* workspace, size_t workspace_bytes, void* stream_) {
+  if (n_segments == 0) return RMH_OK;
+  if (!a && segment_size > 0) return fail(RMH_ERR_INVALID, "rmh_vec3_moments: NULL input");
+  if (n_segments > 65535u) return fail(RMH_ERR_UNSUPPORTED, "rmh_vec3_moments: at most 65535 segments");
+  hipStream_t stream = static_cast<hipStream_t>(stream_);
+  MomArgs A{a, b, mask, segment_size, n_segments, blocks_per_segment(segment_size, n_segments), nullptr,
+            sum_a, mean_a, cov, count};
+  Scratch scratch;
+  if (A.bps > 1) {
+    const size_t need = sizeof(double) * kMom * static_cast<size_t>(A.bps) * n_segments;
+    if (workspace) {
+      if (workspace_bytes < need) return fail(RMH_ERR_INVALID, "rmh_vec3_moments: workspace too small");
+      A.partial = static_cast<double*>(workspace);
+    } else {
+      RMH_HIP_TRY(hipMalloc(&scratch.p, need));
+      A.partial = static_cast<double*>(scratch.p);
+    }
+  }
+  k_moments<<<dim3(A.bps, n_segments), kBlk, 0, stream>>>(A);
+  RMH_HIP_TRY(hipGetLastError());
+  if (A.bps > 1) {
+    k_moments_combine<<<grid1(n_segments), kBlk, 0, stream>>>(A);
+    RMH_HIP_TRY(hipGetLastError());
+    if (scratch.p) RMH_HIP_TRY(hipStreamSynchronize(stream));
+  }
+  return RMH_OK;
+}
+
+int rmh_statistics(int kind, const rmh_transform* pre_transforms, uint32_t n_pre, const rmh_cloud* dataset,
+                   const rmh_cloud* model, uint64_t segment_size, uint32_t n_segments,
+                   const rmh_umeyama_constraints* params, rmh_cross_stats* stats, void* workspace,
+                   size_t workspace_bytes, void* stream_) {
+  if (n_segments == 0) return RMH_OK;
+  if (!pre_transforms || !dataset || !model || !params || !stats)
+    return fail(RMH_ERR_INVALID, "rmh_statistics: NULL argument");
+  if (kind != RMH_STATS_P2P && kind != RMH_STATS_P2L) return fail(RMH_ERR_INVALID, "rmh_statistics: bad kind");
+  if (n_pre != 1 && n_pre != n_segments) return fail(RMH_ERR_INVALID, "rmh_statistics: n_pre must be 1 or n_segments");
+  if (segment_size > 0 && (!dataset->points || !model->points)) return fail(RMH_ERR_INVALID, "rmh_statistics: points are required");
+  if (kind == RMH_STATS_P2L && segment_size > 0 && !model->normals)
+    return fail(RMH_ERR_INVALID, "rmh_statistics: point-to-plane needs model normals");
+  if (n_segments > 65535u) return fail(RMH_ERR_UNSUPPORTED, "rmh_statistics: at most 65535 segments");
+  hipStream_t stream = static_cast<hipStream_t>(stream_);
+  StatArgs A;
+  A.kind = kind;
+  A.pre = pre_transforms;
+  A.npre = n_pre;
+  A.d = *dataset;
+  A.m = *model;
+  A.seg = segment_size;
+  A.nseg = n_segments;
+  A.bps = blocks_per_segment(segment_size, n_segments);
+  A.prm = *params;
+  A.partial = nullptr;
+  A.out = stats;
+  Scratch scratch;
+  if (A.bps > 1) {
+    const size_t need = sizeof(double) * kMom * static_cast<size_t>(A.bps) * n_segments;
+    if (workspace) {
+      if (workspace_bytes < need) return fail(RMH_ERR_INVALID, "rmh_statistics: workspace too small");
+      A.partial = static_cast<double*>(workspace);
+    } else {
+      RMH_HIP_TRY(hipMalloc(&scratch.p, need));
+      A.partial = static_cast<double*>(scratch.p);
+    }
+  }
+  k_stats<<<dim3(A.bps, n_segments), kBlk, 0, stream>>>(A);
+  RMH_HIP_TRY(hipGetLastError());
+  if (A.bps > 1) {
+    k_stats_combine<<<grid1(n_segments), kBlk, 0, stream>>>(A);
+    RMH_HIP_TRY(hipGetLastError());
+    if (scratch.p) RMH_HIP_TRY(hipStreamSynchronize(stream));
+  }
+  return RMH_OK;
+}
+
+int rmh_umeyama(const rmh_cross_stats* stats, uint32_t n, rmh_transform* out, double* out64, void* stream) {
+  if (n == 0) return RMH_OK;
+  if (!stats || (!out && !out64)) return fail(RMH_ERR_INVALID, "rmh_umeyama: NULL argument");
+  k_umeyama<<<grid1(n), kBlk, 0, static_cast<hipStream_t>(stream)>>>(stats, n, out, out64);
+  RMH_HIP_TRY(hipGetLastError());
+  return RMH_OK;
+}
+
+}  // extern "C"

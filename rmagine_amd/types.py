@@ -203,6 +203,21 @@ class rmh_map_info(ctypes.Structure):
                 ("device_bytes", ctypes.c_uint64), ("build_ms", ctypes.c_float), ("sah_cost", ctypes.c_float)]
 
 
+class rmh_cloud(ctypes.Structure):
+    _fields_ = [("points", ctypes.c_void_p), ("mask", ctypes.c_void_p), ("normals", ctypes.c_void_p),
+                ("ids", ctypes.c_void_p), ("segment_stride", ctypes.c_uint64)]
+
+
+class rmh_umeyama_constraints(ctypes.Structure):
+    _fields_ = [("max_dist", ctypes.c_float), ("dataset_id", ctypes.c_uint32), ("model_id", ctypes.c_uint32)]
+
+
+class rmh_cross_stats(ctypes.Structure):
+    _fields_ = [("dataset_mean", ctypes.c_double * 3), ("model_mean", ctypes.c_double * 3),
+                ("covariance", ctypes.c_double * 9), ("n_meas", ctypes.c_uint32), ("_pad", ctypes.c_uint32)]
+
+
+assert ctypes.sizeof(rmh_cross_stats) == 128
 assert ctypes.sizeof(rmh_transform) == 32
 assert ctypes.sizeof(rmh_spherical_model) == 32
 assert ctypes.sizeof(rmh_pinhole_model) == 32

@@ -1,0 +1,161 @@
+// C++ drop-in test of the *SimulatorHip API (runs on a HIP device).
+//
+// Each case follows one of the reference's simulation tests with the Hip simulator in
+// place of the Embree/OptiX one: unit cube map, 100 identity poses, repeated simulate()
+// calls, known-answer ray checked with the reference's tolerance 1e-4.
+//   spherical   tests/embree/simulation_spherical.cpp:26-72 (range 0.500076)
+//   o1dn        tests/embree/simulation_o1dn.cpp:30-78 (point (0.256732, 0.5, -0.1))
+//   pinhole     tests/embree/simulation_pinhole.cpp:28-79 (point (0.5, 0.5, 0.375))
+//   ondn        tests/embree/simulation_ondn.cpp:30-76 (range 0.5)
+//   single ray  tests/embree/simulation_single_ray.cpp:24-70 (range 0.5)
+// plus the error convention (no map -> std::runtime_error).  Throws on failure.
+#include <rmagine/map/HipMap.hpp>
+#include <rmagine/simulation/O1DnSimulatorHip.hpp>
+#include <rmagine/simulation/OnDnSimulatorHip.hpp>
+#include <rmagine/simulation/PinholeSimulatorHip.hpp>
+#include <rmagine/simulation/SphereSimulatorHip.hpp>
+
+#include <cmath>
+#include <iostream>
+#include <sstream>
+#include <string>
+#include <vector>
+
+using namespace rmagine;
+
+static void require(bool ok, const std::string& what) {
+  if (!ok) throw std::runtime_error("FAILED: " + what);
+}
+
+static HipMapPtr make_cube_map() {
+  // genCube (src/rmagine_core/src/util/synthetic.cpp:183-221)
+  const std::vector<Vector> v = {{-0.5f, -0.5f, -0.5f}, {-0.5f, -0.5f, 0.5f}, {-0.5f, 0.5f, 0.5f}, {-0.5f, 0.5f, -0.5f},
+                                 {0.5f, 0.5f, -0.5f},   {0.5f, 0.5f, 0.5f},   {0.5f, -0.5f, 0.5f}, {0.5f, -0.5f, -0.5f}};
+  const std::vector<Face> f = {{0, 1, 2}, {2, 3, 0}, {4, 5, 6}, {6, 7, 4}, {0, 7, 1}, {1, 7, 6},
+                               {1, 6, 5}, {5, 2, 1}, {2, 5, 4}, {3, 2, 4}, {3, 4, 7}, {3, 7, 0}};
+  Memory<Vector, RAM> V(v.size());
+  Memory<Face, RAM> F(f.size());
+  for (size_t i = 0; i < v.size(); i++) V[i] = v[i];
+  for (size_t i = 0; i < f.size(); i++) F[i] = f[i];
+  return std::make_shared<HipMap>(V, F);
+}
+
+static Memory<Transform, VRAM_HIP> identity_poses(size_t n) {
+  Memory<Transform, RAM> T(n);
+  for (size_t i = 0; i < n; i++) T[i] = Transform::Identity();
+  Memory<Transform, VRAM_HIP> Td;
+  Td = T;
+  return Td;
+}
+
+static void test_spherical(const HipMapPtr& map) {
+  SphereSimulatorHip sim;
+  sim.setMap(map);
+  const SphericalModel model = example_spherical();
+  sim.setModel(model);
+  IntAttrAll<VRAM_HIP> result;
+  resize_memory_bundle<VRAM_HIP>(result, model.getWidth(), model.getHeight(), 100);
+  const auto T = identity_poses(100);
+  for (int i = 0; i < 20; i++) {
+    sim.simulate(T, result);
+    Memory<float, RAM> last_scan = result.ranges(model.size() * 99, model.size() * 100);
+    const float range = last_scan[model.getBufferId(model.phi.size / 2, 0)];
+    std::ostringstream ss;
+    ss << "spherical range " << range << " vs 0.500076";
+    require(std::fabs(range - 0.500076f) <= 1e-4f, ss.str());
+  }
+  // returned bundle + single host pose overloads
+  auto res = sim.simulate<Bundle<Ranges<VRAM_HIP>>>(Transform::Identity());
+  Memory<float, RAM> r = res.ranges;
+  require(r.size() == model.size() && std::fabs(r[model.getBufferId(8, 0)] - 0.500076f) <= 1e-4f, "spherical single pose");
+}
+
+static void test_o1dn(const HipMapPtr& map) {
+  O1DnSimulatorHip sim;
+  sim.setMap(map);
+  const O1DnModel model = example_o1dn();
+  sim.setModel(model);
+  IntAttrAll<VRAM_HIP> result;
+  resize_memory_bundle<VRAM_HIP>(result, model.getWidth(), model.getHeight(), 100);
+  const auto T = identity_poses(100);
+  for (int i = 0; i < 100; i++) {
+    sim.simulate(T, result);
+    Memory<float, RAM> last_scan = result.ranges(model.size() * 99, model.size() * 100);
+    const float range = last_scan[model.getBufferId(0, 0)];
+    const Vector p = model.orig + model.dirs[0] * range;
+    const float err = std::fabs(p.x - 0.256732f) + std::fabs(p.y - 0.5f) + std::fabs(p.z + 0.1f);
+    require(err <= 1e-4f, "o1dn point");
+  }
+  Memory<Vector, RAM> normals = result.normals(0, 1);
+  require(std::fabs(normals[0].y + 1.0f) < 1e-6f, "o1dn normal (0,-1,0)");
+}
+
+static void test_pinhole(const HipMapPtr& map) {
+  PinholeSimulatorHip sim(map);
+  const PinholeModel model = example_pinhole();
+  sim.setModel(model);
+  IntAttrAll<VRAM_HIP> result;
+  resize_memory_bundle<VRAM_HIP>(result, model.getWidth(), model.getHeight(), 100);
+  const auto T = identity_poses(100);
+  for (int i = 0; i < 10; i++) {
+    sim.simulate(T, result);
+    Memory<float, RAM> last_scan = result.ranges(model.size() * 99, model.size() * 100);
+    const float range = last_scan[model.getBufferId(0, 0)];
+    const Vector p = model.getDirection(0, 0) * range;
+    const float err = std::fabs(p.x - 0.5f) + std::fabs(p.y - 0.5f) + std::fabs(p.z - 0.375f);
+    require(err <= 1e-4f, "pinhole point");
+  }
+}
+
+static void test_ondn(const HipMapPtr& map) {
+  OnDnSimulatorHip sim(map);
+  const OnDnModel model = example_ondn();
+  sim.setModel(model);
+  IntAttrAll<VRAM_HIP> result;
+  resize_memory_bundle<VRAM_HIP>(result, model.getWidth(), model.getHeight(), 100);
+  const auto T = identity_poses(100);
+  for (int i = 0; i < 100; i++) {
+    sim.simulate(T, result);
+    Memory<float, RAM> last_scan = result.ranges(model.size() * 99, model.size() * 100);
+    require(std::fabs(last_scan[0] - 0.5f) <= 1e-4f, "ondn range");
+  }
+  // single ray (tests/embree/simulation_single_ray.cpp)
+  OnDnModel one;
+  one.width = 1;
+  one.height = 1;
+  one.range = {0.0f, 100.0f};
+  one.dirs.resize(1);
+  one.origs.resize(1);
+  one.dirs[0] = {1.0f, 0.0f, 0.0f};
+  one.origs[0] = {0.0f, 0.0f, 0.0f};
+  sim.setModel(one);
+  auto res = sim.simulate<IntAttrAll<VRAM_HIP>>(identity_poses(100));
+  Memory<float, RAM> r = res.ranges;
+  require(std::fabs(r[0] - 0.5f) <= 1e-4f && std::fabs(r[99] - 0.5f) <= 1e-4f, "single ray range");
+  Memory<unsigned int, RAM> gid = res.geom_ids;
+  require(gid[0] == 0u, "single mesh geom id 0");
+}
+
+static void test_no_map() {
+  SphereSimulatorHip sim;
+  sim.setModel(example_spherical());
+  bool thrown = false;
+  try {
+    IntAttrAll<VRAM_HIP> res;
+    sim.simulate(identity_poses(1), res);
+  } catch (const std::runtime_error&) {
+    thrown = true;
+  }
+  require(thrown, "simulate without a map throws");
+}
+
+int main() {
+  const HipMapPtr map = make_cube_map();
+  test_spherical(map);
+  test_o1dn(map);
+  test_pinhole(map);
+  test_ondn(map);
+  test_no_map();
+  std::cout << "simulation_kat: all C++ drop-in checks passed" << std::endl;
+  return 0;
+}
