@@ -158,6 +158,67 @@ int rmh_simulate(const rmh_map* map, const rmh_sensor* sensor,
                  const rmh_transform* Tsb, const rmh_transform* Tbm_dev, uint32_t n_poses,
                  const rmh_outputs* out, void* stream);
 
+/* ---- batched heterogeneous simulation ---------------------------------------------------
+ * No counterpart in the reference (it launches one sensor model per optixLaunch,
+ * src/rmagine_optix/src/simulation/SphereSimulatorOptix.cpp:64-81; the closest piece is the
+ * model union of src/rmagine_optix/include/rmagine/simulation/optix/sim_program_data.h:32-44).
+ * A batch is a device table of sensors (any mix of models, poses and ray counts) cast in ONE
+ * persistent launch; every entry writes its own rmh_outputs exactly as rmh_simulate would. */
+typedef struct {
+  const rmh_sensor*    sensor;
+  rmh_transform        Tsb;        /* sensor -> base (host value)               */
+  const rmh_transform* Tbm_dev;    /* n_poses base -> map poses (device)        */
+  uint32_t             n_poses;
+  uint32_t             _pad;
+  rmh_outputs          out;        /* device buffers, n_poses*W*H entries each  */
+} rmh_batch_entry;
+
+typedef struct {
+  uint32_t chunk_rays;    /* rays per work grab, multiple of 64 (0 = keep / default 256)  */
+  uint32_t waves_per_cu;  /* persistent waves per compute unit (0 = keep / default 16)   */
+} rmh_batch_params;
+
+typedef struct rmh_batch rmh_batch;
+
+/* Builds the device table (the pointers in the entries must stay valid while the batch is
+ * used).  Synchronous. */
+int rmh_batch_create(const rmh_map* map, const rmh_batch_entry* entries, uint32_t n_entries,
+                     const rmh_batch_params* params, rmh_batch** out);
+int rmh_batch_set_params(rmh_batch* batch, const rmh_batch_params* params);
+int rmh_batch_get_info(const rmh_batch* batch, uint64_t* total_rays, uint32_t* total_chunks,
+                       rmh_batch_params* params);
+/* One counter-reset kernel + one persistent kernel on `stream`; capturable into a graph.
+ * Launches of one batch must not overlap (they share its work counter). */
+int rmh_batch_simulate(rmh_batch* batch, void* stream);
+int rmh_batch_destroy(rmh_batch* batch);
+
+/* ---- graphs: record a sequence of rmh_* launches on a stream once, replay it ------------
+ * No counterpart in the reference.  Capture needs a non-default stream; nothing captured
+ * may allocate or synchronise (pass workspaces to the reductions). */
+typedef struct rmh_graph rmh_graph;
+int rmh_graph_begin_capture(void* stream);
+int rmh_graph_end_capture(void* stream, rmh_graph** out);
+int rmh_graph_launch(rmh_graph* graph, void* stream);
+int rmh_graph_destroy(rmh_graph* graph);
+
+/* ---- pose adjoint ------------------------------------------------------------------------
+ * No counterpart in the reference (closest: jacobian_and_residual_p2l,
+ * src/rmagine_core/include/rmagine/math/optimization.h:113-137; se3/so3 Jacobians,
+ * src/rmagine_core/include/rmagine/math/lie.tcc:252-277).
+ * For every pose: forward closest-hit ranges r (as rmh_simulate), Huber(huber_delta) loss of
+ * r - target over rays that hit (r >= range.min) and whose target is valid (target_mask != 0,
+ * or range.min <= target <= range.max when target_mask is NULL), and its gradient with
+ * respect to a left perturbation of the pose in the map frame (t + tau, Exp(omega) R).
+ * target: W*H floats shared by all poses (per_pose_targets = 0) or n_poses*W*H.
+ * out_dev: 8 doubles per pose {loss, dL/dtau xyz, dL/domega xyz, rays used}.
+ * workspace: rmh_pose_grad_workspace_size bytes of device memory, or NULL (then the call
+ * allocates and synchronises internally). */
+int rmh_pose_grad_workspace_size(const rmh_map* map, const rmh_sensor* sensor, uint32_t n_poses, size_t* bytes);
+int rmh_pose_grad(const rmh_map* map, const rmh_sensor* sensor, const rmh_transform* Tsb,
+                  const rmh_transform* Tbm_dev, uint32_t n_poses, const float* target_dev,
+                  const uint8_t* target_mask_dev, int per_pose_targets, float huber_delta,
+                  double* out_dev, void* workspace, size_t workspace_bytes, void* stream);
+
 /* ---- math layer (rmagine_cuda's math, src/rmagine_cuda/src/math/) --------------------- */
 
 /* C[i] = A[nA == 1 ? 0 : i] * B[nB == 1 ? 0 : i], i < n  (multNxN / multNx1 / mult1xN of
@@ -171,6 +232,9 @@ int rmh_transform_points(const rmh_transform* T_dev, uint32_t nT, const rmh_vec3
                          uint32_t n, void* stream);
 /* normalizeInplace of the rotation quaternions (memory_math.cu:402-413) */
 int rmh_quat_normalize(rmh_transform* T_dev, uint32_t n, void* stream);
+/* T[i] = identity (setIdentity_kernel, memory_math.cu:1342-1352); a kernel, never a memset,
+ * so it can be captured into a graph */
+int rmh_transform_set_identity(rmh_transform* T_dev, uint32_t n, void* stream);
 
 /* Workspace (device bytes) that rmh_vec3_moments / rmh_statistics need for this shape when
  * a segment is split over several blocks; 0 when none is needed. */

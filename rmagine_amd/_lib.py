@@ -55,6 +55,21 @@ _SIGNATURES = {
     "rmh_sensor_dims": (c_int, [c_void_p, ctypes.POINTER(c_u32), ctypes.POINTER(c_u32)]),
     "rmh_simulate": (c_int, [c_void_p, c_void_p, ctypes.POINTER(T.rmh_transform), c_void_p, c_u32,
                              ctypes.POINTER(T.rmh_outputs), c_void_p]),
+    "rmh_batch_create": (c_int, [c_void_p, ctypes.POINTER(T.rmh_batch_entry), c_u32,
+                                 ctypes.POINTER(T.rmh_batch_params), ctypes.POINTER(c_void_p)]),
+    "rmh_batch_set_params": (c_int, [c_void_p, ctypes.POINTER(T.rmh_batch_params)]),
+    "rmh_batch_get_info": (c_int, [c_void_p, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(c_u32),
+                                   ctypes.POINTER(T.rmh_batch_params)]),
+    "rmh_batch_simulate": (c_int, [c_void_p, c_void_p]),
+    "rmh_batch_destroy": (c_int, [c_void_p]),
+    "rmh_graph_begin_capture": (c_int, [c_void_p]),
+    "rmh_graph_end_capture": (c_int, [c_void_p, ctypes.POINTER(c_void_p)]),
+    "rmh_graph_launch": (c_int, [c_void_p, c_void_p]),
+    "rmh_graph_destroy": (c_int, [c_void_p]),
+    "rmh_pose_grad_workspace_size": (c_int, [c_void_p, c_void_p, c_u32, ctypes.POINTER(c_size)]),
+    "rmh_pose_grad": (c_int, [c_void_p, c_void_p, ctypes.POINTER(T.rmh_transform), c_void_p, c_u32, c_void_p,
+                              c_void_p, c_int, ctypes.c_float, c_void_p, c_void_p, c_size, c_void_p]),
+    "rmh_transform_set_identity": (c_int, [c_void_p, c_u32, c_void_p]),
     "rmh_transform_mul": (c_int, [c_void_p, c_u32, c_void_p, c_u32, c_void_p, c_u32, c_void_p]),
     "rmh_transform_inv": (c_int, [c_void_p, c_void_p, c_u32, c_void_p]),
     "rmh_transform_points": (c_int, [c_void_p, c_u32, c_void_p, c_void_p, c_u32, c_void_p]),

@@ -1,0 +1,198 @@
+// Pose adjoint of librmh: gradients of a Huber range-residual loss with respect to 6-DoF
+// sensor poses, through the hit triangle's plane.  The reference has no differentiable
+// path; the closest pieces are its point-to-plane Jacobian
+// (src/rmagine_core/include/rmagine/math/optimization.h:113-137) and the SE(3)
+// exponential / SO(3) left Jacobian (src/rmagine_core/include/rmagine/math/lie.tcc:252-277).
+//
+// For pose p (base -> map transform Tbm = (R, t)) and ray k the forward pass is the same
+// closest-hit query as simulate() (FP64 distance r to the hit triangle with unit world
+// normal n, ray origin o and direction d in the map frame).  With a target range r*_k the
+// loss is L_p = sum_k rho(r_k - r*_k), rho the Huber function with threshold delta, over
+// rays that hit (r >= range.min) and whose target is valid.  The pose is perturbed on the
+// left in the map frame, t -> t + tau, R -> Exp(omega) R; then
+//   dr/dtau   = g,                      g = -n / (n . d)
+//   dr/domega = (o + r d - t) x g       (lever arm from the base origin to the hit point)
+// and dL/d(tau, omega) = sum_k rho'(e_k) dr_k/d(tau, omega).  Per ray in FP64, reduced per
+// pose in FP64 (wave shuffles, LDS, then the per-pose chunks in a fixed order).
+//
+// Output per pose: 8 doubles {L, dL/dtau (3), dL/domega (3), number of rays used}.
+#include "rmh_trace.hpp"
+
+#include <algorithm>
+
+namespace {
+
+using namespace rmh;
+
+struct GradArgs {
+  MapData map;
+  RayGen g;
+  Tf Tsb;
+  const rmh_transform* Tbm;
+  uint32_t pose_offset;
+  const float* target;
+  const uint8_t* target_mask;
+  uint32_t target_stride;     // 0: one target scan shared by all poses; else ray_count
+  double delta;
+  uint32_t chunks;            // blocks per pose
+  double* partial;            // (pose, chunk) -> 8 doubles
+};
+
+template <int KIND, int STACK, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_pose_grad(const GradArgs a) {
+  __shared__ int stk[STACK][BLOCK];
+  __shared__ double red[BLOCK / 64][8];
+  const uint32_t pose = a.pose_offset + blockIdx.y;
+  const uint32_t lid = blockIdx.x * BLOCK + threadIdx.x;
+  double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  if (lid < a.g.ray_count) {
+    const uint32_t vid = lid / a.g.W, hid = lid - vid * a.g.W;
+    V3 ds, os;
+    gen_ray<KIND>(a.g, vid, hid, lid, ds, os);
+    const Tf Tbm = load_tf(a.Tbm[pose]);
+    const Tf Tsm = tmul(Tbm, a.Tsb);
+    const V3 dm = qrot(Tsm.R, ds);
+    const V3 om = map_origin<KIND>(Tsm, os);
+    Hit h;
+    trace<STACK, BLOCK>(a.map, om, dm, a.g.range_max, stk, h);
+    const size_t ti = static_cast<size_t>(pose) * a.target_stride + lid;
+    const float tr = a.target[ti];
+    const bool tvalid = a.target_mask ? (a.target_mask[ti] != 0) : (tr >= a.g.range_min && tr <= a.g.range_max);
+    if (h.slot != 0xFFFFFFFFu && static_cast<float>(h.t) >= a.g.range_min && tvalid) {
+      const float4 nn = a.map.normals[h.slot];
+      const double nx = nn.x, ny = nn.y, nz = nn.z;
+      const double dx = dm.x, dy = dm.y, dz = dm.z;
+      const double nd = nx * dx + ny * dy + nz * dz;
+      if (nd != 0.0) {
+        const double e = h.t - static_cast<double>(tr);
+        const double ae = fabs(e);
+        const double rho = ae <= a.delta ? 0.5 * e * e : a.delta * (ae - 0.5 * a.delta);
+        const double drho = ae <= a.delta ? e : (e > 0 ? a.delta : -a.delta);
+        const double gx = -nx / nd, gy = -ny / nd, gz = -nz / nd;
+        const double lx = (double)om.x + h.t * dx - (double)Tbm.t.x;
+        const double ly = (double)om.y + h.t * dy - (double)Tbm.t.y;
+        const double lz = (double)om.z + h.t * dz - (double)Tbm.t.z;
+        acc[0] = rho;
+        acc[1] = drho * gx;
+        acc[2] = drho * gy;
+        acc[3] = drho * gz;
+        acc[4] = drho * (ly * gz - lz * gy);
+        acc[5] = drho * (lz * gx - lx * gz);
+        acc[6] = drho * (lx * gy - ly * gx);
+        acc[7] = 1.0;
+      }
+    }
+  }
+  for (int k = 0; k < 8; k++) {
+    double x = acc[k];
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+    acc[k] = x;
+  }
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0)
+    for (int k = 0; k < 8; k++) red[w][k] = acc[k];
+  __syncthreads();
+  if (threadIdx.x < 8) {
+    double s = red[0][threadIdx.x];
+    for (int j = 1; j < BLOCK / 64; j++) s += red[j][threadIdx.x];
+    a.partial[(static_cast<size_t>(pose) * a.chunks + blockIdx.x) * 8 + threadIdx.x] = s;
+  }
+}
+
+__global__ void k_pose_grad_combine(const double* partial, uint32_t chunks, uint32_t n_poses, double* out) {
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n_poses * 8u) return;
+  const uint32_t p = i >> 3, k = i & 7u;
+  double s = 0.0;
+  for (uint32_t c = 0; c < chunks; c++) s += partial[(static_cast<size_t>(p) * chunks + c) * 8 + k];
+  out[i] = s;
+}
+
+template <int KIND, int STACK, int BLOCK>
+hipError_t launch_grad(const GradArgs& a, uint32_t np, hipStream_t stream) {
+  k_pose_grad<KIND, STACK, BLOCK><<<dim3(a.chunks, np), BLOCK, 0, stream>>>(a);
+  return hipGetLastError();
+}
+
+template <int STACK, int BLOCK>
+hipError_t launch_grad_stack(int kind, const GradArgs& a, uint32_t np, hipStream_t stream) {
+  switch (kind) {
+    case RMH_MODEL_SPHERICAL: return launch_grad<RMH_MODEL_SPHERICAL, STACK, BLOCK>(a, np, stream);
+    case RMH_MODEL_PINHOLE: return launch_grad<RMH_MODEL_PINHOLE, STACK, BLOCK>(a, np, stream);
+    case RMH_MODEL_O1DN: return launch_grad<RMH_MODEL_O1DN, STACK, BLOCK>(a, np, stream);
+    default: return launch_grad<RMH_MODEL_ONDN, STACK, BLOCK>(a, np, stream);
+  }
+}
+
+uint32_t grad_block(uint32_t depth) { return depth <= 32 ? 256u : (depth <= 64 ? 128u : 64u); }
+
+struct Scratch {
+  void* p = nullptr;
+  ~Scratch() { if (p) (void)hipFree(p); }
+};
+
+}  // namespace
+
+extern "C" {
+
+int rmh_pose_grad_workspace_size(const rmh_map* map, const rmh_sensor* sensor, uint32_t n_poses, size_t* bytes) {
+  if (!map || !sensor || !bytes) return fail(RMH_ERR_INVALID, "rmh_pose_grad_workspace_size: NULL argument");
+  const uint32_t blk = grad_block(map->info.max_depth);
+  const uint64_t rc = static_cast<uint64_t>(sensor->width) * sensor->height;
+  const uint64_t chunks = (rc + blk - 1) / blk;
+  *bytes = sizeof(double) * 8 * chunks * n_poses;
+  return RMH_OK;
+}
+
+int rmh_pose_grad(const rmh_map* map, const rmh_sensor* sensor, const rmh_transform* Tsb, const rmh_transform* Tbm,
+                  uint32_t n_poses, const float* target, const uint8_t* target_mask, int per_pose_targets,
+                  float huber_delta, double* out, void* workspace, size_t workspace_bytes, void* stream_) {
+  if (!map) return fail(RMH_ERR_NOMAP, "rmh_pose_grad: no map set");
+  if (!sensor || !out || !target) return fail(RMH_ERR_INVALID, "rmh_pose_grad: NULL argument");
+  if (n_poses == 0) return RMH_OK;
+  if (n_poses > (1u << 28)) return fail(RMH_ERR_UNSUPPORTED, "rmh_pose_grad: too many poses");
+  if (!Tbm) return fail(RMH_ERR_INVALID, "rmh_pose_grad: Tbm is NULL");
+  if (!(huber_delta > 0.0f)) return fail(RMH_ERR_INVALID, "rmh_pose_grad: huber_delta must be > 0");
+  if (map->info.max_depth > 128) return fail(RMH_ERR_UNSUPPORTED, "rmh_pose_grad: BVH deeper than 128 levels");
+  const uint64_t rc = static_cast<uint64_t>(sensor->width) * sensor->height;
+  if (rc == 0 || rc > (1ull << 30)) return fail(RMH_ERR_UNSUPPORTED, "rmh_pose_grad: bad ray count");
+  const uint32_t depth = map->info.max_depth;
+  const uint32_t blk = grad_block(depth);
+  GradArgs a;
+  a.map = make_mapdata(map);
+  a.g = make_raygen(sensor);
+  rmh_transform I{};
+  I.R.w = 1.0f;
+  a.Tsb = load_tf(Tsb ? *Tsb : I);
+  a.Tbm = Tbm;
+  a.target = target;
+  a.target_mask = target_mask;
+  a.target_stride = per_pose_targets ? static_cast<uint32_t>(rc) : 0u;
+  a.delta = huber_delta;
+  a.chunks = static_cast<uint32_t>((rc + blk - 1) / blk);
+  const size_t need = sizeof(double) * 8 * static_cast<size_t>(a.chunks) * n_poses;
+  Scratch scratch;
+  if (workspace) {
+    if (workspace_bytes < need) return fail(RMH_ERR_INVALID, "rmh_pose_grad: workspace too small");
+    a.partial = static_cast<double*>(workspace);
+  } else {
+    RMH_HIP_TRY(hipMalloc(&scratch.p, need));
+    a.partial = static_cast<double*>(scratch.p);
+  }
+  hipStream_t stream = static_cast<hipStream_t>(stream_);
+  for (uint32_t p0 = 0; p0 < n_poses; p0 += 65535u) {
+    const uint32_t np = std::min<uint32_t>(65535u, n_poses - p0);
+    a.pose_offset = p0;
+    hipError_t e;
+    if (depth <= 32) e = launch_grad_stack<32, 256>(sensor->kind, a, np, stream);
+    else if (depth <= 64) e = launch_grad_stack<64, 128>(sensor->kind, a, np, stream);
+    else e = launch_grad_stack<128, 64>(sensor->kind, a, np, stream);
+    if (e != hipSuccess) return hip_fail(e, "rmh_pose_grad launch");
+  }
+  k_pose_grad_combine<<<(n_poses * 8u + 255u) / 256u, 256, 0, stream>>>(a.partial, a.chunks, n_poses, out);
+  RMH_HIP_TRY(hipGetLastError());
+  if (scratch.p) RMH_HIP_TRY(hipStreamSynchronize(stream));
+  return RMH_OK;
+}
+
+}  // extern "C"

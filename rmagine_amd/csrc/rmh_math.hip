@@ -74,6 +74,14 @@ __global__ void k_tf_points(const rmh_transform* Tr, uint32_t nT, const rmh_vec3
   out[i] = o;
 }
 
+__global__ void k_set_identity(rmh_transform* T, uint32_t n) {
+  const uint32_t i = blockIdx.x * kBlk + threadIdx.x;
+  if (i >= n) return;
+  rmh_transform r{};
+  r.R.w = 1.0f;
+  T[i] = r;
+}
+
 __global__ void k_quat_normalize(rmh_transform* T, uint32_t n) {
   const uint32_t i = blockIdx.x * kBlk + threadIdx.x;
   if (i >= n) return;
@@ -476,6 +484,14 @@ int rmh_transform_points(const rmh_transform* T, uint32_t nT, const rmh_vec3* p,
   if (!T || !p || !out) return fail(RMH_ERR_INVALID, "rmh_transform_points: NULL pointer");
   if (nT != 1 && nT != n) return fail(RMH_ERR_INVALID, "rmh_transform_points: nT must be 1 or n");
   k_tf_points<<<grid1(n), kBlk, 0, static_cast<hipStream_t>(stream)>>>(T, nT, p, out, n);
+  RMH_HIP_TRY(hipGetLastError());
+  return RMH_OK;
+}
+
+int rmh_transform_set_identity(rmh_transform* T, uint32_t n, void* stream) {
+  if (n == 0) return RMH_OK;
+  if (!T) return fail(RMH_ERR_INVALID, "rmh_transform_set_identity: NULL pointer");
+  k_set_identity<<<grid1(n), kBlk, 0, static_cast<hipStream_t>(stream)>>>(T, n);
   RMH_HIP_TRY(hipGetLastError());
   return RMH_OK;
 }

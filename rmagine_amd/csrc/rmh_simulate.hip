@@ -17,10 +17,22 @@
 //     face ids and ranges agree with an FP64 evaluation of the same predicate; ties at
 //     equal distance go to the lower global face index;
 //   * fused attribute epilogue, writing only the requested attributes.
-#include "rmh_internal.hpp"
+//
+// Two launch forms:
+//   rmh_simulate        one sensor model, n poses, a 1-D grid over all rays;
+//   rmh_batch_simulate  many sensors (any mix of models, poses and ray counts) in ONE
+//                       persistent launch: waves grab chunks of rays of one sensor from an
+//                       atomic counter until the work is exhausted.  All pointers and
+//                       sizes live in a device table built once, so the launch (counter
+//                       reset kernel + persistent kernel) can be captured into a graph.
+// The reference has no batched form; its closest piece is the per-launch model union
+// (src/rmagine_optix/include/rmagine/simulation/optix/sim_program_data.h:32-44).
+#include "rmh_trace.hpp"
 
+#include <algorithm>
 #include <cmath>
-#include <limits>
+#include <new>
+#include <vector>
 
 namespace {
 
@@ -28,227 +40,35 @@ using namespace rmh;
 
 struct SimArgs {
   MapData map;
-  int W, H;
-  float range_min, range_max;
-  const float2* lut;                 // spherical direction table
-  float pf0, pf1, pc0, pc1;          // pinhole
-  V3 orig;                           // O1Dn origin (sensor frame)
-  const rmh_vec3* dirs;              // O1Dn / OnDn
-  const rmh_vec3* origs;             // OnDn
+  RayGen g;
   Tf Tsb;
   const rmh_transform* Tbm;
   uint32_t n_rays;                   // rays in this launch (< 2^31)
   uint32_t pose_offset;              // first pose of this launch
-  uint32_t ray_count;                // W*H
   rmh_outputs out;
 };
-
-__device__ __forceinline__ float sel3(const float4& v, int k) { return k == 0 ? v.x : (k == 1 ? v.y : v.z); }
-
-struct TriRay {
-  double okx, oky, okz;
-  double Sx, Sy, Sz;
-  int kx, ky, kz;
-};
-
-__device__ __forceinline__ TriRay make_tri_ray(const V3& o, const V3& d) {
-  TriRay r;
-  const float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
-  int kz = (ax > ay) ? (ax > az ? 0 : 2) : (ay > az ? 1 : 2);
-  int kx = kz == 2 ? 0 : kz + 1;
-  int ky = kx == 2 ? 0 : kx + 1;
-  const float dkz = kz == 0 ? d.x : (kz == 1 ? d.y : d.z);
-  if (dkz < 0.0f) { const int t = kx; kx = ky; ky = t; }
-  const double dd[3] = {(double)d.x, (double)d.y, (double)d.z};
-  const double oo[3] = {(double)o.x, (double)o.y, (double)o.z};
-  r.kx = kx; r.ky = ky; r.kz = kz;
-  r.Sx = dd[kx] / dd[kz];
-  r.Sy = dd[ky] / dd[kz];
-  r.Sz = 1.0 / dd[kz];
-  r.okx = oo[kx]; r.oky = oo[ky]; r.okz = oo[kz];
-  return r;
-}
-
-// Watertight test in FP64.  Returns t >= 0 of the hit or a negative value for no hit.
-__device__ __forceinline__ double tri_test(const TriRay& r, const float4& v0, const float4& v1, const float4& v2) {
-  const double Akx = (double)sel3(v0, r.kx) - r.okx, Aky = (double)sel3(v0, r.ky) - r.oky, Akz = (double)sel3(v0, r.kz) - r.okz;
-  const double Bkx = (double)sel3(v1, r.kx) - r.okx, Bky = (double)sel3(v1, r.ky) - r.oky, Bkz = (double)sel3(v1, r.kz) - r.okz;
-  const double Ckx = (double)sel3(v2, r.kx) - r.okx, Cky = (double)sel3(v2, r.ky) - r.oky, Ckz = (double)sel3(v2, r.kz) - r.okz;
-  const double Ax = Akx - r.Sx * Akz, Ay = Aky - r.Sy * Akz;
-  const double Bx = Bkx - r.Sx * Bkz, By = Bky - r.Sy * Bkz;
-  const double Cx = Ckx - r.Sx * Ckz, Cy = Cky - r.Sy * Ckz;
-  const double U = Cx * By - Cy * Bx;
-  const double V = Ax * Cy - Ay * Cx;
-  const double W = Bx * Ay - By * Ax;
-  if ((U < 0.0 || V < 0.0 || W < 0.0) && (U > 0.0 || V > 0.0 || W > 0.0)) return -1.0;
-  const double det = U + V + W;
-  if (det == 0.0) return -1.0;
-  const double T = U * (r.Sz * Akz) + V * (r.Sz * Bkz) + W * (r.Sz * Ckz);
-  return T / det;  // NaN or negative => no hit (callers test t >= 0)
-}
-
-struct Hit {
-  double t;
-  uint32_t slot;  // Morton slot of the hit triangle, 0xFFFFFFFF = none
-  uint32_t face;  // global face index (tie break)
-};
-
-template <int STACK, int BLOCK>
-__device__ __forceinline__ void trace(const MapData& m, const V3& o, const V3& d, float range_max,
-                                      int (*stk)[BLOCK], Hit& h) {
-  h.t = (double)range_max;
-  h.slot = 0xFFFFFFFFu;
-  h.face = 0xFFFFFFFFu;
-  if (m.n_faces == 0) return;
-  const int tid = threadIdx.x;
-  const float idx = 1.0f / (fabsf(d.x) > 1e-30f ? d.x : copysignf(1e-30f, d.x));
-  const float idy = 1.0f / (fabsf(d.y) > 1e-30f ? d.y : copysignf(1e-30f, d.y));
-  const float idz = 1.0f / (fabsf(d.z) > 1e-30f ? d.z : copysignf(1e-30f, d.z));
-  const TriRay tr = make_tri_ray(o, d);
-  float tbf = range_max;
-  int ref = m.root;
-  int sp = 0;
-  for (;;) {
-    if (ref >= 0) {
-      const Node2 nd = m.nodes[ref];
-      const float x00 = (nd.a.x - o.x) * idx, x01 = (nd.a.y - o.x) * idx;
-      const float y00 = (nd.a.z - o.y) * idy, y01 = (nd.a.w - o.y) * idy;
-      const float z00 = (nd.c.x - o.z) * idz, z01 = (nd.c.y - o.z) * idz;
-      const float x10 = (nd.b.x - o.x) * idx, x11 = (nd.b.y - o.x) * idx;
-      const float y10 = (nd.b.z - o.y) * idy, y11 = (nd.b.w - o.y) * idy;
-      const float z10 = (nd.c.z - o.z) * idz, z11 = (nd.c.w - o.z) * idz;
-      const float tmin0 = fmaxf(fmaxf(fminf(x00, x01), fminf(y00, y01)), fmaxf(fminf(z00, z01), 0.0f));
-      const float tmax0 = fminf(fminf(fmaxf(x00, x01), fmaxf(y00, y01)), fminf(fmaxf(z00, z01), tbf));
-      const float tmin1 = fmaxf(fmaxf(fminf(x10, x11), fminf(y10, y11)), fmaxf(fminf(z10, z11), 0.0f));
-      const float tmax1 = fminf(fminf(fmaxf(x10, x11), fmaxf(y10, y11)), fminf(fmaxf(z10, z11), tbf));
-      const bool h0 = tmin0 <= tmax0 * 1.0000005f;
-      const bool h1 = tmin1 <= tmax1 * 1.0000005f;
-      if (h0 && h1) {
-        int near = nd.d.x, far = nd.d.y;
-        if (tmin1 < tmin0) { near = nd.d.y; far = nd.d.x; }
-        if (sp < STACK) stk[sp++][tid] = far;
-        ref = near;
-        continue;
-      }
-      if (h0) { ref = nd.d.x; continue; }
-      if (h1) { ref = nd.d.y; continue; }
-    } else {
-      const uint32_t e = ~static_cast<uint32_t>(ref);
-      const uint32_t first = e >> 3, cnt = (e & 7u) + 1u;
-      for (uint32_t k = 0; k < cnt; k++) {
-        const size_t s = static_cast<size_t>(first + k) * 3;
-        const float4 v0 = m.tris[s], v1 = m.tris[s + 1], v2 = m.tris[s + 2];
-        const double t = tri_test(tr, v0, v1, v2);
-        if (t >= 0.0 && t <= h.t) {
-          const uint32_t face = __float_as_uint(v2.w);
-          if (t < h.t || face < h.face) {
-            h.t = t;
-            h.face = face;
-            h.slot = first + k;
-          }
-        }
-      }
-      tbf = __double2float_ru(h.t);
-    }
-    if (sp == 0) break;
-    ref = stk[--sp][tid];
-  }
-}
-
-template <int KIND>
-__device__ __forceinline__ void gen_ray(const SimArgs& a, uint32_t vid, uint32_t hid, uint32_t lid, V3& ds, V3& os) {
-  if constexpr (KIND == RMH_MODEL_SPHERICAL) {
-    // SphericalModel::getDirection (sensor_models.h:170-175)
-    const float2 p = a.lut[vid];
-    const float2 t = a.lut[a.H + hid];
-    ds = v3(p.x * t.x, p.x * t.y, p.y);
-    os = v3(0.0f, 0.0f, 0.0f);
-  } else if constexpr (KIND == RMH_MODEL_PINHOLE) {
-    // PinholeModel::getDirectionOptical / getDirection (sensor_models.h:256-277)
-    const float pX = (static_cast<float>(hid) - a.pc0) / a.pf0;
-    const float pY = (static_cast<float>(vid) - a.pc1) / a.pf1;
-    const V3 dopt = normalize(v3(pX, pY, 1.0f));
-    ds = v3(dopt.z, -dopt.x, -dopt.y);
-    os = v3(0.0f, 0.0f, 0.0f);
-  } else if constexpr (KIND == RMH_MODEL_O1DN) {
-    // O1DnModel_::getDirection / getOrigin (sensor_models.h:376-385)
-    const rmh_vec3 dd = a.dirs[lid];
-    ds = v3(dd.x, dd.y, dd.z);
-    os = a.orig;
-  } else {
-    // OnDnModel_::getDirection / getOrigin (sensor_models.h:460-470)
-    const rmh_vec3 dd = a.dirs[lid];
-    const rmh_vec3 oo = a.origs[lid];
-    ds = v3(dd.x, dd.y, dd.z);
-    os = v3(oo.x, oo.y, oo.z);
-  }
-}
 
 template <int KIND, int STACK, int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_simulate(const SimArgs a) {
   __shared__ int stk[STACK][BLOCK];
   const uint32_t gid = blockIdx.x * BLOCK + threadIdx.x;
   if (gid >= a.n_rays) return;
-  const uint32_t pl = gid / a.ray_count;
-  const uint32_t lid = gid - pl * a.ray_count;
+  const uint32_t rc = a.g.ray_count;
+  const uint32_t pl = gid / rc;
+  const uint32_t lid = gid - pl * rc;
   const uint32_t pose = a.pose_offset + pl;
-  const uint32_t vid = lid / static_cast<uint32_t>(a.W);
-  const uint32_t hid = lid - vid * static_cast<uint32_t>(a.W);
-  const uint64_t glob = static_cast<uint64_t>(pose) * a.ray_count + lid;
+  const uint32_t vid = lid / a.g.W;
+  const uint32_t hid = lid - vid * a.g.W;
+  const uint64_t glob = static_cast<uint64_t>(pose) * rc + lid;
 
   V3 ds, os;
-  gen_ray<KIND>(a, vid, hid, lid, ds, os);
-  // Tsm = Tbm * Tsb (SphereSimulatorEmbree.tcc:69-70)
-  const Tf Tsm = tmul(load_tf(a.Tbm[pose]), a.Tsb);
+  gen_ray<KIND>(a.g, vid, hid, lid, ds, os);
+  const Tf Tsm = tmul(load_tf(a.Tbm[pose]), a.Tsb);  // Tsm = Tbm * Tsb (SphereSimulatorEmbree.tcc:69-70)
   const V3 dm = qrot(Tsm.R, ds);
-  V3 om;
-  if constexpr (KIND == RMH_MODEL_SPHERICAL || KIND == RMH_MODEL_PINHOLE) om = Tsm.t;
-  else om = tapply(Tsm, os);  // O1DnSimulatorEmbree.tcc:86-87
-
+  const V3 om = map_origin<KIND>(Tsm, os);
   Hit h;
-  trace<STACK, BLOCK>(a.map, om, dm, a.range_max, stk, h);
-
-  const rmh_outputs& out = a.out;
-  if (h.slot != 0xFFFFFFFFu) {
-    const float t = static_cast<float>(h.t);
-    if (out.hits) out.hits[glob] = (t >= a.range_min) ? 1 : 0;
-    if (out.ranges) out.ranges[glob] = t;
-    if (out.points) {
-      // ray_dir_s * t (+ ray_orig_s)   SphereSimulatorEmbree.tcc:131, O1DnSimulatorEmbree.tcc:132
-      V3 p = scale(ds, t);
-      if constexpr (KIND == RMH_MODEL_O1DN || KIND == RMH_MODEL_ONDN) p = add(p, os);
-      rmh_vec3 pv; pv.x = p.x; pv.y = p.y; pv.z = p.z;
-      out.points[glob] = pv;
-    }
-    if (out.normals) {
-      // normal into the sensor frame, flipped toward the sensor, normalised
-      // (SphereSimulatorEmbree.tcc:140-156)
-      const float4 nn = a.map.normals[h.slot];
-      V3 n = qrot(qconj(Tsm.R), v3(nn.x, nn.y, nn.z));
-      if (dot(ds, n) > 0.0f) n = scale(n, -1.0f);
-      n = normalize(n);
-      rmh_vec3 nv; nv.x = n.x; nv.y = n.y; nv.z = n.z;
-      out.normals[glob] = nv;
-    }
-    if (out.face_ids || out.geom_ids || out.object_ids) {
-      const size_t s = static_cast<size_t>(h.slot) * 3;
-      const uint32_t face = __float_as_uint(a.map.tris[s].w);
-      const uint32_t geom = __float_as_uint(a.map.tris[s + 1].w);
-      if (out.face_ids) out.face_ids[glob] = face;
-      if (out.geom_ids) out.geom_ids[glob] = geom;
-      if (out.object_ids) out.object_ids[glob] = geom;  // no instances: object id = geom id
-    }
-  } else {
-    // SphereSimulatorEmbree.tcc:189-249
-    const float qnan = __builtin_nanf("");
-    if (out.hits) out.hits[glob] = 0;
-    if (out.ranges) out.ranges[glob] = a.range_max + 1.0f;
-    if (out.points) { rmh_vec3 v; v.x = qnan; v.y = qnan; v.z = qnan; out.points[glob] = v; }
-    if (out.normals) { rmh_vec3 v; v.x = qnan; v.y = qnan; v.z = qnan; out.normals[glob] = v; }
-    if (out.face_ids) out.face_ids[glob] = 0xFFFFFFFFu;
-    if (out.geom_ids) out.geom_ids[glob] = 0xFFFFFFFFu;
-    if (out.object_ids) out.object_ids[glob] = 0xFFFFFFFFu;
-  }
+  trace<STACK, BLOCK>(a.map, om, dm, a.g.range_max, stk, h);
+  write_outputs<KIND>(a.out, glob, a.g, a.map, Tsm, ds, os, h);
 }
 
 template <int KIND, int STACK, int BLOCK>
@@ -268,10 +88,140 @@ hipError_t launch_stack(int kind, const SimArgs& a, hipStream_t stream) {
   }
 }
 
+// ---- batched, persistent ---------------------------------------------------------------
+
+struct BatchEntryDev {
+  RayGen g;
+  Tf Tsb;
+  const rmh_transform* Tbm;
+  rmh_outputs out;
+  uint64_t n_rays;  // n_poses * ray_count
+};
+
+struct BatchArgs {
+  MapData map;
+  const BatchEntryDev* entries;
+  const uint32_t* chunk_prefix;  // n_entries + 1
+  uint32_t n_entries;
+  uint32_t total_chunks;
+  uint32_t chunk_rays;           // multiple of 64
+  uint32_t* counter;
+};
+
+template <int KIND, int STACK, int BLOCK>
+__device__ __forceinline__ void batch_ray(const BatchArgs& a, const BatchEntryDev& e, uint64_t r, int (*stk)[BLOCK]) {
+  const uint32_t rc = e.g.ray_count;
+  const uint32_t pose = static_cast<uint32_t>(r / rc);
+  const uint32_t lid = static_cast<uint32_t>(r - static_cast<uint64_t>(pose) * rc);
+  const uint32_t vid = lid / e.g.W;
+  const uint32_t hid = lid - vid * e.g.W;
+  V3 ds, os;
+  gen_ray<KIND>(e.g, vid, hid, lid, ds, os);
+  const Tf Tsm = tmul(load_tf(e.Tbm[pose]), e.Tsb);
+  const V3 dm = qrot(Tsm.R, ds);
+  const V3 om = map_origin<KIND>(Tsm, os);
+  Hit h;
+  trace<STACK, BLOCK>(a.map, om, dm, e.g.range_max, stk, h);
+  write_outputs<KIND>(e.out, r, e.g, a.map, Tsm, ds, os, h);
+}
+
+__global__ void k_reset_counter(uint32_t* counter) {
+  if (threadIdx.x == 0) *counter = 0u;  // a kernel store (not a memset node) keeps graph replays correct
+}
+
+template <int STACK, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_batch(const BatchArgs a) {
+  __shared__ int stk[STACK][BLOCK];
+  const int lane = threadIdx.x & 63;
+  for (;;) {
+    uint32_t c = 0;
+    if (lane == 0) c = atomicAdd(a.counter, 1u);
+    c = __shfl(c, 0, 64);
+    c = __builtin_amdgcn_readfirstlane(c);
+    if (c >= a.total_chunks) break;  // every wave leaves once the counter passes the end
+    uint32_t lo = 0, hi = a.n_entries;
+    while (hi - lo > 1) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (a.chunk_prefix[mid] <= c) lo = mid; else hi = mid;
+    }
+    const BatchEntryDev& e = a.entries[lo];
+    const uint64_t n = e.n_rays;
+    const uint64_t r0 = static_cast<uint64_t>(c - a.chunk_prefix[lo]) * a.chunk_rays;
+    const int kind = e.g.kind;
+    for (uint32_t k = 0; k < a.chunk_rays; k += 64) {
+      const uint64_t r = r0 + k + lane;
+      if (r >= n) break;
+      switch (kind) {
+        case RMH_MODEL_SPHERICAL: batch_ray<RMH_MODEL_SPHERICAL, STACK, BLOCK>(a, e, r, stk); break;
+        case RMH_MODEL_PINHOLE: batch_ray<RMH_MODEL_PINHOLE, STACK, BLOCK>(a, e, r, stk); break;
+        case RMH_MODEL_O1DN: batch_ray<RMH_MODEL_O1DN, STACK, BLOCK>(a, e, r, stk); break;
+        default: batch_ray<RMH_MODEL_ONDN, STACK, BLOCK>(a, e, r, stk); break;
+      }
+    }
+  }
+}
+
+constexpr int kBatchBlock = 64;  // one wave per workgroup: a finished wave frees its LDS at once
+
+template <int STACK>
+hipError_t launch_batch(const BatchArgs& a, uint32_t blocks, hipStream_t stream) {
+  k_reset_counter<<<1, 64, 0, stream>>>(a.counter);
+  k_batch<STACK, kBatchBlock><<<blocks, kBatchBlock, 0, stream>>>(a);
+  return hipGetLastError();
+}
+
 }  // namespace
 
-extern "C" int rmh_simulate(const rmh_map* map, const rmh_sensor* sensor, const rmh_transform* Tsb,
-                            const rmh_transform* Tbm, uint32_t n_poses, const rmh_outputs* out, void* stream_) {
+struct rmh_batch {
+  const rmh_map* map = nullptr;
+  BatchEntryDev* entries = nullptr;
+  uint32_t* chunk_prefix = nullptr;
+  uint32_t* counter = nullptr;
+  uint32_t n_entries = 0;
+  uint32_t total_chunks = 0;
+  uint64_t total_rays = 0;
+  rmh_batch_params params{};
+  uint32_t n_cu = 256;
+  std::vector<uint64_t> entry_rays;
+};
+
+namespace {
+
+void batch_free(rmh_batch* b) {
+  if (!b) return;
+  if (b->entries) (void)hipFree(b->entries);
+  if (b->chunk_prefix) (void)hipFree(b->chunk_prefix);
+  if (b->counter) (void)hipFree(b->counter);
+  delete b;
+}
+
+int batch_upload(rmh_batch* b, const std::vector<BatchEntryDev>& host) {
+  std::vector<uint32_t> prefix(b->n_entries + 1, 0);
+  uint64_t chunks = 0;
+  for (uint32_t i = 0; i < b->n_entries; i++) {
+    prefix[i] = static_cast<uint32_t>(chunks);
+    chunks += (b->entry_rays[i] + b->params.chunk_rays - 1) / b->params.chunk_rays;
+    if (chunks >= (1ull << 31)) return fail(RMH_ERR_UNSUPPORTED, "rmh_batch: too many ray chunks");
+  }
+  prefix[b->n_entries] = static_cast<uint32_t>(chunks);
+  b->total_chunks = static_cast<uint32_t>(chunks);
+  RMH_HIP_TRY(hipMemcpy(b->chunk_prefix, prefix.data(), sizeof(uint32_t) * prefix.size(), hipMemcpyHostToDevice));
+  if (!host.empty())
+    RMH_HIP_TRY(hipMemcpy(b->entries, host.data(), sizeof(BatchEntryDev) * host.size(), hipMemcpyHostToDevice));
+  return RMH_OK;
+}
+
+int check_depth(const rmh_map* map) {
+  if (map->info.max_depth > 128) return fail(RMH_ERR_UNSUPPORTED, "BVH deeper than 128 levels");
+  return RMH_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int rmh_simulate(const rmh_map* map, const rmh_sensor* sensor, const rmh_transform* Tsb, const rmh_transform* Tbm,
+                 uint32_t n_poses, const rmh_outputs* out, void* stream_) {
   // SphereSimulatorOptix.tcc:49-53 throws when no map is set
   if (!map) return fail(RMH_ERR_NOMAP, "rmh_simulate: no map set");
   if (!sensor) return fail(RMH_ERR_INVALID, "rmh_simulate: no sensor model set");
@@ -281,32 +231,16 @@ extern "C" int rmh_simulate(const rmh_map* map, const rmh_sensor* sensor, const 
   const uint64_t ray_count = static_cast<uint64_t>(sensor->width) * sensor->height;
   if (ray_count == 0) return RMH_OK;
   if (ray_count > (1ull << 31)) return fail(RMH_ERR_UNSUPPORTED, "rmh_simulate: more than 2^31 rays per pose");
+  if (int rc = check_depth(map)) return rc;
   const uint32_t depth = map->info.max_depth;
-  if (depth > 128) return fail(RMH_ERR_UNSUPPORTED, "rmh_simulate: BVH deeper than 128 levels");
 
   SimArgs a;
-  a.map.tris = map->tris;
-  a.map.normals = map->normals;
-  a.map.nodes = map->nodes;
-  a.map.root = map->root;
-  a.map.n_faces = map->n_faces;
-  a.W = static_cast<int>(sensor->width);
-  a.H = static_cast<int>(sensor->height);
-  a.range_min = sensor->range.min;
-  a.range_max = sensor->range.max;
-  a.lut = sensor->lut;
-  a.pf0 = sensor->pinhole.f[0];
-  a.pf1 = sensor->pinhole.f[1];
-  a.pc0 = sensor->pinhole.c[0];
-  a.pc1 = sensor->pinhole.c[1];
-  a.orig = v3(sensor->orig.x, sensor->orig.y, sensor->orig.z);
-  a.dirs = sensor->dirs;
-  a.origs = sensor->origs;
+  a.map = make_mapdata(map);
+  a.g = make_raygen(sensor);
   rmh_transform I{};
   I.R.w = 1.0f;
   a.Tsb = load_tf(Tsb ? *Tsb : I);
   a.Tbm = Tbm;
-  a.ray_count = static_cast<uint32_t>(ray_count);
   a.out = *out;
   hipStream_t stream = static_cast<hipStream_t>(stream_);
 
@@ -324,3 +258,149 @@ extern "C" int rmh_simulate(const rmh_map* map, const rmh_sensor* sensor, const 
   }
   return RMH_OK;
 }
+
+int rmh_batch_create(const rmh_map* map, const rmh_batch_entry* entries, uint32_t n_entries,
+                     const rmh_batch_params* params, rmh_batch** out) {
+  if (!out) return fail(RMH_ERR_INVALID, "rmh_batch_create: out is NULL");
+  *out = nullptr;
+  if (!map) return fail(RMH_ERR_NOMAP, "rmh_batch_create: no map set");
+  if (n_entries > 0 && !entries) return fail(RMH_ERR_INVALID, "rmh_batch_create: entries is NULL");
+  if (int rc = check_depth(map)) return rc;
+  rmh_batch* b = new (std::nothrow) rmh_batch();
+  if (!b) return fail(RMH_ERR_ALLOC, "rmh_batch_create: out of host memory");
+  b->map = map;
+  b->n_entries = n_entries;
+  b->params.chunk_rays = 256;
+  b->params.waves_per_cu = 16;
+  if (params) {
+    if (params->chunk_rays) b->params.chunk_rays = params->chunk_rays;
+    if (params->waves_per_cu) b->params.waves_per_cu = params->waves_per_cu;
+  }
+  if (b->params.chunk_rays % 64 != 0 || b->params.chunk_rays > (1u << 20)) {
+    batch_free(b);
+    return fail(RMH_ERR_INVALID, "rmh_batch_create: chunk_rays must be a multiple of 64 (<= 2^20)");
+  }
+  int dev = 0, ncu = 0;
+  (void)hipGetDevice(&dev);
+  if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && ncu > 0)
+    b->n_cu = static_cast<uint32_t>(ncu);
+  std::vector<BatchEntryDev> host(n_entries);
+  b->entry_rays.resize(n_entries);
+  for (uint32_t i = 0; i < n_entries; i++) {
+    const rmh_batch_entry& e = entries[i];
+    if (!e.sensor) { batch_free(b); return fail(RMH_ERR_INVALID, "rmh_batch_create: entry without sensor"); }
+    if (e.n_poses > 0 && !e.Tbm_dev) { batch_free(b); return fail(RMH_ERR_INVALID, "rmh_batch_create: entry without poses"); }
+    host[i].g = make_raygen(e.sensor);
+    host[i].Tsb = load_tf(e.Tsb);
+    host[i].Tbm = e.Tbm_dev;
+    host[i].out = e.out;
+    host[i].n_rays = static_cast<uint64_t>(e.n_poses) * host[i].g.ray_count;
+    b->entry_rays[i] = host[i].n_rays;
+    b->total_rays += host[i].n_rays;
+  }
+  hipError_t err = hipMalloc(&b->entries, sizeof(BatchEntryDev) * std::max<uint32_t>(1, n_entries));
+  if (err == hipSuccess) err = hipMalloc(&b->chunk_prefix, sizeof(uint32_t) * (n_entries + 1));
+  if (err == hipSuccess) err = hipMalloc(&b->counter, sizeof(uint32_t));
+  if (err != hipSuccess) { batch_free(b); return hip_fail(err, "rmh_batch_create"); }
+  if (int rc = batch_upload(b, host)) { batch_free(b); return rc; }
+  *out = b;
+  return RMH_OK;
+}
+
+int rmh_batch_set_params(rmh_batch* b, const rmh_batch_params* params) {
+  if (!b || !params) return fail(RMH_ERR_INVALID, "rmh_batch_set_params: NULL argument");
+  rmh_batch_params p = b->params;
+  if (params->chunk_rays) p.chunk_rays = params->chunk_rays;
+  if (params->waves_per_cu) p.waves_per_cu = params->waves_per_cu;
+  if (p.chunk_rays % 64 != 0 || p.chunk_rays > (1u << 20))
+    return fail(RMH_ERR_INVALID, "rmh_batch_set_params: chunk_rays must be a multiple of 64 (<= 2^20)");
+  if (p.waves_per_cu > 64) return fail(RMH_ERR_INVALID, "rmh_batch_set_params: waves_per_cu must be <= 64");
+  const bool rechunk = p.chunk_rays != b->params.chunk_rays;
+  b->params = p;
+  if (rechunk) {
+    RMH_HIP_TRY(hipDeviceSynchronize());  // the prefix table may be in use by a running launch
+    return batch_upload(b, {});
+  }
+  return RMH_OK;
+}
+
+int rmh_batch_get_info(const rmh_batch* b, uint64_t* total_rays, uint32_t* total_chunks, rmh_batch_params* params) {
+  if (!b) return fail(RMH_ERR_INVALID, "rmh_batch_get_info: NULL batch");
+  if (total_rays) *total_rays = b->total_rays;
+  if (total_chunks) *total_chunks = b->total_chunks;
+  if (params) *params = b->params;
+  return RMH_OK;
+}
+
+int rmh_batch_simulate(rmh_batch* b, void* stream_) {
+  if (!b) return fail(RMH_ERR_INVALID, "rmh_batch_simulate: NULL batch");
+  if (b->total_chunks == 0) return RMH_OK;
+  BatchArgs a;
+  a.map = make_mapdata(b->map);
+  a.entries = b->entries;
+  a.chunk_prefix = b->chunk_prefix;
+  a.n_entries = b->n_entries;
+  a.total_chunks = b->total_chunks;
+  a.chunk_rays = b->params.chunk_rays;
+  a.counter = b->counter;
+  const uint32_t waves = b->n_cu * b->params.waves_per_cu;
+  const uint32_t blocks = std::max<uint32_t>(1, std::min<uint32_t>(waves, b->total_chunks));
+  const uint32_t depth = b->map->info.max_depth;
+  hipStream_t stream = static_cast<hipStream_t>(stream_);
+  hipError_t e;
+  if (depth <= 32) e = launch_batch<32>(a, blocks, stream);
+  else if (depth <= 64) e = launch_batch<64>(a, blocks, stream);
+  else e = launch_batch<128>(a, blocks, stream);
+  if (e != hipSuccess) return hip_fail(e, "rmh_batch_simulate launch");
+  return RMH_OK;
+}
+
+int rmh_batch_destroy(rmh_batch* b) {
+  batch_free(b);
+  return RMH_OK;
+}
+
+// ---- graphs ------------------------------------------------------------------------------
+
+struct rmh_graph {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+int rmh_graph_begin_capture(void* stream) {
+  if (!stream) return fail(RMH_ERR_INVALID, "rmh_graph_begin_capture: the legacy default stream cannot be captured");
+  RMH_HIP_TRY(hipStreamBeginCapture(static_cast<hipStream_t>(stream), hipStreamCaptureModeThreadLocal));
+  return RMH_OK;
+}
+
+int rmh_graph_end_capture(void* stream, rmh_graph** out) {
+  if (!stream || !out) return fail(RMH_ERR_INVALID, "rmh_graph_end_capture: NULL argument");
+  *out = nullptr;
+  rmh_graph* g = new (std::nothrow) rmh_graph();
+  if (!g) return fail(RMH_ERR_ALLOC, "rmh_graph_end_capture: out of host memory");
+  hipError_t e = hipStreamEndCapture(static_cast<hipStream_t>(stream), &g->graph);
+  if (e == hipSuccess) e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);
+  if (e != hipSuccess) {
+    if (g->graph) (void)hipGraphDestroy(g->graph);
+    delete g;
+    return hip_fail(e, "rmh_graph_end_capture");
+  }
+  *out = g;
+  return RMH_OK;
+}
+
+int rmh_graph_launch(rmh_graph* g, void* stream) {
+  if (!g) return fail(RMH_ERR_INVALID, "rmh_graph_launch: NULL graph");
+  RMH_HIP_TRY(hipGraphLaunch(g->exec, static_cast<hipStream_t>(stream)));
+  return RMH_OK;
+}
+
+int rmh_graph_destroy(rmh_graph* g) {
+  if (!g) return RMH_OK;
+  if (g->exec) (void)hipGraphExecDestroy(g->exec);
+  if (g->graph) (void)hipGraphDestroy(g->graph);
+  delete g;
+  return RMH_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,247 @@
+// Device code shared by librmh's traversal kernels (plain simulate, batched simulate,
+// pose adjoint): FP32 ray generation in the reference's operation order, the FP64
+// watertight ray/triangle test, BVH traversal with an LDS stack, and the fused attribute
+// epilogue.  References for each piece are given in rmh_simulate.hip.
+#pragma once
+
+#include "rmh_internal.hpp"
+
+namespace rmh {
+
+// Model parameters of one sensor as the kernels see them.
+struct RayGen {
+  int kind;
+  uint32_t W, H, ray_count;
+  float range_min, range_max;
+  const float2* lut;                 // spherical direction table: H rows then W columns
+  float pf0, pf1, pc0, pc1;          // pinhole
+  V3 orig;                           // O1Dn origin (sensor frame)
+  const rmh_vec3* dirs;              // O1Dn / OnDn
+  const rmh_vec3* origs;             // OnDn
+};
+
+inline RayGen make_raygen(const rmh_sensor* s) {
+  RayGen g;
+  g.kind = s->kind;
+  g.W = s->width;
+  g.H = s->height;
+  g.ray_count = s->width * s->height;
+  g.range_min = s->range.min;
+  g.range_max = s->range.max;
+  g.lut = s->lut;
+  g.pf0 = s->pinhole.f[0];
+  g.pf1 = s->pinhole.f[1];
+  g.pc0 = s->pinhole.c[0];
+  g.pc1 = s->pinhole.c[1];
+  g.orig = v3(s->orig.x, s->orig.y, s->orig.z);
+  g.dirs = s->dirs;
+  g.origs = s->origs;
+  return g;
+}
+
+inline MapData make_mapdata(const rmh_map* m) {
+  MapData d;
+  d.tris = m->tris;
+  d.normals = m->normals;
+  d.nodes = m->nodes;
+  d.root = m->root;
+  d.n_faces = m->n_faces;
+  return d;
+}
+
+__device__ __forceinline__ float sel3(const float4& v, int k) { return k == 0 ? v.x : (k == 1 ? v.y : v.z); }
+
+struct TriRay {
+  double okx, oky, okz;
+  double Sx, Sy, Sz;
+  int kx, ky, kz;
+};
+
+__device__ __forceinline__ TriRay make_tri_ray(const V3& o, const V3& d) {
+  TriRay r;
+  const float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
+  int kz = (ax > ay) ? (ax > az ? 0 : 2) : (ay > az ? 1 : 2);
+  int kx = kz == 2 ? 0 : kz + 1;
+  int ky = kx == 2 ? 0 : kx + 1;
+  const float dkz = kz == 0 ? d.x : (kz == 1 ? d.y : d.z);
+  if (dkz < 0.0f) { const int t = kx; kx = ky; ky = t; }
+  const double dd[3] = {(double)d.x, (double)d.y, (double)d.z};
+  const double oo[3] = {(double)o.x, (double)o.y, (double)o.z};
+  r.kx = kx; r.ky = ky; r.kz = kz;
+  r.Sx = dd[kx] / dd[kz];
+  r.Sy = dd[ky] / dd[kz];
+  r.Sz = 1.0 / dd[kz];
+  r.okx = oo[kx]; r.oky = oo[ky]; r.okz = oo[kz];
+  return r;
+}
+
+// Watertight test in FP64.  Returns t >= 0 of the hit or a negative value for no hit.
+__device__ __forceinline__ double tri_test(const TriRay& r, const float4& v0, const float4& v1, const float4& v2) {
+  const double Akx = (double)sel3(v0, r.kx) - r.okx, Aky = (double)sel3(v0, r.ky) - r.oky, Akz = (double)sel3(v0, r.kz) - r.okz;
+  const double Bkx = (double)sel3(v1, r.kx) - r.okx, Bky = (double)sel3(v1, r.ky) - r.oky, Bkz = (double)sel3(v1, r.kz) - r.okz;
+  const double Ckx = (double)sel3(v2, r.kx) - r.okx, Cky = (double)sel3(v2, r.ky) - r.oky, Ckz = (double)sel3(v2, r.kz) - r.okz;
+  const double Ax = Akx - r.Sx * Akz, Ay = Aky - r.Sy * Akz;
+  const double Bx = Bkx - r.Sx * Bkz, By = Bky - r.Sy * Bkz;
+  const double Cx = Ckx - r.Sx * Ckz, Cy = Cky - r.Sy * Ckz;
+  const double U = Cx * By - Cy * Bx;
+  const double V = Ax * Cy - Ay * Cx;
+  const double W = Bx * Ay - By * Ax;
+  if ((U < 0.0 || V < 0.0 || W < 0.0) && (U > 0.0 || V > 0.0 || W > 0.0)) return -1.0;
+  const double det = U + V + W;
+  if (det == 0.0) return -1.0;
+  const double T = U * (r.Sz * Akz) + V * (r.Sz * Bkz) + W * (r.Sz * Ckz);
+  return T / det;  // NaN or negative => no hit (callers test t >= 0)
+}
+
+struct Hit {
+  double t;
+  uint32_t slot;  // Morton slot of the hit triangle, 0xFFFFFFFF = none
+  uint32_t face;  // global face index (tie break)
+};
+
+template <int STACK, int BLOCK>
+__device__ __forceinline__ void trace(const MapData& m, const V3& o, const V3& d, float range_max,
+                                      int (*stk)[BLOCK], Hit& h) {
+  h.t = (double)range_max;
+  h.slot = 0xFFFFFFFFu;
+  h.face = 0xFFFFFFFFu;
+  if (m.n_faces == 0) return;
+  const int tid = threadIdx.x;
+  const float idx = 1.0f / (fabsf(d.x) > 1e-30f ? d.x : copysignf(1e-30f, d.x));
+  const float idy = 1.0f / (fabsf(d.y) > 1e-30f ? d.y : copysignf(1e-30f, d.y));
+  const float idz = 1.0f / (fabsf(d.z) > 1e-30f ? d.z : copysignf(1e-30f, d.z));
+  const TriRay tr = make_tri_ray(o, d);
+  float tbf = range_max;
+  int ref = m.root;
+  int sp = 0;
+  for (;;) {
+    if (ref >= 0) {
+      const Node2 nd = m.nodes[ref];
+      const float x00 = (nd.a.x - o.x) * idx, x01 = (nd.a.y - o.x) * idx;
+      const float y00 = (nd.a.z - o.y) * idy, y01 = (nd.a.w - o.y) * idy;
+      const float z00 = (nd.c.x - o.z) * idz, z01 = (nd.c.y - o.z) * idz;
+      const float x10 = (nd.b.x - o.x) * idx, x11 = (nd.b.y - o.x) * idx;
+      const float y10 = (nd.b.z - o.y) * idy, y11 = (nd.b.w - o.y) * idy;
+      const float z10 = (nd.c.z - o.z) * idz, z11 = (nd.c.w - o.z) * idz;
+      const float tmin0 = fmaxf(fmaxf(fminf(x00, x01), fminf(y00, y01)), fmaxf(fminf(z00, z01), 0.0f));
+      const float tmax0 = fminf(fminf(fmaxf(x00, x01), fmaxf(y00, y01)), fminf(fmaxf(z00, z01), tbf));
+      const float tmin1 = fmaxf(fmaxf(fminf(x10, x11), fminf(y10, y11)), fmaxf(fminf(z10, z11), 0.0f));
+      const float tmax1 = fminf(fminf(fmaxf(x10, x11), fmaxf(y10, y11)), fminf(fmaxf(z10, z11), tbf));
+      const bool h0 = tmin0 <= tmax0 * 1.0000005f;
+      const bool h1 = tmin1 <= tmax1 * 1.0000005f;
+      if (h0 && h1) {
+        int near = nd.d.x, far = nd.d.y;
+        if (tmin1 < tmin0) { near = nd.d.y; far = nd.d.x; }
+        if (sp < STACK) stk[sp++][tid] = far;
+        ref = near;
+        continue;
+      }
+      if (h0) { ref = nd.d.x; continue; }
+      if (h1) { ref = nd.d.y; continue; }
+    } else {
+      const uint32_t e = ~static_cast<uint32_t>(ref);
+      const uint32_t first = e >> 3, cnt = (e & 7u) + 1u;
+      for (uint32_t k = 0; k < cnt; k++) {
+        const size_t s = static_cast<size_t>(first + k) * 3;
+        const float4 v0 = m.tris[s], v1 = m.tris[s + 1], v2 = m.tris[s + 2];
+        const double t = tri_test(tr, v0, v1, v2);
+        if (t >= 0.0 && t <= h.t) {
+          const uint32_t face = __float_as_uint(v2.w);
+          if (t < h.t || face < h.face) {
+            h.t = t;
+            h.face = face;
+            h.slot = first + k;
+          }
+        }
+      }
+      tbf = __double2float_ru(h.t);
+    }
+    if (sp == 0) break;
+    ref = stk[--sp][tid];
+  }
+}
+
+
+// Sensor-frame ray of (row vid, column hid, buffer id lid).
+template <int KIND>
+__device__ __forceinline__ void gen_ray(const RayGen& a, uint32_t vid, uint32_t hid, uint32_t lid, V3& ds, V3& os) {
+  if constexpr (KIND == RMH_MODEL_SPHERICAL) {
+    // SphericalModel::getDirection (sensor_models.h:170-175)
+    const float2 p = a.lut[vid];
+    const float2 t = a.lut[a.H + hid];
+    ds = v3(p.x * t.x, p.x * t.y, p.y);
+    os = v3(0.0f, 0.0f, 0.0f);
+  } else if constexpr (KIND == RMH_MODEL_PINHOLE) {
+    // PinholeModel::getDirectionOptical / getDirection (sensor_models.h:256-277)
+    const float pX = (static_cast<float>(hid) - a.pc0) / a.pf0;
+    const float pY = (static_cast<float>(vid) - a.pc1) / a.pf1;
+    const V3 dopt = normalize(v3(pX, pY, 1.0f));
+    ds = v3(dopt.z, -dopt.x, -dopt.y);
+    os = v3(0.0f, 0.0f, 0.0f);
+  } else if constexpr (KIND == RMH_MODEL_O1DN) {
+    // O1DnModel_::getDirection / getOrigin (sensor_models.h:376-385)
+    const rmh_vec3 dd = a.dirs[lid];
+    ds = v3(dd.x, dd.y, dd.z);
+    os = a.orig;
+  } else {
+    // OnDnModel_::getDirection / getOrigin (sensor_models.h:460-470)
+    const rmh_vec3 dd = a.dirs[lid];
+    const rmh_vec3 oo = a.origs[lid];
+    ds = v3(dd.x, dd.y, dd.z);
+    os = v3(oo.x, oo.y, oo.z);
+  }
+}
+
+// Ray origin in the map frame: Tsm.t for sphere/pinhole (SphereSimulatorEmbree.tcc:88-90),
+// Tsm * orig for O1Dn/OnDn (O1DnSimulatorEmbree.tcc:86-87).
+template <int KIND>
+__device__ __forceinline__ V3 map_origin(const Tf& Tsm, const V3& os) {
+  if constexpr (KIND == RMH_MODEL_SPHERICAL || KIND == RMH_MODEL_PINHOLE) return Tsm.t;
+  else return tapply(Tsm, os);
+}
+
+// Attribute epilogue (SphereSimulatorEmbree.tcc:104-249, O1DnSimulatorEmbree.tcc:132).
+template <int KIND>
+__device__ __forceinline__ void write_outputs(const rmh_outputs& out, uint64_t glob, const RayGen& g,
+                                              const MapData& map, const Tf& Tsm, const V3& ds, const V3& os,
+                                              const Hit& h) {
+  if (h.slot != 0xFFFFFFFFu) {
+    const float t = static_cast<float>(h.t);
+    if (out.hits) out.hits[glob] = (t >= g.range_min) ? 1 : 0;
+    if (out.ranges) out.ranges[glob] = t;
+    if (out.points) {
+      V3 p = scale(ds, t);
+      if constexpr (KIND == RMH_MODEL_O1DN || KIND == RMH_MODEL_ONDN) p = add(p, os);
+      rmh_vec3 pv; pv.x = p.x; pv.y = p.y; pv.z = p.z;
+      out.points[glob] = pv;
+    }
+    if (out.normals) {
+      // face normal into the sensor frame, flipped toward the sensor, normalised
+      const float4 nn = map.normals[h.slot];
+      V3 n = qrot(qconj(Tsm.R), v3(nn.x, nn.y, nn.z));
+      if (dot(ds, n) > 0.0f) n = scale(n, -1.0f);
+      n = normalize(n);
+      rmh_vec3 nv; nv.x = n.x; nv.y = n.y; nv.z = n.z;
+      out.normals[glob] = nv;
+    }
+    if (out.face_ids || out.geom_ids || out.object_ids) {
+      const size_t s = static_cast<size_t>(h.slot) * 3;
+      const uint32_t face = __float_as_uint(map.tris[s].w);
+      const uint32_t geom = __float_as_uint(map.tris[s + 1].w);
+      if (out.face_ids) out.face_ids[glob] = face;
+      if (out.geom_ids) out.geom_ids[glob] = geom;
+      if (out.object_ids) out.object_ids[glob] = geom;  // no instances: object id = geom id
+    }
+  } else {
+    const float qnan = __builtin_nanf("");
+    if (out.hits) out.hits[glob] = 0;
+    if (out.ranges) out.ranges[glob] = g.range_max + 1.0f;
+    if (out.points) { rmh_vec3 v; v.x = qnan; v.y = qnan; v.z = qnan; out.points[glob] = v; }
+    if (out.normals) { rmh_vec3 v; v.x = qnan; v.y = qnan; v.z = qnan; out.normals[glob] = v; }
+    if (out.face_ids) out.face_ids[glob] = 0xFFFFFFFFu;
+    if (out.geom_ids) out.geom_ids[glob] = 0xFFFFFFFFu;
+    if (out.object_ids) out.object_ids[glob] = 0xFFFFFFFFu;
+  }
+}
+
+}  // namespace rmh

@@ -243,3 +243,103 @@ def simulator_for(model, map: HipMap | None = None, device="cuda") -> SimulatorH
     sim = cls(map, device=device)
     sim.setModel(model)
     return sim
+
+
+class BatchSimulatorHip:
+    """Many sensors -- any mix of models, poses and ray counts -- cast in ONE persistent launch.
+
+    entries: list of dicts {"model": sensor model, "Tbm": (N,8) device tensor,
+    "attrs": attribute names (default hits+ranges), "Tsb": optional (8,) pose}.
+    After simulate(), self.bundles[i] holds entry i's results (the same layout rmh_simulate
+    gives for that sensor alone).  The device table is built once; launches can be captured
+    into a HipGraph.  The reference has no batched form (one optixLaunch per model).
+    """
+
+    def __init__(self, map: HipMap, entries, chunk_rays: int = 0, waves_per_cu: int = 0, device="cuda"):
+        self._map = map
+        self.device = torch.device(device)
+        self._sensors, self._keep, self.bundles = [], [], []
+        table = (T.rmh_batch_entry * max(1, len(entries)))()
+        self.total_rays = 0
+        for i, e in enumerate(entries):
+            sh = _SensorHandle(e["model"], self.device)
+            Tbm = e["Tbm"]
+            if not (isinstance(Tbm, torch.Tensor) and Tbm.dtype == torch.float32 and Tbm.is_contiguous()
+                    and Tbm.dim() == 2 and Tbm.shape[1] == 8 and Tbm.device.type == "cuda"):
+                raise ValueError("entry Tbm must be a contiguous float32 (N,8) device tensor")
+            n = Tbm.shape[0]
+            b = resize_memory_bundle(Bundle(), sh.W, sh.H, n, e.get("attrs", ("hits", "ranges")), device=self.device)
+            outs = T.rmh_outputs()
+            for name in b.enabled():
+                setattr(outs, name, int(getattr(b, name).data_ptr()))
+            Tsb = e.get("Tsb")
+            Tsb = T.identity_transforms(1)[0] if Tsb is None else np.asarray(Tsb, np.float32)
+            table[i] = T.rmh_batch_entry(sh.handle.value, T.transform_to_c(Tsb), int(Tbm.data_ptr()), n, 0, outs)
+            self._sensors.append(sh)
+            self._keep.append(Tbm)
+            self.bundles.append(b)
+            self.total_rays += sh.W * sh.H * n
+        prm = T.rmh_batch_params(chunk_rays, waves_per_cu)
+        self._h = ctypes.c_void_p()
+        _lib.call("rmh_batch_create", map.handle, table, len(entries), ctypes.byref(prm), ctypes.byref(self._h))
+
+    def set_params(self, chunk_rays: int = 0, waves_per_cu: int = 0) -> None:
+        prm = T.rmh_batch_params(chunk_rays, waves_per_cu)
+        _lib.call("rmh_batch_set_params", self._h, ctypes.byref(prm))
+
+    def params(self) -> dict:
+        prm = T.rmh_batch_params()
+        rays = ctypes.c_uint64()
+        chunks = ctypes.c_uint32()
+        _lib.call("rmh_batch_get_info", self._h, ctypes.byref(rays), ctypes.byref(chunks), ctypes.byref(prm))
+        return {"chunk_rays": prm.chunk_rays, "waves_per_cu": prm.waves_per_cu, "total_rays": rays.value,
+                "total_chunks": chunks.value}
+
+    def simulate(self, stream=None) -> list:
+        _lib.call("rmh_batch_simulate", self._h, stream_handle(stream))
+        return self.bundles
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            try:
+                _lib.lib().rmh_batch_destroy(h)
+            except Exception:
+                pass
+            self._h = ctypes.c_void_p()
+
+
+class HipGraph:
+    """A recorded sequence of librmh launches on one stream, replayed with one call.
+
+    Usage: g = HipGraph.capture(lambda s: ..., stream) with a non-default torch stream; the
+    captured function must not allocate (pre-allocate outputs and workspaces).
+    """
+
+    def __init__(self, handle):
+        self._h = handle
+
+    @classmethod
+    def capture(cls, fn, stream: torch.cuda.Stream):
+        if stream is None or int(stream.cuda_stream) == 0:
+            raise ValueError("capture needs a non-default stream")
+        torch.cuda.synchronize()
+        _lib.call("rmh_graph_begin_capture", stream_handle(stream))
+        h = ctypes.c_void_p()
+        try:
+            fn(stream)
+        finally:
+            _lib.call("rmh_graph_end_capture", stream_handle(stream), ctypes.byref(h))
+        return cls(h)
+
+    def launch(self, stream: torch.cuda.Stream) -> None:
+        _lib.call("rmh_graph_launch", self._h, stream_handle(stream))
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            try:
+                _lib.lib().rmh_graph_destroy(h)
+            except Exception:
+                pass
+            self._h = ctypes.c_void_p()

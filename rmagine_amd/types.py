@@ -217,6 +217,15 @@ class rmh_cross_stats(ctypes.Structure):
                 ("covariance", ctypes.c_double * 9), ("n_meas", ctypes.c_uint32), ("_pad", ctypes.c_uint32)]
 
 
+class rmh_batch_entry(ctypes.Structure):
+    _fields_ = [("sensor", ctypes.c_void_p), ("Tsb", rmh_transform), ("Tbm_dev", ctypes.c_void_p),
+                ("n_poses", ctypes.c_uint32), ("_pad", ctypes.c_uint32), ("out", rmh_outputs)]
+
+
+class rmh_batch_params(ctypes.Structure):
+    _fields_ = [("chunk_rays", ctypes.c_uint32), ("waves_per_cu", ctypes.c_uint32)]
+
+
 assert ctypes.sizeof(rmh_cross_stats) == 128
 assert ctypes.sizeof(rmh_transform) == 32
 assert ctypes.sizeof(rmh_spherical_model) == 32
