@@ -40,6 +40,7 @@ def parse():
     ap.add_argument("--sensors", type=int, default=1000)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extra", action="store_true", help="skip the batched / pose-gradient / ICP workloads")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy 5 times (scale factor for HBM counters)")
     return ap.parse_args()
@@ -78,6 +79,186 @@ def cpu_baseline(V, F, model, Tbm, seconds, gpu_numpy_for_pose):
             "sample": f"{poses} of the {len(Tbm)} sensor poses ({rays} rays), all attributes, oracle "
                       f"(exact FP64 predicate, binned-SAH BVH) on {threads} host threads, BVH build excluded"}
     return base, worst
+
+
+def _events():
+    import torch
+    return torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+
+
+def extra_batched(cpu_seconds: float, do_cpu: bool) -> dict:
+    """Heterogeneous batch: 2,000 sensors (40 % VLP-16, 30 % pinhole 640x480, 30 % O1Dn with
+    1k-50k rays) on a 500k-triangle scene, hits + ranges, one persistent launch, tuned."""
+    import torch
+
+    from rmagine_amd import autotune, scenes
+    from rmagine_amd.simulation import BatchSimulatorHip, HipMap, simulator_for
+
+    V, F, sensors = scenes.mixed_sensor_workload()
+    hm = HipMap(V, F)
+    entries = [{"model": m, "Tbm": torch.as_tensor(P, device="cuda"), "attrs": ("hits", "ranges")} for m, P in sensors]
+    batch = BatchSimulatorHip(hm, entries)
+    rays = batch.total_rays
+    mix = {"vlp16": 0, "pinhole": 0, "o1dn": 0}
+    for m, _ in sensors:
+        mix[{0: "vlp16", 1: "pinhole", 2: "o1dn"}[m.kind]] += 1
+    key = autotune.cache_key(len(F), mix)
+    tuned = autotune.tune_batch(batch, key=key, chunks=(64, 256, 1024), waves=(8, 16), reps=2, use_cache=False)
+    ms = autotune.time_batch(batch, reps=5)
+    log(f"batched: {rays} rays, {ms:.2f} ms -> {rays / ms / 1e3:.1f} Mrays/s (tuned {tuned['chunk_rays']}/{tuned['waves_per_cu']})")
+    # the same sensors with one rmh_simulate launch each
+    sims = [simulator_for(m, hm) for m, _ in sensors]
+    bundles = [None] * len(sims)
+    for i, (e, sim) in enumerate(zip(entries, sims)):
+        bundles[i] = sim.simulate(e["Tbm"], attrs=("hits", "ranges"))
+    torch.cuda.synchronize()
+    a, b = _events()
+    a.record()
+    for i, (e, sim) in enumerate(zip(entries, sims)):
+        sim.simulate(e["Tbm"], bundles[i])
+    b.record()
+    b.synchronize()
+    plain_ms = a.elapsed_time(b)
+    # parity of the batched results with the per-sensor launches (both pinned to the oracle)
+    mism = 0
+    for bb, pb in zip(batch.bundles, bundles):
+        mism += int((bb.ranges != pb.ranges).sum().item()) + int((bb.hits != pb.hits).sum().item())
+    out = {"workload": f"{len(F)} tris (random boxes + icospheres, 200 m cube), {len(sensors)} sensors "
+                       f"(800 VLP-16, 600 pinhole 640x480 fov 90, 600 O1Dn 1k-50k rays), hits+ranges",
+           "rays": rays, "batched_ms": round(ms, 3), "batched_mrays_s": round(rays / ms / 1e3, 1),
+           "per_sensor_launches_ms": round(plain_ms, 3), "per_sensor_mrays_s": round(rays / plain_ms / 1e3, 1),
+           "tuned": {k: tuned[k] for k in ("chunk_rays", "waves_per_cu")},
+           "batched_vs_per_sensor_mismatches": mism}
+    if do_cpu:
+        import oracle
+        om = oracle.OracleMap(V, F)
+        rng = np.random.default_rng(0)
+        idx = rng.permutation(len(sensors))
+        crays, cel = 0, 0.0
+        for i in idx:
+            m, P = sensors[i]
+            t = time.perf_counter()
+            oracle.simulate(om, m, P, attrs=("hits", "ranges"), threads=os.cpu_count() or 1, flags=False)
+            cel += time.perf_counter() - t
+            crays += m.size()
+            if cel > cpu_seconds:
+                break
+        cpu = crays / cel / 1e6
+        out["cpu_oracle_mrays_s"] = round(cpu, 2)
+        out["cpu_sample"] = f"{crays} rays of randomly drawn sensors of this batch, {os.cpu_count()} threads"
+        out["batched_over_cpu"] = round(out["batched_mrays_s"] / cpu, 1)
+    return out
+
+
+def extra_pose_grad() -> dict:
+    """10k VLP-16 particles on the ~50k-triangle room: loss + 6-DoF gradient per particle,
+    one batched launch vs one launch per particle."""
+    import torch
+
+    from rmagine_amd import adjoint as A
+    from rmagine_amd import scenes
+    from rmagine_amd.simulation import HipMap, simulator_for
+
+    V, F, T_true, P = scenes.fine_room_workload()
+    hm = HipMap(V, F)
+    sim = simulator_for(scenes.vlp16_workload_model(), hm)
+    res = sim.simulate(torch.as_tensor(T_true, device="cuda"), attrs=("hits", "ranges"))
+    torch.cuda.synchronize()
+    rng = np.random.default_rng(3)
+    target = (res.ranges + torch.as_tensor(rng.normal(scale=0.01, size=res.ranges.shape[0]).astype(np.float32),
+                                           device="cuda")).contiguous()
+    mask = res.hits.contiguous()
+    Pd = torch.as_tensor(P, device="cuda")
+    n = Pd.shape[0]
+    out = torch.empty((n, 8), dtype=torch.float64, device="cuda")
+    ws = torch.empty(A.workspace_bytes(sim, n), dtype=torch.uint8, device="cuda")
+    A.pose_grad(sim, Pd, target, mask, out=out, workspace=ws)
+    torch.cuda.synchronize()
+    a, b = _events()
+    reps = 5
+    a.record()
+    for _ in range(reps):
+        A.pose_grad(sim, Pd, target, mask, out=out, workspace=ws)
+    b.record()
+    b.synchronize()
+    batched = a.elapsed_time(b) / reps
+    outs = [torch.empty((1, 8), dtype=torch.float64, device="cuda") for _ in range(n)]
+    ws1 = torch.empty(A.workspace_bytes(sim, 1), dtype=torch.uint8, device="cuda")
+    for i in range(min(n, 100)):
+        A.pose_grad(sim, Pd[i:i + 1], target, mask, out=outs[i], workspace=ws1)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    a.record()
+    for i in range(n):
+        A.pose_grad(sim, Pd[i:i + 1], target, mask, out=outs[i], workspace=ws1)
+    b.record()
+    b.synchronize()
+    plain = (time.perf_counter() - t) * 1e3
+    same = bool(torch.equal(torch.cat(outs), out))
+    return {"workload": f"{len(F)} tris room, {n} VLP-16 particles (sigma 0.2 m, 3 deg), Huber(0.05) on "
+                        "true-pose ranges + N(0, 0.01 m)",
+            "rays_per_eval": n * 14400, "batched_ms_per_eval": round(batched, 3),
+            "batched_evals_s": round(1e3 / batched, 2), "plain_ms_per_eval": round(plain, 3),
+            "plain_evals_s": round(1e3 / plain, 3), "batched_over_plain": round(plain / batched, 1),
+            "batched_equals_plain": same}
+
+
+def extra_icp() -> dict:
+    """2,000 pose hypotheses x 32,768-ray O1Dn scans on the 500k-triangle scene: 10 RCC
+    iterations (simulate, 5 x (p2l statistics FP64, Umeyama FP64, pose update), pose update)
+    recorded once into a HIP graph and replayed."""
+    import torch
+
+    from rmagine_amd import icp, scenes
+    from rmagine_amd import types as T
+    from rmagine_amd.simulation import HipMap, simulator_for
+
+    V, F = scenes.objects_mesh()
+    hm = HipMap(V, F)
+    rng = np.random.default_rng(4)
+    model = T.O1DnModel(32768, 1, T.Interval(0.1, 100.0), (0.0, 0.0, 0.0), scenes.random_unit_dirs(32768, rng))
+    sim = simulator_for(model, hm)
+    T_true = T.make_transforms([[0.0, 0.0, 0.0]], [[0.0, 0.0, 0.0, 1.0]])
+    data = sim.simulate(torch.as_tensor(T_true, device="cuda"), attrs=("hits", "points"))
+    n = 2000
+    hyp = np.repeat(T_true, n, 0).copy()
+    hyp[:, 4:7] += rng.normal(scale=0.2, size=(n, 3)).astype(np.float32)
+    for i in range(n):
+        a = 0.5 * rng.normal(scale=np.deg2rad(2.0))
+        hyp[i, 0:4] = [0.0, 0.0, np.sin(a), np.cos(a)]
+    # ray-cast correspondences between different poses land on different objects of this
+    # sparse scene unless they are gated: 0.5 m point-to-plane gate
+    rcc = icp.BatchedRCC(hm, model, data.points, data.hits, torch.as_tensor(hyp, device="cuda"), max_dist=0.5)
+    s = torch.cuda.Stream()
+    rcc.capture(10, s)
+    torch.cuda.synchronize()
+    a, b = _events()
+    a.record(s)
+    rcc.graph.launch(s)
+    b.record(s)
+    b.synchronize()
+    ms = a.elapsed_time(b)
+    err0 = np.linalg.norm(hyp[:, 4:7], axis=1)
+    err1 = np.linalg.norm(rcc.Tbm.cpu().numpy()[:, 4:7], axis=1)
+    # FP64 Umeyama of the last statistics against the FP64 CPU oracle (LAPACK SVD)
+    from oracle import stats as S
+    from rmagine_amd import math as M
+    out64 = torch.empty((n, 7), dtype=torch.float64, device="cuda")
+    M.umeyama_transform(rcc.stats, out64=out64)
+    st = M.stats_to_numpy(rcc.stats)
+    g64 = out64.cpu().numpy()
+    umerr = 0.0
+    for i in range(0, n, 10):
+        ref = S.umeyama({k: st[k][i] for k in ("dataset_mean", "model_mean", "covariance", "n_meas")})
+        sgn = 1.0 if np.dot(ref[:4], g64[i, :4]) >= 0 else -1.0
+        umerr = max(umerr, float(np.abs(np.r_[g64[i, :4] * sgn, g64[i, 4:]] - ref).max()))
+    return {"workload": f"{len(F)} tris, {n} hypotheses x 32768 O1Dn rays, 10 x (simulate + 5 x (p2l FP64 "
+                        "stats + FP64 Umeyama + update)), one HIP graph",
+            "graph_ms_10_iterations": round(ms, 3), "ms_per_iteration": round(ms / 10, 3),
+            "simulated_mrays_s": round(10 * n * 32768 / ms / 1e3, 1),
+            "median_translation_error_m": {"before": round(float(np.median(err0)), 4),
+                                           "after": round(float(np.median(err1)), 6)},
+            "umeyama_fp64_max_abs_err_vs_cpu_oracle": umerr}
 
 
 def main():
@@ -184,6 +365,17 @@ def main():
     }
     if cpu:
         out["gpu_over_cpu"] = round(value / cpu["value"], 1)
+    if not args.no_extra:
+        extra = {}
+        del res
+        torch.cuda.empty_cache()
+        for name, fn in (("batched_heterogeneous", lambda: extra_batched(8.0, not args.no_cpu)),
+                         ("pose_gradient", extra_pose_grad), ("icp_graph", extra_icp)):
+            t0 = time.time()
+            extra[name] = fn()
+            log(f"{name}: {extra[name]} ({time.time() - t0:.1f} s)")
+            torch.cuda.empty_cache()
+        out["extra"] = extra
     print(json.dumps(out))
 
 

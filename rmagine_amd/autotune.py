@@ -1,0 +1,96 @@
+"""Autotuner for the batched persistent launch (rmh_batch_*).
+
+The persistent kernel has two launch parameters: rays handed out per work grab
+(`chunk_rays`, a multiple of 64) and resident waves per compute unit (`waves_per_cu`,
+which sets the persistent grid).  Small chunks balance uneven sensors better, large ones
+cost fewer atomics; more waves hide more latency until LDS and registers run out.  The
+tuner times every candidate on the caller's actual batch with HIP events and keeps the
+fastest.  Results are cached as JSON keyed by device architecture, mesh size (power of
+two bucket of the face count) and sensor mix, in $RMH_TUNE_CACHE or
+~/.cache/rmagine_amd/autotune.json.
+
+The reference has no counterpart (its launch shape is fixed by optixLaunch).
+"""
+from __future__ import annotations
+
+import json
+import math
+import os
+from pathlib import Path
+
+import torch
+
+DEFAULT_CHUNKS = (64, 128, 256, 512, 1024)
+DEFAULT_WAVES = (8, 12, 16, 20)
+
+
+def cache_path() -> Path:
+    p = os.environ.get("RMH_TUNE_CACHE")
+    return Path(p) if p else Path.home() / ".cache" / "rmagine_amd" / "autotune.json"
+
+
+def cache_key(n_faces: int, model_mix: dict, device=None) -> str:
+    props = torch.cuda.get_device_properties(device or torch.cuda.current_device())
+    arch = getattr(props, "gcnArchName", props.name)
+    bucket = int(round(math.log2(max(1, n_faces))))
+    mix = ",".join(f"{k}:{model_mix[k]}" for k in sorted(model_mix))
+    return f"{arch}|faces~2^{bucket}|{mix}"
+
+
+def _load(path: Path) -> dict:
+    try:
+        return json.loads(path.read_text())
+    except (OSError, ValueError):
+        return {}
+
+
+def _store(path: Path, data: dict) -> None:
+    try:
+        path.parent.mkdir(parents=True, exist_ok=True)
+        tmp = path.with_suffix(".tmp")
+        tmp.write_text(json.dumps(data, indent=1, sort_keys=True))
+        os.replace(tmp, path)
+    except OSError:
+        pass  # a read-only home is not an error; the tuned values are still applied
+
+
+def time_batch(batch, reps: int = 3, stream=None) -> float:
+    """Median milliseconds of batch.simulate() over `reps` launches (after one warm-up)."""
+    stream = stream or torch.cuda.current_stream()
+    batch.simulate(stream=stream)
+    ms = []
+    for _ in range(reps):
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        batch.simulate(stream=stream)
+        b.record(stream)
+        b.synchronize()
+        ms.append(a.elapsed_time(b))
+    ms.sort()
+    return ms[len(ms) // 2]
+
+
+def tune_batch(batch, key: str | None = None, chunks=DEFAULT_CHUNKS, waves=DEFAULT_WAVES, reps: int = 3,
+               use_cache: bool = True, stream=None) -> dict:
+    """Picks (chunk_rays, waves_per_cu) for `batch`, applies it and returns the record."""
+    path = cache_path()
+    if use_cache and key:
+        rec = _load(path).get(key)
+        if rec:
+            batch.set_params(chunk_rays=rec["chunk_rays"], waves_per_cu=rec["waves_per_cu"])
+            return dict(rec, cached=True)
+    trials = []
+    for c in chunks:
+        for w in waves:
+            batch.set_params(chunk_rays=c, waves_per_cu=w)
+            trials.append((time_batch(batch, reps, stream), c, w))
+    trials.sort()
+    ms, c, w = trials[0]
+    batch.set_params(chunk_rays=c, waves_per_cu=w)
+    rec = {"chunk_rays": c, "waves_per_cu": w, "ms": ms,
+           "trials": [{"chunk_rays": tc, "waves_per_cu": tw, "ms": tm} for tm, tc, tw in trials]}
+    if key:
+        data = _load(path)
+        data[key] = {k: rec[k] for k in ("chunk_rays", "waves_per_cu", "ms")}
+        _store(path, data)
+    return dict(rec, cached=False)

@@ -255,3 +255,121 @@ def room_workload(n_sensors: int = 100, n_boxes: int = 200, seed: int = SEED_BAS
     V, F, boxes = room_with_boxes(n_boxes, seed=seed)
     Tbm = poses_in_room(n_sensors, boxes, seed=seed)
     return V, F, Tbm
+
+
+def _random_rotation(rng) -> np.ndarray:
+    q = rng.normal(size=4)
+    q /= np.linalg.norm(q)
+    x, y, z, w = q
+    return np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - w * z), 2 * (x * z + w * y)],
+                     [2 * (x * y + w * z), 1 - 2 * (x * x + z * z), 2 * (y * z - w * x)],
+                     [2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)]])
+
+
+def objects_mesh(n_objects: int = 5000, n_ico: int = 1429, extent: float = 200.0, seed: int = SEED_BASE + 2):
+    """Random rotated boxes (12 tris) and subdivision-2 icospheres (320 tris) in a cube.
+
+    The defaults give 3,571 boxes + 1,429 icospheres = 500,132 triangles in a 200 m cube,
+    object sizes U[0.5, 5] m.
+    """
+    rng = np.random.default_rng(seed)
+    iv, if_ = icosphere(2)
+    cv, cf = genCube()
+    kinds = np.zeros(n_objects, dtype=bool)
+    kinds[rng.choice(n_objects, n_ico, replace=False)] = True
+    parts = []
+    for is_ico in kinds:
+        R = _random_rotation(rng)
+        s = rng.uniform(0.5, 5.0, size=3)
+        c = rng.uniform(-extent / 2, extent / 2, size=3)
+        v, f = (iv, if_) if is_ico else (cv, cf)
+        vv = (v.astype(np.float64) * (s / (2.0 if is_ico else 1.0))) @ R.T + c
+        parts.append((vv.astype(np.float32), f))
+    return _concat(parts)
+
+
+def random_unit_dirs(n: int, rng) -> np.ndarray:
+    d = rng.normal(size=(n, 3))
+    return (d / np.linalg.norm(d, axis=1, keepdims=True)).astype(np.float32)
+
+
+def pinhole_640x480_fov90() -> T.PinholeModel:
+    """640 x 480 depth camera, 90 deg horizontal field of view: fx = fy = 320, c = (319.5, 239.5)."""
+    return T.PinholeModel(width=640, height=480, range=T.Interval(0.1, 100.0), f=(320.0, 320.0), c=(319.5, 239.5))
+
+
+def mixed_sensor_workload(n_sensors: int = 2000, extent: float = 200.0, seed: int = SEED_BASE + 2,
+                          o1dn_range=(1000, 50000)):
+    """Heterogeneous batch: 40 % VLP-16, 30 % pinhole 640x480 (fov 90), 30 % O1Dn with
+    N ~ U[1k, 50k] random unit directions; one random pose per sensor inside the cube.
+
+    Returns (V, F, entries) with entries = list of (model, pose (1, 8) float32).
+    """
+    V, F = objects_mesh(extent=extent, seed=seed)
+    rng = np.random.default_rng(seed + 7919)
+    n_vlp = int(round(0.4 * n_sensors))
+    n_pin = int(round(0.3 * n_sensors))
+    n_o1 = n_sensors - n_vlp - n_pin
+    vlp = vlp16_workload_model()
+    pin = pinhole_640x480_fov90()
+    models = [vlp] * n_vlp + [pin] * n_pin
+    for _ in range(n_o1):
+        n = int(rng.integers(o1dn_range[0], o1dn_range[1] + 1))
+        models.append(T.O1DnModel(n, 1, T.Interval(0.1, 100.0), (0.0, 0.0, 0.0), random_unit_dirs(n, rng)))
+    entries = []
+    for m in models:
+        q = rng.normal(size=4)
+        q /= np.linalg.norm(q)
+        t = rng.uniform(-extent / 2, extent / 2, size=3)
+        entries.append((m, T.make_transforms(t, q)))
+    return V, F, entries
+
+
+def tessellated_box(lo, hi, n: int):
+    """Axis-aligned box with every face split into an n x n grid (12 n^2 triangles)."""
+    lo = np.asarray(lo, np.float64)
+    hi = np.asarray(hi, np.float64)
+    g = np.linspace(0.0, 1.0, n + 1)
+    U, W = np.meshgrid(g, g, indexing="ij")
+    parts = []
+    for axis in range(3):
+        a1, a2 = (axis + 1) % 3, (axis + 2) % 3
+        for side in (0.0, 1.0):
+            P = np.zeros((n + 1, n + 1, 3))
+            P[..., axis] = lo[axis] + side * (hi[axis] - lo[axis])
+            P[..., a1] = lo[a1] + U * (hi[a1] - lo[a1])
+            P[..., a2] = lo[a2] + W * (hi[a2] - lo[a2])
+            idx = np.arange((n + 1) ** 2).reshape(n + 1, n + 1)
+            a, b, c, d = idx[:-1, :-1].ravel(), idx[1:, :-1].ravel(), idx[:-1, 1:].ravel(), idx[1:, 1:].ravel()
+            f = np.concatenate([np.stack([a, b, d], 1), np.stack([a, d, c], 1)])
+            parts.append((P.reshape(-1, 3).astype(np.float32), f.astype(np.uint32)))
+    return _concat(parts)
+
+
+def fine_room_workload(n_particles: int = 10000, seed: int = SEED_BASE + 3):
+    """Room with boxes tessellated to ~50k triangles, a true VLP-16 pose and particles
+    perturbed by sigma = 0.2 m (x, y) and 3 deg (yaw).
+
+    Room walls 58 x 58 grids (40,368 tris) + 200 boxes 2 x 2 grids (9,600 tris) = 49,968.
+    Returns (V, F, T_true (1,8), particles (n,8)).
+    """
+    rng = np.random.default_rng(seed)
+    _, _, boxes = room_with_boxes(seed=SEED_BASE + 0)
+    parts = [tessellated_box((-10.0, -10.0, 0.0), (10.0, 10.0, 4.0), 58)]
+    for b in boxes:
+        parts.append(tessellated_box(b[0:3], b[3:6], 2))
+    V, F = _concat(parts)
+    T_true = poses_in_room(1, boxes, seed=seed)
+    xy = rng.normal(scale=0.2, size=(n_particles, 2))
+    yaw = rng.normal(scale=np.deg2rad(3.0), size=n_particles)
+    P = np.repeat(T_true, n_particles, 0).copy()
+    P[:, 4:6] += xy.astype(np.float32)
+    q0 = T_true[0, 0:4].astype(np.float64)
+    for i in range(n_particles):
+        h = 0.5 * yaw[i]
+        qz = np.array([0.0, 0.0, np.sin(h), np.cos(h)])
+        x1, y1, z1, w1 = qz
+        x2, y2, z2, w2 = q0
+        P[i, 0:4] = [w1 * x2 + x1 * w2 + y1 * z2 - z1 * y2, w1 * y2 - x1 * z2 + y1 * w2 + z1 * x2,
+                     w1 * z2 + x1 * y2 - y1 * x2 + z1 * w2, w1 * w2 - x1 * x2 - y1 * y2 - z1 * z2]
+    return V, F, T_true, P
