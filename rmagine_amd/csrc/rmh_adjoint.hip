@@ -13,7 +13,8 @@
 //   dr/dtau   = g,                      g = -n / (n . d)
 //   dr/domega = (o + r d - t) x g       (lever arm from the base origin to the hit point)
 // and dL/d(tau, omega) = sum_k rho'(e_k) dr_k/d(tau, omega).  Per ray in FP64, reduced per
-// pose in FP64 (wave shuffles, LDS, then the per-pose chunks in a fixed order).
+// pose in FP64 (wave shuffles within each 64-ray tile, then the tiles of a pose in a fixed
+// order).
 //
 // Output per pose: 8 doubles {L, dL/dtau (3), dL/domega (3), number of rays used}.
 #include "rmh_trace.hpp"
@@ -34,19 +35,21 @@ struct GradArgs {
   const uint8_t* target_mask;
   uint32_t target_stride;     // 0: one target scan shared by all poses; else ray_count
   double delta;
-  uint32_t chunks;            // blocks per pose
-  double* partial;            // (pose, chunk) -> 8 doubles
+  Tiling tl;                  // one 64-ray tile per workgroup; chunks = tiles per pose
+  double* partial;            // (pose, tile) -> 8 doubles
 };
 
-template <int KIND, int STACK, int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_pose_grad(const GradArgs a) {
-  __shared__ int stk[STACK][BLOCK];
-  __shared__ double red[BLOCK / 64][8];
+constexpr int kWave = 64;
+
+template <int KIND, int STACK>
+__global__ __launch_bounds__(kWave) void k_pose_grad(const GradArgs a) {
+  __shared__ int stk[kLdsStack][kWave];
   const uint32_t pose = a.pose_offset + blockIdx.y;
-  const uint32_t lid = blockIdx.x * BLOCK + threadIdx.x;
+  uint32_t vid, hid;
+  const bool active = tile_ray(a.tl, a.g.W, a.g.H, blockIdx.x, threadIdx.x, vid, hid);
   double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  if (lid < a.g.ray_count) {
-    const uint32_t vid = lid / a.g.W, hid = lid - vid * a.g.W;
+  if (active) {
+    const uint32_t lid = vid * a.g.W + hid;
     V3 ds, os;
     gen_ray<KIND>(a.g, vid, hid, lid, ds, os);
     const Tf Tbm = load_tf(a.Tbm[pose]);
@@ -54,7 +57,7 @@ __global__ __launch_bounds__(BLOCK) void k_pose_grad(const GradArgs a) {
     const V3 dm = qrot(Tsm.R, ds);
     const V3 om = map_origin<KIND>(Tsm, os);
     Hit h;
-    trace<STACK, BLOCK>(a.map, om, dm, a.g.range_max, stk, h);
+    trace<STACK, kWave>(a.map, om, dm, a.g.range_max, stk, h);
     const size_t ti = static_cast<size_t>(pose) * a.target_stride + lid;
     const float tr = a.target[ti];
     const bool tvalid = a.target_mask ? (a.target_mask[ti] != 0) : (tr >= a.g.range_min && tr <= a.g.range_max);
@@ -88,14 +91,9 @@ __global__ __launch_bounds__(BLOCK) void k_pose_grad(const GradArgs a) {
     for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
     acc[k] = x;
   }
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  if (lane == 0)
-    for (int k = 0; k < 8; k++) red[w][k] = acc[k];
-  __syncthreads();
-  if (threadIdx.x < 8) {
-    double s = red[0][threadIdx.x];
-    for (int j = 1; j < BLOCK / 64; j++) s += red[j][threadIdx.x];
-    a.partial[(static_cast<size_t>(pose) * a.chunks + blockIdx.x) * 8 + threadIdx.x] = s;
+  if (threadIdx.x == 0) {
+    double* p = a.partial + (static_cast<size_t>(pose) * a.tl.tiles + blockIdx.x) * 8;
+    for (int k = 0; k < 8; k++) p[k] = acc[k];
   }
 }
 
@@ -108,23 +106,21 @@ __global__ void k_pose_grad_combine(const double* partial, uint32_t chunks, uint
   out[i] = s;
 }
 
-template <int KIND, int STACK, int BLOCK>
+template <int KIND, int STACK>
 hipError_t launch_grad(const GradArgs& a, uint32_t np, hipStream_t stream) {
-  k_pose_grad<KIND, STACK, BLOCK><<<dim3(a.chunks, np), BLOCK, 0, stream>>>(a);
+  k_pose_grad<KIND, STACK><<<dim3(a.tl.tiles, np), kWave, 0, stream>>>(a);
   return hipGetLastError();
 }
 
-template <int STACK, int BLOCK>
+template <int STACK>
 hipError_t launch_grad_stack(int kind, const GradArgs& a, uint32_t np, hipStream_t stream) {
   switch (kind) {
-    case RMH_MODEL_SPHERICAL: return launch_grad<RMH_MODEL_SPHERICAL, STACK, BLOCK>(a, np, stream);
-    case RMH_MODEL_PINHOLE: return launch_grad<RMH_MODEL_PINHOLE, STACK, BLOCK>(a, np, stream);
-    case RMH_MODEL_O1DN: return launch_grad<RMH_MODEL_O1DN, STACK, BLOCK>(a, np, stream);
-    default: return launch_grad<RMH_MODEL_ONDN, STACK, BLOCK>(a, np, stream);
+    case RMH_MODEL_SPHERICAL: return launch_grad<RMH_MODEL_SPHERICAL, STACK>(a, np, stream);
+    case RMH_MODEL_PINHOLE: return launch_grad<RMH_MODEL_PINHOLE, STACK>(a, np, stream);
+    case RMH_MODEL_O1DN: return launch_grad<RMH_MODEL_O1DN, STACK>(a, np, stream);
+    default: return launch_grad<RMH_MODEL_ONDN, STACK>(a, np, stream);
   }
 }
-
-uint32_t grad_block(uint32_t depth) { return depth <= 32 ? 256u : (depth <= 64 ? 128u : 64u); }
 
 struct Scratch {
   void* p = nullptr;
@@ -137,10 +133,8 @@ extern "C" {
 
 int rmh_pose_grad_workspace_size(const rmh_map* map, const rmh_sensor* sensor, uint32_t n_poses, size_t* bytes) {
   if (!map || !sensor || !bytes) return fail(RMH_ERR_INVALID, "rmh_pose_grad_workspace_size: NULL argument");
-  const uint32_t blk = grad_block(map->info.max_depth);
-  const uint64_t rc = static_cast<uint64_t>(sensor->width) * sensor->height;
-  const uint64_t chunks = (rc + blk - 1) / blk;
-  *bytes = sizeof(double) * 8 * chunks * n_poses;
+  const Tiling tl = make_tiling(sensor->width, sensor->height);
+  *bytes = sizeof(double) * 8 * static_cast<uint64_t>(tl.tiles) * n_poses;
   return RMH_OK;
 }
 
@@ -157,7 +151,6 @@ int rmh_pose_grad(const rmh_map* map, const rmh_sensor* sensor, const rmh_transf
   const uint64_t rc = static_cast<uint64_t>(sensor->width) * sensor->height;
   if (rc == 0 || rc > (1ull << 30)) return fail(RMH_ERR_UNSUPPORTED, "rmh_pose_grad: bad ray count");
   const uint32_t depth = map->info.max_depth;
-  const uint32_t blk = grad_block(depth);
   GradArgs a;
   a.map = make_mapdata(map);
   a.g = make_raygen(sensor);
@@ -169,8 +162,9 @@ int rmh_pose_grad(const rmh_map* map, const rmh_sensor* sensor, const rmh_transf
   a.target_mask = target_mask;
   a.target_stride = per_pose_targets ? static_cast<uint32_t>(rc) : 0u;
   a.delta = huber_delta;
-  a.chunks = static_cast<uint32_t>((rc + blk - 1) / blk);
-  const size_t need = sizeof(double) * 8 * static_cast<size_t>(a.chunks) * n_poses;
+  a.tl = make_tiling(sensor->width, sensor->height);
+  if (a.tl.tiles > 65535u) return fail(RMH_ERR_UNSUPPORTED, "rmh_pose_grad: more than 65535 ray tiles per pose");
+  const size_t need = sizeof(double) * 8 * static_cast<size_t>(a.tl.tiles) * n_poses;
   Scratch scratch;
   if (workspace) {
     if (workspace_bytes < need) return fail(RMH_ERR_INVALID, "rmh_pose_grad: workspace too small");
@@ -184,12 +178,12 @@ int rmh_pose_grad(const rmh_map* map, const rmh_sensor* sensor, const rmh_transf
     const uint32_t np = std::min<uint32_t>(65535u, n_poses - p0);
     a.pose_offset = p0;
     hipError_t e;
-    if (depth <= 32) e = launch_grad_stack<32, 256>(sensor->kind, a, np, stream);
-    else if (depth <= 64) e = launch_grad_stack<64, 128>(sensor->kind, a, np, stream);
-    else e = launch_grad_stack<128, 64>(sensor->kind, a, np, stream);
+    if (depth <= 32) e = launch_grad_stack<32>(sensor->kind, a, np, stream);
+    else if (depth <= 64) e = launch_grad_stack<64>(sensor->kind, a, np, stream);
+    else e = launch_grad_stack<128>(sensor->kind, a, np, stream);
     if (e != hipSuccess) return hip_fail(e, "rmh_pose_grad launch");
   }
-  k_pose_grad_combine<<<(n_poses * 8u + 255u) / 256u, 256, 0, stream>>>(a.partial, a.chunks, n_poses, out);
+  k_pose_grad_combine<<<(n_poses * 8u + 255u) / 256u, 256, 0, stream>>>(a.partial, a.tl.tiles, n_poses, out);
   RMH_HIP_TRY(hipGetLastError());
   if (scratch.p) RMH_HIP_TRY(hipStreamSynchronize(stream));
   return RMH_OK;

@@ -12,7 +12,7 @@
 #include <hip/hip_runtime.h>
 #include <cstdint>
 #include <string>
-#include "../../include/rmh.h"
+#include <rmh.h>
 
 #define RMH_HD __host__ __device__ __forceinline__
 

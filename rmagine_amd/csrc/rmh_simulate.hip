@@ -6,12 +6,12 @@
 // simulators (src/rmagine_embree/include/rmagine/simulation/SphereSimulatorEmbree.tcc:55-253,
 // PinholeSimulatorEmbree.tcc, O1DnSimulatorEmbree.tcc:50-254, OnDnSimulatorEmbree.tcc).
 //
-// One lane per ray, wave64.  Rays of a wave are consecutive columns of one sensor row,
-// so a wave traverses coherently.  Per lane:
+// One lane per ray, wave64, one wave per workgroup.  A wave casts an 8 x 8 tile of one
+// sensor image (1 x 64 for O1Dn/OnDn lists), so its rays stay close in direction.  Per lane:
 //   * ray set-up in FP32 in the reference's operation order (contraction off);
-//   * traversal of the binary BVH with an LDS stack (one slot column per lane, so the
-//     64 lanes of a wave hit 64 different banks), near child first, conservative FP32
-//     slab tests against outward-padded boxes;
+//   * while-while traversal of the binary BVH with an LDS stack (one slot column per
+//     lane, so the 64 lanes of a wave hit 64 different banks), near child first,
+//     conservative FP32 slab tests against outward-padded boxes;
 //   * watertight ray/triangle test (Woop, Benthin, Wald, JCGT 2013) evaluated in FP64:
 //     the decision and the distance are exact to FP64 rounding, which makes hit masks,
 //     face ids and ranges agree with an FP64 evaluation of the same predicate; ties at
@@ -19,10 +19,10 @@
 //   * fused attribute epilogue, writing only the requested attributes.
 //
 // Two launch forms:
-//   rmh_simulate        one sensor model, n poses, a 1-D grid over all rays;
+//   rmh_simulate        one sensor model, n poses, one workgroup per (pose, tile);
 //   rmh_batch_simulate  many sensors (any mix of models, poses and ray counts) in ONE
-//                       persistent launch: waves grab chunks of rays of one sensor from an
-//                       atomic counter until the work is exhausted.  All pointers and
+//                       persistent launch: waves grab chunks of tiles of one sensor from
+//                       an atomic counter until the work is exhausted.  All pointers and
 //                       sizes live in a device table built once, so the launch (counter
 //                       reset kernel + persistent kernel) can be captured into a graph.
 // The reference has no batched form; its closest piece is the per-launch model union
@@ -38,27 +38,30 @@ namespace {
 
 using namespace rmh;
 
+// One wave per workgroup: a wave that finishes frees its LDS stack at once, so
+// occupancy follows the live waves instead of the slowest wave of a larger workgroup.
+constexpr int kWave = 64;
+
 struct SimArgs {
   MapData map;
   RayGen g;
   Tf Tsb;
   const rmh_transform* Tbm;
-  uint32_t n_rays;                   // rays in this launch (< 2^31)
   uint32_t pose_offset;              // first pose of this launch
+  Tiling tl;
   rmh_outputs out;
 };
 
-template <int KIND, int STACK, int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_simulate(const SimArgs a) {
-  __shared__ int stk[STACK][BLOCK];
-  const uint32_t gid = blockIdx.x * BLOCK + threadIdx.x;
-  if (gid >= a.n_rays) return;
+template <int KIND, int STACK>
+__global__ __launch_bounds__(kWave) void k_simulate(const SimArgs a) {
+  __shared__ int stk[kLdsStack][kWave];
+  const uint32_t pl = blockIdx.x / a.tl.tiles;
+  const uint32_t tile = blockIdx.x - pl * a.tl.tiles;
+  uint32_t vid, hid;
+  if (!tile_ray(a.tl, a.g.W, a.g.H, tile, threadIdx.x, vid, hid)) return;
   const uint32_t rc = a.g.ray_count;
-  const uint32_t pl = gid / rc;
-  const uint32_t lid = gid - pl * rc;
+  const uint32_t lid = vid * a.g.W + hid;
   const uint32_t pose = a.pose_offset + pl;
-  const uint32_t vid = lid / a.g.W;
-  const uint32_t hid = lid - vid * a.g.W;
   const uint64_t glob = static_cast<uint64_t>(pose) * rc + lid;
 
   V3 ds, os;
@@ -67,24 +70,23 @@ __global__ __launch_bounds__(BLOCK) void k_simulate(const SimArgs a) {
   const V3 dm = qrot(Tsm.R, ds);
   const V3 om = map_origin<KIND>(Tsm, os);
   Hit h;
-  trace<STACK, BLOCK>(a.map, om, dm, a.g.range_max, stk, h);
+  trace<STACK, kWave>(a.map, om, dm, a.g.range_max, stk, h);
   write_outputs<KIND>(a.out, glob, a.g, a.map, Tsm, ds, os, h);
 }
 
-template <int KIND, int STACK, int BLOCK>
-hipError_t launch_kind(const SimArgs& a, hipStream_t stream) {
-  const uint32_t blocks = (a.n_rays + BLOCK - 1) / BLOCK;
-  k_simulate<KIND, STACK, BLOCK><<<blocks, BLOCK, 0, stream>>>(a);
+template <int KIND, int STACK>
+hipError_t launch_kind(const SimArgs& a, uint32_t n_poses, hipStream_t stream) {
+  k_simulate<KIND, STACK><<<n_poses * a.tl.tiles, kWave, 0, stream>>>(a);
   return hipGetLastError();
 }
 
-template <int STACK, int BLOCK>
-hipError_t launch_stack(int kind, const SimArgs& a, hipStream_t stream) {
+template <int STACK>
+hipError_t launch_stack(int kind, const SimArgs& a, uint32_t n_poses, hipStream_t stream) {
   switch (kind) {
-    case RMH_MODEL_SPHERICAL: return launch_kind<RMH_MODEL_SPHERICAL, STACK, BLOCK>(a, stream);
-    case RMH_MODEL_PINHOLE: return launch_kind<RMH_MODEL_PINHOLE, STACK, BLOCK>(a, stream);
-    case RMH_MODEL_O1DN: return launch_kind<RMH_MODEL_O1DN, STACK, BLOCK>(a, stream);
-    default: return launch_kind<RMH_MODEL_ONDN, STACK, BLOCK>(a, stream);
+    case RMH_MODEL_SPHERICAL: return launch_kind<RMH_MODEL_SPHERICAL, STACK>(a, n_poses, stream);
+    case RMH_MODEL_PINHOLE: return launch_kind<RMH_MODEL_PINHOLE, STACK>(a, n_poses, stream);
+    case RMH_MODEL_O1DN: return launch_kind<RMH_MODEL_O1DN, STACK>(a, n_poses, stream);
+    default: return launch_kind<RMH_MODEL_ONDN, STACK>(a, n_poses, stream);
   }
 }
 
@@ -95,7 +97,9 @@ struct BatchEntryDev {
   Tf Tsb;
   const rmh_transform* Tbm;
   rmh_outputs out;
-  uint64_t n_rays;  // n_poses * ray_count
+  Tiling tl;
+  uint32_t n_poses;
+  uint64_t n_slots;  // n_poses * tiles * 64 ray slots, enumerated tile by tile
 };
 
 struct BatchArgs {
@@ -109,12 +113,11 @@ struct BatchArgs {
 };
 
 template <int KIND, int STACK, int BLOCK>
-__device__ __forceinline__ void batch_ray(const BatchArgs& a, const BatchEntryDev& e, uint64_t r, int (*stk)[BLOCK]) {
+__device__ __forceinline__ void batch_ray(const BatchArgs& a, const BatchEntryDev& e, uint32_t pose, uint32_t vid,
+                                          uint32_t hid, int (*stk)[BLOCK]) {
   const uint32_t rc = e.g.ray_count;
-  const uint32_t pose = static_cast<uint32_t>(r / rc);
-  const uint32_t lid = static_cast<uint32_t>(r - static_cast<uint64_t>(pose) * rc);
-  const uint32_t vid = lid / e.g.W;
-  const uint32_t hid = lid - vid * e.g.W;
+  const uint32_t lid = vid * e.g.W + hid;
+  const uint64_t r = static_cast<uint64_t>(pose) * rc + lid;
   V3 ds, os;
   gen_ray<KIND>(e.g, vid, hid, lid, ds, os);
   const Tf Tsm = tmul(load_tf(e.Tbm[pose]), e.Tsb);
@@ -131,7 +134,7 @@ __global__ void k_reset_counter(uint32_t* counter) {
 
 template <int STACK, int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_batch(const BatchArgs a) {
-  __shared__ int stk[STACK][BLOCK];
+  __shared__ int stk[kLdsStack][BLOCK];
   const int lane = threadIdx.x & 63;
   for (;;) {
     uint32_t c = 0;
@@ -145,28 +148,30 @@ __global__ __launch_bounds__(BLOCK) void k_batch(const BatchArgs a) {
       if (a.chunk_prefix[mid] <= c) lo = mid; else hi = mid;
     }
     const BatchEntryDev& e = a.entries[lo];
-    const uint64_t n = e.n_rays;
-    const uint64_t r0 = static_cast<uint64_t>(c - a.chunk_prefix[lo]) * a.chunk_rays;
+    const uint64_t tile0 = static_cast<uint64_t>(c - a.chunk_prefix[lo]) * (a.chunk_rays / 64);
+    const uint64_t n_tiles = static_cast<uint64_t>(e.n_poses) * e.tl.tiles;
     const int kind = e.g.kind;
-    for (uint32_t k = 0; k < a.chunk_rays; k += 64) {
-      const uint64_t r = r0 + k + lane;
-      if (r >= n) break;
+    for (uint32_t k = 0; k < a.chunk_rays / 64; k++) {
+      const uint64_t tg = tile0 + k;  // wave-uniform
+      if (tg >= n_tiles) break;
+      const uint32_t pose = static_cast<uint32_t>(tg / e.tl.tiles);
+      const uint32_t tile = static_cast<uint32_t>(tg - static_cast<uint64_t>(pose) * e.tl.tiles);
+      uint32_t vid, hid;
+      if (!tile_ray(e.tl, e.g.W, e.g.H, tile, lane, vid, hid)) continue;
       switch (kind) {
-        case RMH_MODEL_SPHERICAL: batch_ray<RMH_MODEL_SPHERICAL, STACK, BLOCK>(a, e, r, stk); break;
-        case RMH_MODEL_PINHOLE: batch_ray<RMH_MODEL_PINHOLE, STACK, BLOCK>(a, e, r, stk); break;
-        case RMH_MODEL_O1DN: batch_ray<RMH_MODEL_O1DN, STACK, BLOCK>(a, e, r, stk); break;
-        default: batch_ray<RMH_MODEL_ONDN, STACK, BLOCK>(a, e, r, stk); break;
+        case RMH_MODEL_SPHERICAL: batch_ray<RMH_MODEL_SPHERICAL, STACK, BLOCK>(a, e, pose, vid, hid, stk); break;
+        case RMH_MODEL_PINHOLE: batch_ray<RMH_MODEL_PINHOLE, STACK, BLOCK>(a, e, pose, vid, hid, stk); break;
+        case RMH_MODEL_O1DN: batch_ray<RMH_MODEL_O1DN, STACK, BLOCK>(a, e, pose, vid, hid, stk); break;
+        default: batch_ray<RMH_MODEL_ONDN, STACK, BLOCK>(a, e, pose, vid, hid, stk); break;
       }
     }
   }
 }
 
-constexpr int kBatchBlock = 64;  // one wave per workgroup: a finished wave frees its LDS at once
-
 template <int STACK>
 hipError_t launch_batch(const BatchArgs& a, uint32_t blocks, hipStream_t stream) {
   k_reset_counter<<<1, 64, 0, stream>>>(a.counter);
-  k_batch<STACK, kBatchBlock><<<blocks, kBatchBlock, 0, stream>>>(a);
+  k_batch<STACK, kWave><<<blocks, kWave, 0, stream>>>(a);
   return hipGetLastError();
 }
 
@@ -244,16 +249,16 @@ int rmh_simulate(const rmh_map* map, const rmh_sensor* sensor, const rmh_transfo
   a.out = *out;
   hipStream_t stream = static_cast<hipStream_t>(stream_);
 
-  const uint64_t max_rays = 1ull << 31;
-  const uint32_t poses_per_launch = static_cast<uint32_t>(std::max<uint64_t>(1, max_rays / ray_count));
+  a.tl = make_tiling(sensor->width, sensor->height);
+  const uint64_t max_blocks = 1ull << 31;
+  const uint32_t poses_per_launch = static_cast<uint32_t>(std::max<uint64_t>(1, max_blocks / a.tl.tiles));
   for (uint32_t p0 = 0; p0 < n_poses; p0 += poses_per_launch) {
     const uint32_t np = std::min<uint32_t>(poses_per_launch, n_poses - p0);
     a.pose_offset = p0;
-    a.n_rays = static_cast<uint32_t>(np * ray_count);
     hipError_t e;
-    if (depth <= 32) e = launch_stack<32, 256>(sensor->kind, a, stream);
-    else if (depth <= 64) e = launch_stack<64, 128>(sensor->kind, a, stream);
-    else e = launch_stack<128, 64>(sensor->kind, a, stream);
+    if (depth <= 32) e = launch_stack<32>(sensor->kind, a, np, stream);
+    else if (depth <= 64) e = launch_stack<64>(sensor->kind, a, np, stream);
+    else e = launch_stack<128>(sensor->kind, a, np, stream);
     if (e != hipSuccess) return hip_fail(e, "rmh_simulate launch");
   }
   return RMH_OK;
@@ -294,9 +299,11 @@ int rmh_batch_create(const rmh_map* map, const rmh_batch_entry* entries, uint32_
     host[i].Tsb = load_tf(e.Tsb);
     host[i].Tbm = e.Tbm_dev;
     host[i].out = e.out;
-    host[i].n_rays = static_cast<uint64_t>(e.n_poses) * host[i].g.ray_count;
-    b->entry_rays[i] = host[i].n_rays;
-    b->total_rays += host[i].n_rays;
+    host[i].tl = make_tiling(host[i].g.W, host[i].g.H);
+    host[i].n_poses = e.n_poses;
+    host[i].n_slots = static_cast<uint64_t>(e.n_poses) * host[i].tl.tiles * 64u;
+    b->entry_rays[i] = host[i].n_slots;
+    b->total_rays += static_cast<uint64_t>(e.n_poses) * host[i].g.ray_count;
   }
   hipError_t err = hipMalloc(&b->entries, sizeof(BatchEntryDev) * std::max<uint32_t>(1, n_entries));
   if (err == hipSuccess) err = hipMalloc(&b->chunk_prefix, sizeof(uint32_t) * (n_entries + 1));

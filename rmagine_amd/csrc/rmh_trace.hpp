@@ -99,6 +99,16 @@ struct Hit {
   uint32_t face;  // global face index (tie break)
 };
 
+// Traversal stack: the top kLdsStack entries of every lane live in LDS (one column per
+// lane); deeper entries go to a private per-lane array, which measured rays almost never
+// reach (the 2.1M-triangle terrain uses at most 16 of its 29 levels).  Keeping only 4 KB of
+// LDS per wave lets 6 waves per SIMD stay resident instead of 5.
+constexpr int kLdsStack = 16;
+
+// Closest hit along o + t d, 0 <= t <= range_max.  While-while traversal: a lane walks
+// interior nodes (near child first, far child onto its stack) until it reaches a leaf or
+// runs out of work, then the lanes holding leaves test their triangles together.  The
+// stack never overflows: the host picks STACK >= the depth of the BVH.
 template <int STACK, int BLOCK>
 __device__ __forceinline__ void trace(const MapData& m, const V3& o, const V3& d, float range_max,
                                       int (*stk)[BLOCK], Hit& h) {
@@ -111,11 +121,13 @@ __device__ __forceinline__ void trace(const MapData& m, const V3& o, const V3& d
   const float idy = 1.0f / (fabsf(d.y) > 1e-30f ? d.y : copysignf(1e-30f, d.y));
   const float idz = 1.0f / (fabsf(d.z) > 1e-30f ? d.z : copysignf(1e-30f, d.z));
   const TriRay tr = make_tri_ray(o, d);
+  constexpr int kDone = 0x7fffffff;
+  int spill[STACK > kLdsStack ? STACK - kLdsStack : 1];
   float tbf = range_max;
   int ref = m.root;
   int sp = 0;
   for (;;) {
-    if (ref >= 0) {
+    while (ref >= 0) {
       const Node2 nd = m.nodes[ref];
       const float x00 = (nd.a.x - o.x) * idx, x01 = (nd.a.y - o.x) * idx;
       const float y00 = (nd.a.z - o.y) * idy, y01 = (nd.a.w - o.y) * idy;
@@ -132,35 +144,66 @@ __device__ __forceinline__ void trace(const MapData& m, const V3& o, const V3& d
       if (h0 && h1) {
         int near = nd.d.x, far = nd.d.y;
         if (tmin1 < tmin0) { near = nd.d.y; far = nd.d.x; }
-        if (sp < STACK) stk[sp++][tid] = far;
+        if (sp < kLdsStack) stk[sp][tid] = far;
+        else if (sp < STACK) spill[sp - kLdsStack] = far;
+        sp++;
         ref = near;
-        continue;
+      } else if (h0) {
+        ref = nd.d.x;
+      } else if (h1) {
+        ref = nd.d.y;
+      } else {
+        if (sp == 0) { ref = kDone; break; }
+        --sp;
+        ref = sp < kLdsStack ? stk[sp][tid] : spill[sp - kLdsStack];
       }
-      if (h0) { ref = nd.d.x; continue; }
-      if (h1) { ref = nd.d.y; continue; }
-    } else {
-      const uint32_t e = ~static_cast<uint32_t>(ref);
-      const uint32_t first = e >> 3, cnt = (e & 7u) + 1u;
-      for (uint32_t k = 0; k < cnt; k++) {
-        const size_t s = static_cast<size_t>(first + k) * 3;
-        const float4 v0 = m.tris[s], v1 = m.tris[s + 1], v2 = m.tris[s + 2];
-        const double t = tri_test(tr, v0, v1, v2);
-        if (t >= 0.0 && t <= h.t) {
-          const uint32_t face = __float_as_uint(v2.w);
-          if (t < h.t || face < h.face) {
-            h.t = t;
-            h.face = face;
-            h.slot = first + k;
-          }
+    }
+    if (ref == kDone) break;
+    const uint32_t e = ~static_cast<uint32_t>(ref);
+    const uint32_t first = e >> 3, cnt = (e & 7u) + 1u;
+    for (uint32_t k = 0; k < cnt; k++) {
+      const size_t s = static_cast<size_t>(first + k) * 3;
+      const float4 v0 = m.tris[s], v1 = m.tris[s + 1], v2 = m.tris[s + 2];
+      const double t = tri_test(tr, v0, v1, v2);
+      if (t >= 0.0 && t <= h.t) {
+        const uint32_t face = __float_as_uint(v2.w);
+        if (t < h.t || face < h.face) {
+          h.t = t;
+          h.face = face;
+          h.slot = first + k;
         }
       }
-      tbf = __double2float_ru(h.t);
     }
+    tbf = __double2float_ru(h.t);
     if (sp == 0) break;
-    ref = stk[--sp][tid];
+    --sp;
+    ref = sp < kLdsStack ? stk[sp][tid] : spill[sp - kLdsStack];
   }
 }
 
+// Ray tiles: each wave casts a th x tw block of one sensor image (th * tw = 64), which keeps
+// the 64 rays of a wave close in direction.  O1Dn/OnDn images of height 1 use 1 x 64.
+struct Tiling {
+  uint32_t tw, th, tiles_w, tiles;  // tiles per pose image
+};
+
+inline Tiling make_tiling(uint32_t W, uint32_t H) {
+  Tiling t;
+  t.th = H >= 8 ? 8u : (H >= 4 ? 4u : (H >= 2 ? 2u : 1u));
+  t.tw = 64u / t.th;
+  t.tiles_w = (W + t.tw - 1) / t.tw;
+  t.tiles = t.tiles_w * ((H + t.th - 1) / t.th);
+  return t;
+}
+
+// (row, column) of lane `lane` in tile `tile`; false for lanes outside the image.
+__device__ __forceinline__ bool tile_ray(const Tiling& t, uint32_t W, uint32_t H, uint32_t tile, uint32_t lane,
+                                         uint32_t& vid, uint32_t& hid) {
+  const uint32_t ty = tile / t.tiles_w, tx = tile - ty * t.tiles_w;
+  vid = ty * t.th + lane / t.tw;
+  hid = tx * t.tw + lane % t.tw;
+  return vid < H && hid < W;
+}
 
 // Sensor-frame ray of (row vid, column hid, buffer id lid).
 template <int KIND>
