@@ -103,7 +103,7 @@ def extra_batched(cpu_seconds: float, do_cpu: bool) -> dict:
     for m, _ in sensors:
         mix[{0: "vlp16", 1: "pinhole", 2: "o1dn"}[m.kind]] += 1
     key = autotune.cache_key(len(F), mix)
-    tuned = autotune.tune_batch(batch, key=key, chunks=(64, 256, 1024), waves=(8, 16), reps=2, use_cache=False)
+    tuned = autotune.tune_batch(batch, key=key, chunks=(64, 256, 1024), waves=(16, 32), reps=2, use_cache=False)
     ms = autotune.time_batch(batch, reps=5)
     log(f"batched: {rays} rays, {ms:.2f} ms -> {rays / ms / 1e3:.1f} Mrays/s (tuned {tuned['chunk_rays']}/{tuned['waves_per_cu']})")
     # the same sensors with one rmh_simulate launch each

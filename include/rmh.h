@@ -175,7 +175,7 @@ typedef struct {
 
 typedef struct {
   uint32_t chunk_rays;    /* rays per work grab, multiple of 64 (0 = keep / default 256)  */
-  uint32_t waves_per_cu;  /* persistent waves per compute unit (0 = keep / default 16)   */
+  uint32_t waves_per_cu;  /* persistent waves per compute unit (0 = keep / default 32)   */
 } rmh_batch_params;
 
 typedef struct rmh_batch rmh_batch;

@@ -21,7 +21,7 @@ from pathlib import Path
 import torch
 
 DEFAULT_CHUNKS = (64, 128, 256, 512, 1024)
-DEFAULT_WAVES = (8, 12, 16, 20)
+DEFAULT_WAVES = (8, 16, 24, 32)
 
 
 def cache_path() -> Path:

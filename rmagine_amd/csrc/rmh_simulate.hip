@@ -6,7 +6,7 @@
 // simulators (src/rmagine_embree/include/rmagine/simulation/SphereSimulatorEmbree.tcc:55-253,
 // PinholeSimulatorEmbree.tcc, O1DnSimulatorEmbree.tcc:50-254, OnDnSimulatorEmbree.tcc).
 //
-// One lane per ray, wave64, one wave per workgroup.  A wave casts an 8 x 8 tile of one
+// One lane per ray, wave64, one wave per workgroup.  A wave casts a 4 x 16 tile of one
 // sensor image (1 x 64 for O1Dn/OnDn lists), so its rays stay close in direction.  Per lane:
 //   * ray set-up in FP32 in the reference's operation order (contraction off);
 //   * while-while traversal of the binary BVH with an LDS stack (one slot column per
@@ -52,11 +52,22 @@ struct SimArgs {
   rmh_outputs out;
 };
 
+// Workgroups are dispatched round-robin over the 8 XCDs (workgroup b runs on XCD b % 8).
+// Renumbering so that XCD x receives one contiguous range of (pose, tile) indices keeps
+// the BVH nodes of neighbouring tiles in that XCD's L2.  A bijection on [0, n).
+__device__ __forceinline__ uint32_t xcd_contiguous(uint32_t b, uint32_t n) {
+  const uint32_t xcd = b & 7u, idx = b >> 3, q = n >> 3, r = n & 7u;
+  return xcd < r ? xcd * (q + 1) + idx : r * (q + 1) + (xcd - r) * q + idx;
+}
+
+// Launch bounds ask for 8 waves per SIMD (64 VGPRs); the few values that do not fit are
+// set-up constants spilled outside the node loop.
 template <int KIND, int STACK>
-__global__ __launch_bounds__(kWave) void k_simulate(const SimArgs a) {
+__global__ __launch_bounds__(kWave, 8) void k_simulate(const SimArgs a) {
   __shared__ int stk[kLdsStack][kWave];
-  const uint32_t pl = blockIdx.x / a.tl.tiles;
-  const uint32_t tile = blockIdx.x - pl * a.tl.tiles;
+  const uint32_t lb = xcd_contiguous(blockIdx.x, gridDim.x);
+  const uint32_t pl = lb / a.tl.tiles;
+  const uint32_t tile = lb - pl * a.tl.tiles;
   uint32_t vid, hid;
   if (!tile_ray(a.tl, a.g.W, a.g.H, tile, threadIdx.x, vid, hid)) return;
   const uint32_t rc = a.g.ray_count;
@@ -133,7 +144,7 @@ __global__ void k_reset_counter(uint32_t* counter) {
 }
 
 template <int STACK, int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_batch(const BatchArgs a) {
+__global__ __launch_bounds__(BLOCK, 8) void k_batch(const BatchArgs a) {
   __shared__ int stk[kLdsStack][BLOCK];
   const int lane = threadIdx.x & 63;
   for (;;) {
@@ -276,7 +287,7 @@ int rmh_batch_create(const rmh_map* map, const rmh_batch_entry* entries, uint32_
   b->map = map;
   b->n_entries = n_entries;
   b->params.chunk_rays = 256;
-  b->params.waves_per_cu = 16;
+  b->params.waves_per_cu = 32;
   if (params) {
     if (params->chunk_rays) b->params.chunk_rays = params->chunk_rays;
     if (params->waves_per_cu) b->params.waves_per_cu = params->waves_per_cu;
