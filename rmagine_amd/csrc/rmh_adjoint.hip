@@ -42,7 +42,7 @@ struct GradArgs {
 constexpr int kWave = 64;
 
 template <int KIND, int STACK>
-__global__ __launch_bounds__(kWave) void k_pose_grad(const GradArgs a) {
+__global__ __launch_bounds__(kWave, 8) void k_pose_grad(const GradArgs a) {
   __shared__ int stk[kLdsStack][kWave];
   const uint32_t pose = a.pose_offset + blockIdx.y;
   uint32_t vid, hid;

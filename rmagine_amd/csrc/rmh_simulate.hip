@@ -123,20 +123,40 @@ struct BatchArgs {
   uint32_t* counter;
 };
 
-template <int KIND, int STACK, int BLOCK>
-__device__ __forceinline__ void batch_ray(const BatchArgs& a, const BatchEntryDev& e, uint32_t pose, uint32_t vid,
-                                          uint32_t hid, int (*stk)[BLOCK]) {
+// Sensor-frame ray and map-frame ray of one slot of a batch entry.
+template <int KIND>
+__device__ __forceinline__ void batch_gen(const BatchEntryDev& e, uint32_t vid, uint32_t hid, uint32_t lid,
+                                          const Tf& Tsm, V3& ds, V3& os, V3& dm, V3& om) {
+  gen_ray<KIND>(e.g, vid, hid, lid, ds, os);
+  dm = qrot(Tsm.R, ds);
+  om = map_origin<KIND>(Tsm, os);
+}
+
+// One batch slot: ray generation and attribute writing depend on the sensor kind, the
+// traversal does not, so it is instantiated once (one copy of the octant-specialised
+// walks instead of one per kind keeps the kernel inside the instruction cache).
+template <int STACK, int BLOCK>
+__device__ __forceinline__ void batch_ray(const BatchArgs& a, const BatchEntryDev& e, int kind, uint32_t pose,
+                                          uint32_t vid, uint32_t hid, int (*stk)[BLOCK]) {
   const uint32_t rc = e.g.ray_count;
   const uint32_t lid = vid * e.g.W + hid;
   const uint64_t r = static_cast<uint64_t>(pose) * rc + lid;
-  V3 ds, os;
-  gen_ray<KIND>(e.g, vid, hid, lid, ds, os);
   const Tf Tsm = tmul(load_tf(e.Tbm[pose]), e.Tsb);
-  const V3 dm = qrot(Tsm.R, ds);
-  const V3 om = map_origin<KIND>(Tsm, os);
+  V3 ds, os, dm, om;
+  switch (kind) {
+    case RMH_MODEL_SPHERICAL: batch_gen<RMH_MODEL_SPHERICAL>(e, vid, hid, lid, Tsm, ds, os, dm, om); break;
+    case RMH_MODEL_PINHOLE: batch_gen<RMH_MODEL_PINHOLE>(e, vid, hid, lid, Tsm, ds, os, dm, om); break;
+    case RMH_MODEL_O1DN: batch_gen<RMH_MODEL_O1DN>(e, vid, hid, lid, Tsm, ds, os, dm, om); break;
+    default: batch_gen<RMH_MODEL_ONDN>(e, vid, hid, lid, Tsm, ds, os, dm, om); break;
+  }
   Hit h;
   trace<STACK, BLOCK>(a.map, om, dm, e.g.range_max, stk, h);
-  write_outputs<KIND>(e.out, r, e.g, a.map, Tsm, ds, os, h);
+  switch (kind) {
+    case RMH_MODEL_SPHERICAL: write_outputs<RMH_MODEL_SPHERICAL>(e.out, r, e.g, a.map, Tsm, ds, os, h); break;
+    case RMH_MODEL_PINHOLE: write_outputs<RMH_MODEL_PINHOLE>(e.out, r, e.g, a.map, Tsm, ds, os, h); break;
+    case RMH_MODEL_O1DN: write_outputs<RMH_MODEL_O1DN>(e.out, r, e.g, a.map, Tsm, ds, os, h); break;
+    default: write_outputs<RMH_MODEL_ONDN>(e.out, r, e.g, a.map, Tsm, ds, os, h); break;
+  }
 }
 
 __global__ void k_reset_counter(uint32_t* counter) {
@@ -169,12 +189,7 @@ __global__ __launch_bounds__(BLOCK, 8) void k_batch(const BatchArgs a) {
       const uint32_t tile = static_cast<uint32_t>(tg - static_cast<uint64_t>(pose) * e.tl.tiles);
       uint32_t vid, hid;
       if (!tile_ray(e.tl, e.g.W, e.g.H, tile, lane, vid, hid)) continue;
-      switch (kind) {
-        case RMH_MODEL_SPHERICAL: batch_ray<RMH_MODEL_SPHERICAL, STACK, BLOCK>(a, e, pose, vid, hid, stk); break;
-        case RMH_MODEL_PINHOLE: batch_ray<RMH_MODEL_PINHOLE, STACK, BLOCK>(a, e, pose, vid, hid, stk); break;
-        case RMH_MODEL_O1DN: batch_ray<RMH_MODEL_O1DN, STACK, BLOCK>(a, e, pose, vid, hid, stk); break;
-        default: batch_ray<RMH_MODEL_ONDN, STACK, BLOCK>(a, e, pose, vid, hid, stk); break;
-      }
+      batch_ray<STACK, BLOCK>(a, e, kind, pose, vid, hid, stk);
     }
   }
 }

@@ -108,12 +108,10 @@ constexpr int kLdsStack = 16;
 // Node words read through the constant address space, which the compiler serves with
 // scalar loads when the address is wave-uniform.
 typedef __attribute__((address_space(4))) const float CFloat;
+typedef __attribute__((address_space(1))) const char GChar;
+typedef __attribute__((address_space(1))) const float GFloat;
+typedef __attribute__((address_space(1))) const int GInt;
 
-// Closest hit along o + t d, 0 <= t <= range_max.  While-while traversal: a lane walks
-// interior nodes (near child first, far child onto its stack) until it reaches a leaf or
-// runs out of work, then the lanes holding leaves test their triangles together.  The
-// stack never overflows: the host picks STACK >= the depth of the BVH.
-//
 // Slab test.  A slab value is fma(plane, 1/d, -(o +- P) / d) with a per-ray margin
 //   P = 2^-21 max|o| + 2^-62 range_max + 1e-30:
 // the rounding of (o + P)/d stays below P, so each value is, up to the one rounding of
@@ -121,7 +119,108 @@ typedef __attribute__((address_space(4))) const float CFloat;
 // plane with o + P, the hi plane with o - P).  P also covers replacing a direction
 // component below 2^-64 by +-2^-64 (position error <= range_max * 2^-64).  The remaining
 // relative errors (fma, reciprocal: 2 roundings) are absorbed by tmin <= tmax (1 + 5e-7),
-// so a box that the exact ray meets within [0, tbf] is never culled.
+// so a box that the exact ray meets within [0, tbf] is never culled.  SlabRay holds the
+// per-ray constants of that test.
+struct SlabRay {
+  float idx, idy, idz;   // reciprocal direction (components clamped away from zero)
+  float lx, ly, lz;      // -(o + P) / d: offsets of the lo planes
+  float hx, hy, hz;      // -(o - P) / d: offsets of the hi planes
+};
+
+// Entry/exit distances of both children of one node.  OCT >= 0: the sign of every
+// direction component is known for the whole wave (bit 0/1/2 set = x/y/z negative), so
+// the near and far plane of each axis are known and min/max per axis are not needed.
+// The values are the same as the generic form: with lo <= hi and (o + P) >= (o - P), the
+// lo-plane value is below the hi-plane value whenever 1/d > 0 (monotone rounding).
+template <int OCT>
+__device__ __forceinline__ void slab2(const SlabRay& r, float tbf, const float4& na, const float4& nb,
+                                      const float4& nc, float& tmin0, float& tmax0, float& tmin1, float& tmax1) {
+  const float x00 = __fmaf_rn(na.x, r.idx, r.lx), x01 = __fmaf_rn(na.y, r.idx, r.hx);
+  const float y00 = __fmaf_rn(na.z, r.idy, r.ly), y01 = __fmaf_rn(na.w, r.idy, r.hy);
+  const float z00 = __fmaf_rn(nc.x, r.idz, r.lz), z01 = __fmaf_rn(nc.y, r.idz, r.hz);
+  const float x10 = __fmaf_rn(nb.x, r.idx, r.lx), x11 = __fmaf_rn(nb.y, r.idx, r.hx);
+  const float y10 = __fmaf_rn(nb.z, r.idy, r.ly), y11 = __fmaf_rn(nb.w, r.idy, r.hy);
+  const float z10 = __fmaf_rn(nc.z, r.idz, r.lz), z11 = __fmaf_rn(nc.w, r.idz, r.hz);
+  if constexpr (OCT < 0) {
+    tmin0 = fmaxf(fmaxf(fminf(x00, x01), fminf(y00, y01)), fmaxf(fminf(z00, z01), 0.0f));
+    tmax0 = fminf(fminf(fmaxf(x00, x01), fmaxf(y00, y01)), fminf(fmaxf(z00, z01), tbf));
+    tmin1 = fmaxf(fmaxf(fminf(x10, x11), fminf(y10, y11)), fmaxf(fminf(z10, z11), 0.0f));
+    tmax1 = fminf(fminf(fmaxf(x10, x11), fmaxf(y10, y11)), fminf(fmaxf(z10, z11), tbf));
+  } else {
+    constexpr bool nx = OCT & 1, ny = OCT & 2, nz = OCT & 4;
+    tmin0 = fmaxf(fmaxf(nx ? x01 : x00, ny ? y01 : y00), fmaxf(nz ? z01 : z00, 0.0f));
+    tmax0 = fminf(fminf(nx ? x00 : x01, ny ? y00 : y01), fminf(nz ? z00 : z01, tbf));
+    tmin1 = fmaxf(fmaxf(nx ? x11 : x10, ny ? y11 : y10), fmaxf(nz ? z11 : z10, 0.0f));
+    tmax1 = fminf(fminf(nx ? x10 : x11, ny ? y10 : y11), fminf(nz ? z10 : z11, tbf));
+  }
+}
+
+constexpr int kDone = 0x7fffffff;
+
+// Walks interior nodes from `ref` until a leaf is reached (ref < 0) or the stack runs
+// dry (ref = kDone).
+template <int OCT, int STACK, int BLOCK>
+__device__ __forceinline__ void walk(const MapData& m, const SlabRay& r, float tbf, int (*stk)[BLOCK],
+                                     volatile int* spill, int& ref, int& sp) {
+  const int tid = threadIdx.x;
+  while (ref >= 0) {
+    float tmin0, tmax0, tmin1, tmax1;
+    int2 nr;
+    const int r0 = __builtin_amdgcn_readfirstlane(ref);
+    if (__ballot(ref != r0) == 0) {
+      // every active lane is at the same node (45% of visits on the terrain): read it
+      // through the scalar cache, which keeps the vector memory return path free; the
+      // slab code sits in both branches so that it reads the node words from SGPRs
+      const CFloat* cf = (const CFloat*)(m.nodes) + static_cast<size_t>(r0) * 16;
+      slab2<OCT>(r, tbf, make_float4(cf[0], cf[1], cf[2], cf[3]), make_float4(cf[4], cf[5], cf[6], cf[7]),
+                 make_float4(cf[8], cf[9], cf[10], cf[11]), tmin0, tmax0, tmin1, tmax1);
+      nr = make_int2(__float_as_int(cf[12]), __float_as_int(cf[13]));
+    } else {
+      // 32-bit byte offset from the map's base: one shift, and the loads take the
+      // scalar-base + vector-offset form
+      const uint32_t off = static_cast<uint32_t>(ref) << 6;
+      const GFloat* gf = (const GFloat*)((const GChar*)(m.nodes) + off);
+      const float4 na = make_float4(gf[0], gf[1], gf[2], gf[3]);
+      const float4 nb = make_float4(gf[4], gf[5], gf[6], gf[7]);
+      const float4 nc = make_float4(gf[8], gf[9], gf[10], gf[11]);
+      // only the two child references of the last 16 bytes are used; hiding the
+      // offset's alignment keeps the compiler from widening this read to 16 bytes
+      uint32_t roff = off + 48u;
+      asm volatile("" : "+v"(roff));
+      __builtin_assume((roff & 7u) == 0u);
+      const GInt* gr = (const GInt*)((const GChar*)(m.nodes) + roff);
+      nr = make_int2(gr[0], gr[1]);
+      slab2<OCT>(r, tbf, na, nb, nc, tmin0, tmax0, tmin1, tmax1);
+    }
+    const bool h0 = tmin0 <= tmax0 * 1.0000005f;
+    const bool h1 = tmin1 <= tmax1 * 1.0000005f;
+    if (h0 && h1) {
+      int near = nr.x, far = nr.y;
+      if (tmin1 < tmin0) { near = nr.y; far = nr.x; }
+      if (sp < kLdsStack) stk[sp][tid] = far;
+      else if (sp < STACK) spill[sp - kLdsStack] = far;
+      sp++;
+      ref = near;
+    } else if (h0) {
+      ref = nr.x;
+    } else if (h1) {
+      ref = nr.y;
+    } else {
+      if (sp == 0) { ref = kDone; return; }
+      --sp;
+      if (sp < kLdsStack) ref = stk[sp][tid];
+      else ref = spill[sp - kLdsStack];
+    }
+  }
+}
+
+// Closest hit along o + t d, 0 <= t <= range_max.  While-while traversal: a lane walks
+// interior nodes (near child first, far child onto its stack) until it reaches a leaf or
+// runs out of work, then the lanes holding leaves test their triangles together.  The
+// stack never overflows: the host picks STACK >= the depth of the BVH.
+//
+// Waves whose 64 rays share one direction octant (almost all tiles) run the walk
+// specialised for it; mixed waves run the generic one.
 template <int STACK, int BLOCK>
 __device__ __forceinline__ void trace(const MapData& m, const V3& o, const V3& d, float range_max,
                                       int (*stk)[BLOCK], Hit& h) {
@@ -131,16 +230,20 @@ __device__ __forceinline__ void trace(const MapData& m, const V3& o, const V3& d
   if (m.n_faces == 0) return;
   const int tid = threadIdx.x;
   constexpr float kMinDir = 0x1p-64f;
-  const float idx = 1.0f / (fabsf(d.x) > kMinDir ? d.x : copysignf(kMinDir, d.x));
-  const float idy = 1.0f / (fabsf(d.y) > kMinDir ? d.y : copysignf(kMinDir, d.y));
-  const float idz = 1.0f / (fabsf(d.z) > kMinDir ? d.z : copysignf(kMinDir, d.z));
+  SlabRay r;
+  r.idx = 1.0f / (fabsf(d.x) > kMinDir ? d.x : copysignf(kMinDir, d.x));
+  r.idy = 1.0f / (fabsf(d.y) > kMinDir ? d.y : copysignf(kMinDir, d.y));
+  r.idz = 1.0f / (fabsf(d.z) > kMinDir ? d.z : copysignf(kMinDir, d.z));
   const float omax = fmaxf(fmaxf(fabsf(o.x), fabsf(o.y)), fabsf(o.z));
   const float P = omax * 0x1p-21f + range_max * 0x1p-62f + 1e-30f;
-  const float lx = -((o.x + P) * idx), hx = -((o.x - P) * idx);
-  const float ly = -((o.y + P) * idy), hy = -((o.y - P) * idy);
-  const float lz = -((o.z + P) * idz), hz = -((o.z - P) * idz);
+  r.lx = -((o.x + P) * r.idx); r.hx = -((o.x - P) * r.idx);
+  r.ly = -((o.y + P) * r.idy); r.hy = -((o.y - P) * r.idy);
+  r.lz = -((o.z + P) * r.idz); r.hz = -((o.z - P) * r.idz);
+  // direction octant, specialised when the whole wave shares it (almost all tiles)
+  const int oct = (r.idx < 0.0f ? 1 : 0) | (r.idy < 0.0f ? 2 : 0) | (r.idz < 0.0f ? 4 : 0);
+  const int oct0 = __builtin_amdgcn_readfirstlane(oct);
+  const int woct = __ballot(oct != oct0) == 0 ? oct0 : -1;
   const TriRay tr = make_tri_ray(o, d);
-  constexpr int kDone = 0x7fffffff;
   // volatile keeps the rare deep entries in scratch and stops the compiler from merging
   // the LDS and scratch pops into one flat load
   volatile int spill[STACK > kLdsStack ? STACK - kLdsStack : 1];
@@ -148,56 +251,16 @@ __device__ __forceinline__ void trace(const MapData& m, const V3& o, const V3& d
   int ref = m.root;
   int sp = 0;
   for (;;) {
-    while (ref >= 0) {
-      float4 na, nb, nc;
-      int2 nr;
-      const int r0 = __builtin_amdgcn_readfirstlane(ref);
-      if (__ballot(ref != r0) == 0) {
-        // every active lane is at the same node (45% of visits on the terrain): read it
-        // through the scalar cache, which keeps the vector memory return path free
-        const CFloat* cf = (const CFloat*)(m.nodes) + static_cast<size_t>(r0) * 16;
-        na = make_float4(cf[0], cf[1], cf[2], cf[3]);
-        nb = make_float4(cf[4], cf[5], cf[6], cf[7]);
-        nc = make_float4(cf[8], cf[9], cf[10], cf[11]);
-        nr = make_int2(__float_as_int(cf[12]), __float_as_int(cf[13]));
-      } else {
-        const Node2* gn = m.nodes + ref;
-        na = gn->a; nb = gn->b; nc = gn->c;
-        // only the two child references of the last 16 bytes are used; hiding the
-        // pointer's alignment keeps the compiler from widening this read to 16 bytes
-        const int2* gr = reinterpret_cast<const int2*>(&gn->d);
-        asm volatile("" : "+v"(gr));
-        nr = *gr;
-      }
-      const float x00 = __fmaf_rn(na.x, idx, lx), x01 = __fmaf_rn(na.y, idx, hx);
-      const float y00 = __fmaf_rn(na.z, idy, ly), y01 = __fmaf_rn(na.w, idy, hy);
-      const float z00 = __fmaf_rn(nc.x, idz, lz), z01 = __fmaf_rn(nc.y, idz, hz);
-      const float x10 = __fmaf_rn(nb.x, idx, lx), x11 = __fmaf_rn(nb.y, idx, hx);
-      const float y10 = __fmaf_rn(nb.z, idy, ly), y11 = __fmaf_rn(nb.w, idy, hy);
-      const float z10 = __fmaf_rn(nc.z, idz, lz), z11 = __fmaf_rn(nc.w, idz, hz);
-      const float tmin0 = fmaxf(fmaxf(fminf(x00, x01), fminf(y00, y01)), fmaxf(fminf(z00, z01), 0.0f));
-      const float tmax0 = fminf(fminf(fmaxf(x00, x01), fmaxf(y00, y01)), fminf(fmaxf(z00, z01), tbf));
-      const float tmin1 = fmaxf(fmaxf(fminf(x10, x11), fminf(y10, y11)), fmaxf(fminf(z10, z11), 0.0f));
-      const float tmax1 = fminf(fminf(fmaxf(x10, x11), fmaxf(y10, y11)), fminf(fmaxf(z10, z11), tbf));
-      const bool h0 = tmin0 <= tmax0 * 1.0000005f;
-      const bool h1 = tmin1 <= tmax1 * 1.0000005f;
-      if (h0 && h1) {
-        int near = nr.x, far = nr.y;
-        if (tmin1 < tmin0) { near = nr.y; far = nr.x; }
-        if (sp < kLdsStack) stk[sp][tid] = far;
-        else if (sp < STACK) spill[sp - kLdsStack] = far;
-        sp++;
-        ref = near;
-      } else if (h0) {
-        ref = nr.x;
-      } else if (h1) {
-        ref = nr.y;
-      } else {
-        if (sp == 0) { ref = kDone; break; }
-        --sp;
-        if (sp < kLdsStack) ref = stk[sp][tid];
-        else ref = spill[sp - kLdsStack];
-      }
+    switch (woct) {
+      case 0: walk<0, STACK, BLOCK>(m, r, tbf, stk, spill, ref, sp); break;
+      case 1: walk<1, STACK, BLOCK>(m, r, tbf, stk, spill, ref, sp); break;
+      case 2: walk<2, STACK, BLOCK>(m, r, tbf, stk, spill, ref, sp); break;
+      case 3: walk<3, STACK, BLOCK>(m, r, tbf, stk, spill, ref, sp); break;
+      case 4: walk<4, STACK, BLOCK>(m, r, tbf, stk, spill, ref, sp); break;
+      case 5: walk<5, STACK, BLOCK>(m, r, tbf, stk, spill, ref, sp); break;
+      case 6: walk<6, STACK, BLOCK>(m, r, tbf, stk, spill, ref, sp); break;
+      case 7: walk<7, STACK, BLOCK>(m, r, tbf, stk, spill, ref, sp); break;
+      default: walk<-1, STACK, BLOCK>(m, r, tbf, stk, spill, ref, sp); break;
     }
     if (ref == kDone) break;
     const uint32_t e = ~static_cast<uint32_t>(ref);
