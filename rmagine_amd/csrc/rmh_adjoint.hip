@@ -45,11 +45,10 @@ template <int KIND, int STACK>
 __global__ __launch_bounds__(kWave, 8) void k_pose_grad(const GradArgs a) {
   __shared__ int stk[kLdsStack][kWave];
   const uint32_t pose = a.pose_offset + blockIdx.y;
-  uint32_t vid, hid;
-  const bool active = tile_ray(a.tl, a.g.W, a.g.H, blockIdx.x, threadIdx.x, vid, hid);
+  uint32_t vid, hid, lid;
+  const bool active = tile_ray(a.tl, a.g.grid_w, a.g.grid_h, blockIdx.x, threadIdx.x, vid, hid);
   double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   if (active) {
-    const uint32_t lid = vid * a.g.W + hid;
     V3 ds, os;
     gen_ray<KIND>(a.g, vid, hid, lid, ds, os);
     const Tf Tbm = load_tf(a.Tbm[pose]);
@@ -133,7 +132,7 @@ extern "C" {
 
 int rmh_pose_grad_workspace_size(const rmh_map* map, const rmh_sensor* sensor, uint32_t n_poses, size_t* bytes) {
   if (!map || !sensor || !bytes) return fail(RMH_ERR_INVALID, "rmh_pose_grad_workspace_size: NULL argument");
-  const Tiling tl = make_tiling(sensor->width, sensor->height);
+  const Tiling tl = sensor_tiling(sensor);
   *bytes = sizeof(double) * 8 * static_cast<uint64_t>(tl.tiles) * n_poses;
   return RMH_OK;
 }
@@ -162,7 +161,7 @@ int rmh_pose_grad(const rmh_map* map, const rmh_sensor* sensor, const rmh_transf
   a.target_mask = target_mask;
   a.target_stride = per_pose_targets ? static_cast<uint32_t>(rc) : 0u;
   a.delta = huber_delta;
-  a.tl = make_tiling(sensor->width, sensor->height);
+  a.tl = sensor_tiling(sensor);
   if (a.tl.tiles > 65535u) return fail(RMH_ERR_UNSUPPORTED, "rmh_pose_grad: more than 65535 ray tiles per pose");
   const size_t need = sizeof(double) * 8 * static_cast<size_t>(a.tl.tiles) * n_poses;
   Scratch scratch;

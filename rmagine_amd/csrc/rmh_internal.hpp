@@ -139,7 +139,10 @@ struct rmh_sensor {
   rmh_pinhole_model pinhole{};
   rmh_vec3 orig{};
   float2* lut = nullptr;          // spherical: H rows (cos,sin phi) then W cols (cos,sin theta)
-  rmh_vec3* dirs = nullptr;       // O1Dn / OnDn directions, W*H
-  rmh_vec3* origs = nullptr;      // OnDn origins, W*H
+  // O1Dn / OnDn rays in traversal order (sorted by direction); order[s] is the buffer
+  // index (row * W + column) of traversal slot s
+  rmh_vec3* dirs = nullptr;       // W*H
+  rmh_vec3* origs = nullptr;      // OnDn only, W*H
+  uint32_t* order = nullptr;      // W*H
   int device = 0;
 };

@@ -7,6 +7,8 @@
 // O1DnSimulatorOptix.cpp:41-83).
 #include "rmh_internal.hpp"
 
+#include <rocprim/device/device_radix_sort.hpp>
+
 #include <cmath>
 #include <cstring>
 #include <new>
@@ -141,7 +143,89 @@ static void sensor_free(rmh_sensor* s) {
   if (s->lut) (void)hipFree(s->lut);
   if (s->dirs) (void)hipFree(s->dirs);
   if (s->origs) (void)hipFree(s->origs);
+  if (s->order) (void)hipFree(s->order);
   delete s;
+}
+
+// ---- traversal order of O1Dn / OnDn rays -------------------------------------------------
+// An O1Dn/OnDn model is an arbitrary list of rays; in list order the 64 rays of a wave can
+// point anywhere, and their traversals share almost nothing.  The sensor therefore keeps
+// its rays sorted by direction (2D Morton code of the octahedral map of the direction,
+// 16 bits per axis) and the kernels trace in that order, writing each result at the
+// ray's own buffer index.  Results do not depend on the order.
+
+__device__ inline uint32_t spread16(uint32_t x) {
+  x &= 0xFFFFu;
+  x = (x | (x << 8)) & 0x00FF00FFu;
+  x = (x | (x << 4)) & 0x0F0F0F0Fu;
+  x = (x | (x << 2)) & 0x33333333u;
+  x = (x | (x << 1)) & 0x55555555u;
+  return x;
+}
+
+__global__ void k_dir_keys(const rmh_vec3* __restrict__ d, uint32_t n, uint32_t* __restrict__ keys,
+                           uint32_t* __restrict__ idx) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const rmh_vec3 v = d[i];
+  const float l1 = fabsf(v.x) + fabsf(v.y) + fabsf(v.z);
+  float u = 0.0f, w = 0.0f;
+  if (l1 > 0.0f) {  // octahedral map: the unit sphere onto the square [-1, 1]^2
+    u = v.x / l1;
+    w = v.y / l1;
+    if (v.z < 0.0f) {
+      const float uu = (1.0f - fabsf(w)) * (u < 0.0f ? -1.0f : 1.0f);
+      w = (1.0f - fabsf(u)) * (w < 0.0f ? -1.0f : 1.0f);
+      u = uu;
+    }
+  }
+  // NaN-safe clamp (fminf/fmaxf return the number) and 16-bit quantisation
+  const float qu = fminf(fmaxf((u + 1.0f) * 32767.5f, 0.0f), 65535.0f);
+  const float qw = fminf(fmaxf((w + 1.0f) * 32767.5f, 0.0f), 65535.0f);
+  keys[i] = spread16(static_cast<uint32_t>(qu)) | (spread16(static_cast<uint32_t>(qw)) << 1);
+  idx[i] = i;
+}
+
+__global__ void k_gather_rays(const uint32_t* __restrict__ order, uint32_t n, const rmh_vec3* __restrict__ d_in,
+                              const rmh_vec3* __restrict__ o_in, rmh_vec3* __restrict__ d_out,
+                              rmh_vec3* __restrict__ o_out) {
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  const uint32_t i = order[s];
+  d_out[s] = d_in[i];
+  if (o_in) o_out[s] = o_in[i];
+}
+
+// Uploads n rays (device pointers from the caller) into s->dirs / s->origs in direction
+// order and fills s->order.  Blocking (sensor creation is not on a timed path).
+static hipError_t upload_sorted_rays(rmh_sensor* s, const rmh_vec3* dirs_in, const rmh_vec3* origs_in, uint32_t n) {
+  uint32_t *keys = nullptr, *keys2 = nullptr, *idx = nullptr;
+  void* tmp = nullptr;
+  size_t tmp_bytes = 0;
+  const uint32_t blocks = (n + 255) / 256;
+  hipError_t e = hipMalloc(&s->dirs, n * sizeof(rmh_vec3));
+  if (e == hipSuccess && origs_in) e = hipMalloc(&s->origs, n * sizeof(rmh_vec3));
+  if (e == hipSuccess) e = hipMalloc(&s->order, n * sizeof(uint32_t));
+  if (e == hipSuccess) e = hipMalloc(&keys, n * sizeof(uint32_t));
+  if (e == hipSuccess) e = hipMalloc(&keys2, n * sizeof(uint32_t));
+  if (e == hipSuccess) e = hipMalloc(&idx, n * sizeof(uint32_t));
+  if (e == hipSuccess) {
+    k_dir_keys<<<blocks, 256>>>(dirs_in, n, keys, idx);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = rocprim::radix_sort_pairs(nullptr, tmp_bytes, keys, keys2, idx, s->order, n);
+  if (e == hipSuccess) e = hipMalloc(&tmp, tmp_bytes > 0 ? tmp_bytes : 1);
+  if (e == hipSuccess) e = rocprim::radix_sort_pairs(tmp, tmp_bytes, keys, keys2, idx, s->order, n);
+  if (e == hipSuccess) {
+    k_gather_rays<<<blocks, 256>>>(s->order, n, dirs_in, origs_in, s->dirs, s->origs);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  if (keys) (void)hipFree(keys);
+  if (keys2) (void)hipFree(keys2);
+  if (idx) (void)hipFree(idx);
+  if (tmp) (void)hipFree(tmp);
+  return e;
 }
 
 int rmh_sensor_create(int kind, const void* model, rmh_sensor** out) {
@@ -185,35 +269,28 @@ int rmh_sensor_create(int kind, const void* model, rmh_sensor** out) {
     case RMH_MODEL_O1DN: {
       const auto* m = static_cast<const rmh_o1dn_model*>(model);
       const size_t n = static_cast<size_t>(m->width) * m->height;
-      if (n == 0 || !m->dirs_dev) {
+      if (n == 0 || n > 0xFFFFFFFFu || !m->dirs_dev) {
         sensor_free(s);
-        return fail(RMH_ERR_INVALID, "rmh_sensor_create: O1Dn model needs width*height > 0 and dirs");
+        return fail(RMH_ERR_INVALID, "rmh_sensor_create: O1Dn model needs 0 < width*height < 2^32 and dirs");
       }
       s->width = m->width;
       s->height = m->height;
       s->range = m->range;
       s->orig = m->orig;
-      e = hipMalloc(&s->dirs, n * sizeof(rmh_vec3));
-      if (e == hipSuccess)
-        e = hipMemcpy(s->dirs, m->dirs_dev, n * sizeof(rmh_vec3), hipMemcpyDeviceToDevice);
+      e = upload_sorted_rays(s, m->dirs_dev, nullptr, static_cast<uint32_t>(n));
       break;
     }
     case RMH_MODEL_ONDN: {
       const auto* m = static_cast<const rmh_ondn_model*>(model);
       const size_t n = static_cast<size_t>(m->width) * m->height;
-      if (n == 0 || !m->dirs_dev || !m->origs_dev) {
+      if (n == 0 || n > 0xFFFFFFFFu || !m->dirs_dev || !m->origs_dev) {
         sensor_free(s);
-        return fail(RMH_ERR_INVALID, "rmh_sensor_create: OnDn model needs width*height > 0, dirs and origs");
+        return fail(RMH_ERR_INVALID, "rmh_sensor_create: OnDn model needs 0 < width*height < 2^32, dirs and origs");
       }
       s->width = m->width;
       s->height = m->height;
       s->range = m->range;
-      e = hipMalloc(&s->dirs, n * sizeof(rmh_vec3));
-      if (e == hipSuccess)
-        e = hipMemcpy(s->dirs, m->dirs_dev, n * sizeof(rmh_vec3), hipMemcpyDeviceToDevice);
-      if (e == hipSuccess) e = hipMalloc(&s->origs, n * sizeof(rmh_vec3));
-      if (e == hipSuccess)
-        e = hipMemcpy(s->origs, m->origs_dev, n * sizeof(rmh_vec3), hipMemcpyDeviceToDevice);
+      e = upload_sorted_rays(s, m->dirs_dev, m->origs_dev, static_cast<uint32_t>(n));
       break;
     }
     default:

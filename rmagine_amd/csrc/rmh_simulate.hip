@@ -68,15 +68,12 @@ __global__ __launch_bounds__(kWave, 8) void k_simulate(const SimArgs a) {
   const uint32_t lb = xcd_contiguous(blockIdx.x, gridDim.x);
   const uint32_t pl = lb / a.tl.tiles;
   const uint32_t tile = lb - pl * a.tl.tiles;
-  uint32_t vid, hid;
-  if (!tile_ray(a.tl, a.g.W, a.g.H, tile, threadIdx.x, vid, hid)) return;
-  const uint32_t rc = a.g.ray_count;
-  const uint32_t lid = vid * a.g.W + hid;
+  uint32_t vid, hid, lid;
+  if (!tile_ray(a.tl, a.g.grid_w, a.g.grid_h, tile, threadIdx.x, vid, hid)) return;
   const uint32_t pose = a.pose_offset + pl;
-  const uint64_t glob = static_cast<uint64_t>(pose) * rc + lid;
-
   V3 ds, os;
   gen_ray<KIND>(a.g, vid, hid, lid, ds, os);
+  const uint64_t glob = static_cast<uint64_t>(pose) * a.g.ray_count + lid;
   const Tf Tsm = tmul(load_tf(a.Tbm[pose]), a.Tsb);  // Tsm = Tbm * Tsb (SphereSimulatorEmbree.tcc:69-70)
   const V3 dm = qrot(Tsm.R, ds);
   const V3 om = map_origin<KIND>(Tsm, os);
@@ -125,7 +122,7 @@ struct BatchArgs {
 
 // Sensor-frame ray and map-frame ray of one slot of a batch entry.
 template <int KIND>
-__device__ __forceinline__ void batch_gen(const BatchEntryDev& e, uint32_t vid, uint32_t hid, uint32_t lid,
+__device__ __forceinline__ void batch_gen(const BatchEntryDev& e, uint32_t vid, uint32_t hid, uint32_t& lid,
                                           const Tf& Tsm, V3& ds, V3& os, V3& dm, V3& om) {
   gen_ray<KIND>(e.g, vid, hid, lid, ds, os);
   dm = qrot(Tsm.R, ds);
@@ -138,10 +135,8 @@ __device__ __forceinline__ void batch_gen(const BatchEntryDev& e, uint32_t vid, 
 template <int STACK, int BLOCK>
 __device__ __forceinline__ void batch_ray(const BatchArgs& a, const BatchEntryDev& e, int kind, uint32_t pose,
                                           uint32_t vid, uint32_t hid, int (*stk)[BLOCK]) {
-  const uint32_t rc = e.g.ray_count;
-  const uint32_t lid = vid * e.g.W + hid;
-  const uint64_t r = static_cast<uint64_t>(pose) * rc + lid;
   const Tf Tsm = tmul(load_tf(e.Tbm[pose]), e.Tsb);
+  uint32_t lid;
   V3 ds, os, dm, om;
   switch (kind) {
     case RMH_MODEL_SPHERICAL: batch_gen<RMH_MODEL_SPHERICAL>(e, vid, hid, lid, Tsm, ds, os, dm, om); break;
@@ -149,6 +144,7 @@ __device__ __forceinline__ void batch_ray(const BatchArgs& a, const BatchEntryDe
     case RMH_MODEL_O1DN: batch_gen<RMH_MODEL_O1DN>(e, vid, hid, lid, Tsm, ds, os, dm, om); break;
     default: batch_gen<RMH_MODEL_ONDN>(e, vid, hid, lid, Tsm, ds, os, dm, om); break;
   }
+  const uint64_t r = static_cast<uint64_t>(pose) * e.g.ray_count + lid;
   Hit h;
   trace<STACK, BLOCK>(a.map, om, dm, e.g.range_max, stk, h);
   switch (kind) {
@@ -188,7 +184,7 @@ __global__ __launch_bounds__(BLOCK, 8) void k_batch(const BatchArgs a) {
       const uint32_t pose = static_cast<uint32_t>(tg / e.tl.tiles);
       const uint32_t tile = static_cast<uint32_t>(tg - static_cast<uint64_t>(pose) * e.tl.tiles);
       uint32_t vid, hid;
-      if (!tile_ray(e.tl, e.g.W, e.g.H, tile, lane, vid, hid)) continue;
+      if (!tile_ray(e.tl, e.g.grid_w, e.g.grid_h, tile, lane, vid, hid)) continue;
       batch_ray<STACK, BLOCK>(a, e, kind, pose, vid, hid, stk);
     }
   }
@@ -275,7 +271,7 @@ int rmh_simulate(const rmh_map* map, const rmh_sensor* sensor, const rmh_transfo
   a.out = *out;
   hipStream_t stream = static_cast<hipStream_t>(stream_);
 
-  a.tl = make_tiling(sensor->width, sensor->height);
+  a.tl = sensor_tiling(sensor);
   const uint64_t max_blocks = 1ull << 31;
   const uint32_t poses_per_launch = static_cast<uint32_t>(std::max<uint64_t>(1, max_blocks / a.tl.tiles));
   for (uint32_t p0 = 0; p0 < n_poses; p0 += poses_per_launch) {
@@ -325,7 +321,7 @@ int rmh_batch_create(const rmh_map* map, const rmh_batch_entry* entries, uint32_
     host[i].Tsb = load_tf(e.Tsb);
     host[i].Tbm = e.Tbm_dev;
     host[i].out = e.out;
-    host[i].tl = make_tiling(host[i].g.W, host[i].g.H);
+    host[i].tl = sensor_tiling(e.sensor);
     host[i].n_poses = e.n_poses;
     host[i].n_slots = static_cast<uint64_t>(e.n_poses) * host[i].tl.tiles * 64u;
     b->entry_rays[i] = host[i].n_slots;

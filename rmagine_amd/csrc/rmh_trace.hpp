@@ -16,8 +16,10 @@ struct RayGen {
   const float2* lut;                 // spherical direction table: H rows then W columns
   float pf0, pf1, pc0, pc1;          // pinhole
   V3 orig;                           // O1Dn origin (sensor frame)
-  const rmh_vec3* dirs;              // O1Dn / OnDn
-  const rmh_vec3* origs;             // OnDn
+  const rmh_vec3* dirs;              // O1Dn / OnDn, in traversal order
+  const rmh_vec3* origs;             // OnDn, in traversal order
+  const uint32_t* order;             // O1Dn / OnDn: buffer index of each traversal slot
+  uint32_t grid_w, grid_h;           // image that is cut into tiles: W x H, or (W*H) x 1 for ray lists
 };
 
 inline RayGen make_raygen(const rmh_sensor* s) {
@@ -36,6 +38,10 @@ inline RayGen make_raygen(const rmh_sensor* s) {
   g.orig = v3(s->orig.x, s->orig.y, s->orig.z);
   g.dirs = s->dirs;
   g.origs = s->origs;
+  g.order = s->order;
+  const bool list = s->kind == RMH_MODEL_O1DN || s->kind == RMH_MODEL_ONDN;
+  g.grid_w = list ? g.ray_count : g.W;
+  g.grid_h = list ? 1u : g.H;
   return g;
 }
 
@@ -304,6 +310,12 @@ inline Tiling make_tiling(uint32_t W, uint32_t H) {
   return t;
 }
 
+// Tiling of a sensor's image as the kernels cut it (ray lists: one row in traversal order).
+inline Tiling sensor_tiling(const rmh_sensor* s) {
+  const bool list = s->kind == RMH_MODEL_O1DN || s->kind == RMH_MODEL_ONDN;
+  return list ? make_tiling(s->width * s->height, 1u) : make_tiling(s->width, s->height);
+}
+
 // (row, column) of lane `lane` in tile `tile`; false for lanes outside the image.
 __device__ __forceinline__ bool tile_ray(const Tiling& t, uint32_t W, uint32_t H, uint32_t tile, uint32_t lane,
                                          uint32_t& vid, uint32_t& hid) {
@@ -313,9 +325,13 @@ __device__ __forceinline__ bool tile_ray(const Tiling& t, uint32_t W, uint32_t H
   return vid < H && hid < W;
 }
 
-// Sensor-frame ray of (row vid, column hid, buffer id lid).
+// Sensor-frame ray of tile coordinates (vid, hid) and its buffer index lid.  Spherical
+// and pinhole images are tiled directly (lid = row * W + column); ray lists are tiled in
+// traversal order (vid = 0, hid = slot) and lid comes from the order table.
 template <int KIND>
-__device__ __forceinline__ void gen_ray(const RayGen& a, uint32_t vid, uint32_t hid, uint32_t lid, V3& ds, V3& os) {
+__device__ __forceinline__ void gen_ray(const RayGen& a, uint32_t vid, uint32_t hid, uint32_t& lid, V3& ds, V3& os) {
+  if constexpr (KIND == RMH_MODEL_SPHERICAL || KIND == RMH_MODEL_PINHOLE) lid = vid * a.W + hid;
+  else lid = a.order[hid];
   if constexpr (KIND == RMH_MODEL_SPHERICAL) {
     // SphericalModel::getDirection (sensor_models.h:170-175)
     const float2 p = a.lut[vid];
@@ -331,13 +347,13 @@ __device__ __forceinline__ void gen_ray(const RayGen& a, uint32_t vid, uint32_t 
     os = v3(0.0f, 0.0f, 0.0f);
   } else if constexpr (KIND == RMH_MODEL_O1DN) {
     // O1DnModel_::getDirection / getOrigin (sensor_models.h:376-385)
-    const rmh_vec3 dd = a.dirs[lid];
+    const rmh_vec3 dd = a.dirs[hid];
     ds = v3(dd.x, dd.y, dd.z);
     os = a.orig;
   } else {
     // OnDnModel_::getDirection / getOrigin (sensor_models.h:460-470)
-    const rmh_vec3 dd = a.dirs[lid];
-    const rmh_vec3 oo = a.origs[lid];
+    const rmh_vec3 dd = a.dirs[hid];
+    const rmh_vec3 oo = a.origs[hid];
     ds = v3(dd.x, dd.y, dd.z);
     os = v3(oo.x, oo.y, oo.z);
   }

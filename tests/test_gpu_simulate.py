@@ -119,6 +119,30 @@ def test_room_other_models_and_tsb(room):
         compare(g, o, f"room/{type(m).__name__}")
 
 
+def test_ray_list_order_independence(room):
+    """O1Dn/OnDn rays are traced in direction-sorted order; results must follow each ray's
+    own buffer index: permuting the ray list permutes every attribute the same way."""
+    V, F, Tbm, hm, om = room
+    rng = np.random.default_rng(11)
+    n = 4099  # not a multiple of 64: a partial last tile
+    dirs = rng.normal(size=(n, 3))
+    dirs = (dirs / np.linalg.norm(dirs, axis=1, keepdims=True)).astype(np.float32)
+    dirs[7] = 0.0          # zero direction: a miss, and a zero sort key
+    dirs[8] = [0.0, 0.0, -1.0]
+    origs = rng.uniform(-0.3, 0.3, size=(n, 3)).astype(np.float32)
+    perm = rng.permutation(n)
+    for make in (lambda d, o: T.O1DnModel(n, 1, T.Interval(0.1, 60.0), (0.05, -0.02, 0.1), d),
+                 lambda d, o: T.OnDnModel(n, 1, T.Interval(0.1, 60.0), o, d)):
+        a = run_gpu(hm, make(dirs, origs), Tbm[:6])
+        b = run_gpu(hm, make(np.ascontiguousarray(dirs[perm]), np.ascontiguousarray(origs[perm])), Tbm[:6])
+        for k in a:
+            ra = a[k].reshape((6, n) + a[k].shape[1:])
+            rb = b[k].reshape((6, n) + b[k].shape[1:])
+            np.testing.assert_array_equal(rb, ra[:, perm], err_msg=k)
+        o = oracle.simulate(om, make(dirs, origs), Tbm[:6])
+        compare(a, o, "ray list")
+
+
 def test_attribute_subsets(room):
     V, F, Tbm, hm, om = room
     m = scenes.vlp16_workload_model()
