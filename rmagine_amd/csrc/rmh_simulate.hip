@@ -68,17 +68,27 @@ __global__ __launch_bounds__(kWave, 8) void k_simulate(const SimArgs a) {
   const uint32_t lb = xcd_contiguous(blockIdx.x, gridDim.x);
   const uint32_t pl = lb / a.tl.tiles;
   const uint32_t tile = lb - pl * a.tl.tiles;
-  uint32_t vid, hid, lid;
+  uint32_t vid, hid;
   if (!tile_ray(a.tl, a.g.grid_w, a.g.grid_h, tile, threadIdx.x, vid, hid)) return;
   const uint32_t pose = a.pose_offset + pl;
+  Hit h;
+  {
+    uint32_t lid;
+    V3 ds, os;
+    gen_ray<KIND>(a.g, vid, hid, lid, ds, os);
+    const Tf Tsm = tmul(load_tf(a.Tbm[pose]), a.Tsb);  // Tsm = Tbm * Tsb (SphereSimulatorEmbree.tcc:69-70)
+    const V3 dm = qrot(Tsm.R, ds);
+    const V3 om = map_origin<KIND>(Tsm, os);
+    trace<STACK, kWave>(a.map, om, dm, a.g.range_max, stk, h);
+  }
+  // The ray set-up is recomputed for the attribute writers instead of being held through
+  // the traversal: at 64 VGPRs those values would otherwise be spilled to scratch.
+  asm volatile("" : "+v"(vid), "+v"(hid));
+  uint32_t lid;
   V3 ds, os;
   gen_ray<KIND>(a.g, vid, hid, lid, ds, os);
+  const Tf Tsm = tmul(load_tf(a.Tbm[pose]), a.Tsb);
   const uint64_t glob = static_cast<uint64_t>(pose) * a.g.ray_count + lid;
-  const Tf Tsm = tmul(load_tf(a.Tbm[pose]), a.Tsb);  // Tsm = Tbm * Tsb (SphereSimulatorEmbree.tcc:69-70)
-  const V3 dm = qrot(Tsm.R, ds);
-  const V3 om = map_origin<KIND>(Tsm, os);
-  Hit h;
-  trace<STACK, kWave>(a.map, om, dm, a.g.range_max, stk, h);
   write_outputs<KIND>(a.out, glob, a.g, a.map, Tsm, ds, os, h);
 }
 
@@ -131,22 +141,36 @@ __device__ __forceinline__ void batch_gen(const BatchEntryDev& e, uint32_t vid, 
 
 // One batch slot: ray generation and attribute writing depend on the sensor kind, the
 // traversal does not, so it is instantiated once (one copy of the octant-specialised
-// walks instead of one per kind keeps the kernel inside the instruction cache).
+// walks instead of one per kind keeps the kernel inside the instruction cache).  As in
+// k_simulate, the ray set-up is recomputed for the writers rather than held (spilled)
+// through the traversal.
 template <int STACK, int BLOCK>
 __device__ __forceinline__ void batch_ray(const BatchArgs& a, const BatchEntryDev& e, int kind, uint32_t pose,
                                           uint32_t vid, uint32_t hid, int (*stk)[BLOCK]) {
+  Hit h;
+  {
+    const Tf Tsm = tmul(load_tf(e.Tbm[pose]), e.Tsb);
+    uint32_t lid;
+    V3 ds, os, dm, om;
+    switch (kind) {
+      case RMH_MODEL_SPHERICAL: batch_gen<RMH_MODEL_SPHERICAL>(e, vid, hid, lid, Tsm, ds, os, dm, om); break;
+      case RMH_MODEL_PINHOLE: batch_gen<RMH_MODEL_PINHOLE>(e, vid, hid, lid, Tsm, ds, os, dm, om); break;
+      case RMH_MODEL_O1DN: batch_gen<RMH_MODEL_O1DN>(e, vid, hid, lid, Tsm, ds, os, dm, om); break;
+      default: batch_gen<RMH_MODEL_ONDN>(e, vid, hid, lid, Tsm, ds, os, dm, om); break;
+    }
+    trace<STACK, BLOCK>(a.map, om, dm, e.g.range_max, stk, h);
+  }
+  asm volatile("" : "+v"(vid), "+v"(hid));
   const Tf Tsm = tmul(load_tf(e.Tbm[pose]), e.Tsb);
   uint32_t lid;
-  V3 ds, os, dm, om;
+  V3 ds, os;
   switch (kind) {
-    case RMH_MODEL_SPHERICAL: batch_gen<RMH_MODEL_SPHERICAL>(e, vid, hid, lid, Tsm, ds, os, dm, om); break;
-    case RMH_MODEL_PINHOLE: batch_gen<RMH_MODEL_PINHOLE>(e, vid, hid, lid, Tsm, ds, os, dm, om); break;
-    case RMH_MODEL_O1DN: batch_gen<RMH_MODEL_O1DN>(e, vid, hid, lid, Tsm, ds, os, dm, om); break;
-    default: batch_gen<RMH_MODEL_ONDN>(e, vid, hid, lid, Tsm, ds, os, dm, om); break;
+    case RMH_MODEL_SPHERICAL: gen_ray<RMH_MODEL_SPHERICAL>(e.g, vid, hid, lid, ds, os); break;
+    case RMH_MODEL_PINHOLE: gen_ray<RMH_MODEL_PINHOLE>(e.g, vid, hid, lid, ds, os); break;
+    case RMH_MODEL_O1DN: gen_ray<RMH_MODEL_O1DN>(e.g, vid, hid, lid, ds, os); break;
+    default: gen_ray<RMH_MODEL_ONDN>(e.g, vid, hid, lid, ds, os); break;
   }
   const uint64_t r = static_cast<uint64_t>(pose) * e.g.ray_count + lid;
-  Hit h;
-  trace<STACK, BLOCK>(a.map, om, dm, e.g.range_max, stk, h);
   switch (kind) {
     case RMH_MODEL_SPHERICAL: write_outputs<RMH_MODEL_SPHERICAL>(e.out, r, e.g, a.map, Tsm, ds, os, h); break;
     case RMH_MODEL_PINHOLE: write_outputs<RMH_MODEL_PINHOLE>(e.out, r, e.g, a.map, Tsm, ds, os, h); break;
