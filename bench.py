@@ -28,6 +28,23 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# rocprofv3 FETCH_SIZE/WRITE_SIZE summary of the default workload's traversal kernel
+TRAFFIC_PROFILE = os.path.join("profiles", "k_simulate_terrain_1000x128x1024.md")
+
+
+def profiled_traffic(default_workload: bool):
+    """HBM-side bytes per launch measured with rocprofv3 for the default workload (the
+    profile summary records them on a `traffic_bytes_per_launch:` line), or None."""
+    if not default_workload:
+        return None
+    try:
+        with open(os.path.join(ROOT, TRAFFIC_PROFILE)) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return int(line.split(":", 1)[1].strip())
+    except (OSError, ValueError):
+        pass
+    return None
 BYTES_PER_RAY_ALL = 1 + 4 + 12 + 12 + 4 + 4 + 4  # hits, ranges, points, normals, 3 ids
 
 
@@ -330,6 +347,7 @@ def main():
     if not args.no_cpu:
         cpu, accuracy = cpu_baseline(V, F, model, Tbm, args.cpu_seconds, gpu_numpy_for_pose)
 
+    traffic = profiled_traffic(args.grid == 1024 and args.sensors == 1000)
     out = {
         "metric": "Mrays/s at <=1e-5 m range err vs Embree semantics (simulate, all attributes)",
         "value": round(value, 2),
@@ -355,7 +373,8 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 5),
-            "traffic": None,
+            "traffic": traffic,
+            "traffic_source": TRAFFIC_PROFILE if traffic else None,
             "kernel": "k_simulate<spherical> (traversal + attribute writers)",
             "kernel_avg_ms": round(avg_kernel_s * 1e3, 4),
             "alg_bytes_per_launch": alg_bytes,
