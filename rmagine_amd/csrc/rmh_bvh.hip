@@ -123,9 +123,12 @@ __global__ __launch_bounds__(kBlock) void k_morton(const float4* __restrict__ lo
   if (f >= nf) return;
   const float4 a = lo[f], b = hi[f];
   const float c[3] = {0.5f * a.x + 0.5f * b.x, 0.5f * a.y + 0.5f * b.y, 0.5f * a.z + 0.5f * b.z};
+  // One scale for all three axes (a cube around the scene): with a per-axis scale, a flat
+  // scene such as a 100 m x 100 m x 16 m terrain gets as many z splits as x splits and the
+  // top of the tree cuts thin slabs (SAH cost 46.8 instead of 32.5 on that terrain).
   uint32_t q[3];
+  const float ext = fmaxf(fmaxf(bounds[3] - bounds[0], bounds[4] - bounds[1]), bounds[5] - bounds[2]);
   for (int k = 0; k < 3; k++) {
-    const float ext = bounds[3 + k] - bounds[k];
     float u = ext > 0.0f ? (c[k] - bounds[k]) / ext : 0.5f;
     u = fminf(fmaxf(u, 0.0f), 1.0f);
     q[k] = static_cast<uint32_t>(u * 2097151.0f);
