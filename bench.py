@@ -120,9 +120,10 @@ def extra_batched(cpu_seconds: float, do_cpu: bool) -> dict:
     for m, _ in sensors:
         mix[{0: "vlp16", 1: "pinhole", 2: "o1dn"}[m.kind]] += 1
     key = autotune.cache_key(len(F), mix)
-    tuned = autotune.tune_batch(batch, key=key, chunks=(64, 256, 1024), waves=(16, 32), reps=2, use_cache=False)
+    tuned = autotune.tune_batch(batch, key=key, chunks=(64, 256, 1024), waves=(16, 32), tile_rows=(2, 4, 8),
+                                reps=2, use_cache=False)
     ms = autotune.time_batch(batch, reps=5)
-    log(f"batched: {rays} rays, {ms:.2f} ms -> {rays / ms / 1e3:.1f} Mrays/s (tuned {tuned['chunk_rays']}/{tuned['waves_per_cu']})")
+    log(f"batched: {rays} rays, {ms:.2f} ms -> {rays / ms / 1e3:.1f} Mrays/s (tuned {tuned['chunk_rays']}/{tuned['waves_per_cu']}/{tuned['tile_rows']})")
     # the same sensors with one rmh_simulate launch each
     sims = [simulator_for(m, hm) for m, _ in sensors]
     bundles = [None] * len(sims)
@@ -144,7 +145,7 @@ def extra_batched(cpu_seconds: float, do_cpu: bool) -> dict:
                        f"(800 VLP-16, 600 pinhole 640x480 fov 90, 600 O1Dn 1k-50k rays), hits+ranges",
            "rays": rays, "batched_ms": round(ms, 3), "batched_mrays_s": round(rays / ms / 1e3, 1),
            "per_sensor_launches_ms": round(plain_ms, 3), "per_sensor_mrays_s": round(rays / plain_ms / 1e3, 1),
-           "tuned": {k: tuned[k] for k in ("chunk_rays", "waves_per_cu")},
+           "tuned": {k: tuned[k] for k in ("chunk_rays", "waves_per_cu", "tile_rows")},
            "batched_vs_per_sensor_mismatches": mism}
     if do_cpu:
         import oracle

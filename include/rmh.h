@@ -176,6 +176,9 @@ typedef struct {
 typedef struct {
   uint32_t chunk_rays;    /* rays per work grab, multiple of 64 (0 = keep / default 256)  */
   uint32_t waves_per_cu;  /* persistent waves per compute unit (0 = keep / default 32)   */
+  uint32_t tile_rows;     /* image rows per 64-ray tile of spherical/pinhole sensors:
+                             1, 2, 4, 8, 16, 32 or 64, capped by the image height
+                             (0 = keep / default 4)                                      */
 } rmh_batch_params;
 
 typedef struct rmh_batch rmh_batch;

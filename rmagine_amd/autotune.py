@@ -1,11 +1,14 @@
 """Autotuner for the batched persistent launch (rmh_batch_*).
 
-The persistent kernel has two launch parameters: rays handed out per work grab
-(`chunk_rays`, a multiple of 64) and resident waves per compute unit (`waves_per_cu`,
-which sets the persistent grid).  Small chunks balance uneven sensors better, large ones
-cost fewer atomics; more waves hide more latency until LDS and registers run out.  The
-tuner times every candidate on the caller's actual batch with HIP events and keeps the
-fastest.  Results are cached as JSON keyed by device architecture, mesh size (power of
+The persistent kernel has three launch parameters: rays handed out per work grab
+(`chunk_rays`, a multiple of 64), resident waves per compute unit (`waves_per_cu`, which
+sets the persistent grid) and the shape of the 64-ray tile a wave traces (`tile_rows` rows
+x 64/tile_rows columns of a spherical or pinhole image).  Small chunks balance uneven
+sensors better, large ones cost fewer atomics; more waves hide more latency until LDS and
+registers run out; the tile shape decides how coherent a wave's rays are (on a spinning
+lidar wide tiles win because rays of one row end at similar ranges).  The tuner times
+every candidate on the caller's actual batch with HIP events and keeps the fastest:
+first chunk x waves at the default tile, then the tile rows at the best pair.  Results are cached as JSON keyed by device architecture, mesh size (power of
 two bucket of the face count) and sensor mix, in $RMH_TUNE_CACHE or
 ~/.cache/rmagine_amd/autotune.json.
 
@@ -22,6 +25,7 @@ import torch
 
 DEFAULT_CHUNKS = (64, 128, 256, 512, 1024)
 DEFAULT_WAVES = (8, 16, 24, 32)
+DEFAULT_TILE_ROWS = (1, 2, 4, 8, 16)
 
 
 def cache_path() -> Path:
@@ -70,27 +74,35 @@ def time_batch(batch, reps: int = 3, stream=None) -> float:
     return ms[len(ms) // 2]
 
 
-def tune_batch(batch, key: str | None = None, chunks=DEFAULT_CHUNKS, waves=DEFAULT_WAVES, reps: int = 3,
-               use_cache: bool = True, stream=None) -> dict:
-    """Picks (chunk_rays, waves_per_cu) for `batch`, applies it and returns the record."""
+def tune_batch(batch, key: str | None = None, chunks=DEFAULT_CHUNKS, waves=DEFAULT_WAVES,
+               tile_rows=DEFAULT_TILE_ROWS, reps: int = 3, use_cache: bool = True, stream=None) -> dict:
+    """Picks (chunk_rays, waves_per_cu, tile_rows) for `batch`, applies it and returns the record."""
     path = cache_path()
     if use_cache and key:
         rec = _load(path).get(key)
         if rec:
-            batch.set_params(chunk_rays=rec["chunk_rays"], waves_per_cu=rec["waves_per_cu"])
+            batch.set_params(chunk_rays=rec["chunk_rays"], waves_per_cu=rec["waves_per_cu"],
+                             tile_rows=rec.get("tile_rows", 4))
             return dict(rec, cached=True)
     trials = []
+    rows0 = batch.params()["tile_rows"]
     for c in chunks:
         for w in waves:
-            batch.set_params(chunk_rays=c, waves_per_cu=w)
-            trials.append((time_batch(batch, reps, stream), c, w))
+            batch.set_params(chunk_rays=c, waves_per_cu=w, tile_rows=rows0)
+            trials.append((time_batch(batch, reps, stream), c, w, rows0))
+    ms, c, w, r = min(trials)
+    for rr in tile_rows:
+        if rr == rows0:
+            continue
+        batch.set_params(chunk_rays=c, waves_per_cu=w, tile_rows=rr)
+        trials.append((time_batch(batch, reps, stream), c, w, rr))
     trials.sort()
-    ms, c, w = trials[0]
-    batch.set_params(chunk_rays=c, waves_per_cu=w)
-    rec = {"chunk_rays": c, "waves_per_cu": w, "ms": ms,
-           "trials": [{"chunk_rays": tc, "waves_per_cu": tw, "ms": tm} for tm, tc, tw in trials]}
+    ms, c, w, r = trials[0]
+    batch.set_params(chunk_rays=c, waves_per_cu=w, tile_rows=r)
+    rec = {"chunk_rays": c, "waves_per_cu": w, "tile_rows": r, "ms": ms,
+           "trials": [{"chunk_rays": tc, "waves_per_cu": tw, "tile_rows": tr, "ms": tm} for tm, tc, tw, tr in trials]}
     if key:
         data = _load(path)
-        data[key] = {k: rec[k] for k in ("chunk_rays", "waves_per_cu", "ms")}
+        data[key] = {k: rec[k] for k in ("chunk_rays", "waves_per_cu", "tile_rows", "ms")}
         _store(path, data)
     return dict(rec, cached=False)

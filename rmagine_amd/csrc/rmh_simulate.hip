@@ -234,6 +234,8 @@ struct rmh_batch {
   rmh_batch_params params{};
   uint32_t n_cu = 256;
   std::vector<uint64_t> entry_rays;
+  std::vector<BatchEntryDev> host;     // host copy of the table (re-tiled by set_params)
+  std::vector<const rmh_sensor*> sensors;
 };
 
 namespace {
@@ -244,6 +246,20 @@ void batch_free(rmh_batch* b) {
   if (b->chunk_prefix) (void)hipFree(b->chunk_prefix);
   if (b->counter) (void)hipFree(b->counter);
   delete b;
+}
+
+bool valid_tile_rows(uint32_t r) { return r >= 1 && r <= 64 && (r & (r - 1)) == 0; }
+
+// Tiling, slot and ray counts of every entry for the batch's tile_rows.
+void batch_tile(rmh_batch* b) {
+  b->total_rays = 0;
+  for (uint32_t i = 0; i < b->n_entries; i++) {
+    BatchEntryDev& h = b->host[i];
+    h.tl = sensor_tiling(b->sensors[i], b->params.tile_rows);
+    h.n_slots = static_cast<uint64_t>(h.n_poses) * h.tl.tiles * 64u;
+    b->entry_rays[i] = h.n_slots;
+    b->total_rays += static_cast<uint64_t>(h.n_poses) * h.g.ray_count;
+  }
 }
 
 int batch_upload(rmh_batch* b, const std::vector<BatchEntryDev>& host) {
@@ -323,39 +339,44 @@ int rmh_batch_create(const rmh_map* map, const rmh_batch_entry* entries, uint32_
   b->n_entries = n_entries;
   b->params.chunk_rays = 256;
   b->params.waves_per_cu = 32;
+  b->params.tile_rows = kDefaultTileRows;
   if (params) {
     if (params->chunk_rays) b->params.chunk_rays = params->chunk_rays;
     if (params->waves_per_cu) b->params.waves_per_cu = params->waves_per_cu;
+    if (params->tile_rows) b->params.tile_rows = params->tile_rows;
   }
   if (b->params.chunk_rays % 64 != 0 || b->params.chunk_rays > (1u << 20)) {
     batch_free(b);
     return fail(RMH_ERR_INVALID, "rmh_batch_create: chunk_rays must be a multiple of 64 (<= 2^20)");
   }
+  if (b->params.waves_per_cu > 64 || !valid_tile_rows(b->params.tile_rows)) {
+    batch_free(b);
+    return fail(RMH_ERR_INVALID, "rmh_batch_create: waves_per_cu must be <= 64 and tile_rows a power of two <= 64");
+  }
   int dev = 0, ncu = 0;
   (void)hipGetDevice(&dev);
   if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && ncu > 0)
     b->n_cu = static_cast<uint32_t>(ncu);
-  std::vector<BatchEntryDev> host(n_entries);
+  b->host.resize(n_entries);
+  b->sensors.resize(n_entries);
   b->entry_rays.resize(n_entries);
   for (uint32_t i = 0; i < n_entries; i++) {
     const rmh_batch_entry& e = entries[i];
     if (!e.sensor) { batch_free(b); return fail(RMH_ERR_INVALID, "rmh_batch_create: entry without sensor"); }
     if (e.n_poses > 0 && !e.Tbm_dev) { batch_free(b); return fail(RMH_ERR_INVALID, "rmh_batch_create: entry without poses"); }
-    host[i].g = make_raygen(e.sensor);
-    host[i].Tsb = load_tf(e.Tsb);
-    host[i].Tbm = e.Tbm_dev;
-    host[i].out = e.out;
-    host[i].tl = sensor_tiling(e.sensor);
-    host[i].n_poses = e.n_poses;
-    host[i].n_slots = static_cast<uint64_t>(e.n_poses) * host[i].tl.tiles * 64u;
-    b->entry_rays[i] = host[i].n_slots;
-    b->total_rays += static_cast<uint64_t>(e.n_poses) * host[i].g.ray_count;
+    b->sensors[i] = e.sensor;
+    b->host[i].g = make_raygen(e.sensor);
+    b->host[i].Tsb = load_tf(e.Tsb);
+    b->host[i].Tbm = e.Tbm_dev;
+    b->host[i].out = e.out;
+    b->host[i].n_poses = e.n_poses;
   }
+  batch_tile(b);
   hipError_t err = hipMalloc(&b->entries, sizeof(BatchEntryDev) * std::max<uint32_t>(1, n_entries));
   if (err == hipSuccess) err = hipMalloc(&b->chunk_prefix, sizeof(uint32_t) * (n_entries + 1));
   if (err == hipSuccess) err = hipMalloc(&b->counter, sizeof(uint32_t));
   if (err != hipSuccess) { batch_free(b); return hip_fail(err, "rmh_batch_create"); }
-  if (int rc = batch_upload(b, host)) { batch_free(b); return rc; }
+  if (int rc = batch_upload(b, b->host)) { batch_free(b); return rc; }
   *out = b;
   return RMH_OK;
 }
@@ -365,13 +386,21 @@ int rmh_batch_set_params(rmh_batch* b, const rmh_batch_params* params) {
   rmh_batch_params p = b->params;
   if (params->chunk_rays) p.chunk_rays = params->chunk_rays;
   if (params->waves_per_cu) p.waves_per_cu = params->waves_per_cu;
+  if (params->tile_rows) p.tile_rows = params->tile_rows;
   if (p.chunk_rays % 64 != 0 || p.chunk_rays > (1u << 20))
     return fail(RMH_ERR_INVALID, "rmh_batch_set_params: chunk_rays must be a multiple of 64 (<= 2^20)");
   if (p.waves_per_cu > 64) return fail(RMH_ERR_INVALID, "rmh_batch_set_params: waves_per_cu must be <= 64");
-  const bool rechunk = p.chunk_rays != b->params.chunk_rays;
+  if (!valid_tile_rows(p.tile_rows))
+    return fail(RMH_ERR_INVALID, "rmh_batch_set_params: tile_rows must be a power of two <= 64");
+  const bool retile = p.tile_rows != b->params.tile_rows;
+  const bool rechunk = retile || p.chunk_rays != b->params.chunk_rays;
   b->params = p;
   if (rechunk) {
-    RMH_HIP_TRY(hipDeviceSynchronize());  // the prefix table may be in use by a running launch
+    RMH_HIP_TRY(hipDeviceSynchronize());  // the tables may be in use by a running launch
+    if (retile) {
+      batch_tile(b);
+      return batch_upload(b, b->host);
+    }
     return batch_upload(b, {});
   }
   return RMH_OK;

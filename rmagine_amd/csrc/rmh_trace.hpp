@@ -301,9 +301,13 @@ struct Tiling {
   uint32_t tw, th, tiles_w, tiles;  // tiles per pose image
 };
 
-inline Tiling make_tiling(uint32_t W, uint32_t H) {
+constexpr uint32_t kDefaultTileRows = 4;
+
+// rows: wanted tile height (a power of two <= 64), capped by the image height.
+inline Tiling make_tiling(uint32_t W, uint32_t H, uint32_t rows = kDefaultTileRows) {
   Tiling t;
-  t.th = H >= 4 ? 4u : (H >= 2 ? 2u : 1u);
+  t.th = 1;
+  while (t.th * 2 <= rows && t.th * 2 <= H && t.th < 64) t.th *= 2;
   t.tw = 64u / t.th;
   t.tiles_w = (W + t.tw - 1) / t.tw;
   t.tiles = t.tiles_w * ((H + t.th - 1) / t.th);
@@ -311,9 +315,9 @@ inline Tiling make_tiling(uint32_t W, uint32_t H) {
 }
 
 // Tiling of a sensor's image as the kernels cut it (ray lists: one row in traversal order).
-inline Tiling sensor_tiling(const rmh_sensor* s) {
+inline Tiling sensor_tiling(const rmh_sensor* s, uint32_t rows = kDefaultTileRows) {
   const bool list = s->kind == RMH_MODEL_O1DN || s->kind == RMH_MODEL_ONDN;
-  return list ? make_tiling(s->width * s->height, 1u) : make_tiling(s->width, s->height);
+  return list ? make_tiling(s->width * s->height, 1u) : make_tiling(s->width, s->height, rows);
 }
 
 // (row, column) of lane `lane` in tile `tile`; false for lanes outside the image.

@@ -255,7 +255,8 @@ class BatchSimulatorHip:
     into a HipGraph.  The reference has no batched form (one optixLaunch per model).
     """
 
-    def __init__(self, map: HipMap, entries, chunk_rays: int = 0, waves_per_cu: int = 0, device="cuda"):
+    def __init__(self, map: HipMap, entries, chunk_rays: int = 0, waves_per_cu: int = 0, tile_rows: int = 0,
+                 device="cuda"):
         self._map = map
         self.device = torch.device(device)
         self._sensors, self._keep, self.bundles = [], [], []
@@ -279,12 +280,14 @@ class BatchSimulatorHip:
             self._keep.append(Tbm)
             self.bundles.append(b)
             self.total_rays += sh.W * sh.H * n
-        prm = T.rmh_batch_params(chunk_rays, waves_per_cu)
+        prm = T.rmh_batch_params(chunk_rays, waves_per_cu, tile_rows)
         self._h = ctypes.c_void_p()
         _lib.call("rmh_batch_create", map.handle, table, len(entries), ctypes.byref(prm), ctypes.byref(self._h))
 
-    def set_params(self, chunk_rays: int = 0, waves_per_cu: int = 0) -> None:
-        prm = T.rmh_batch_params(chunk_rays, waves_per_cu)
+    def set_params(self, chunk_rays: int = 0, waves_per_cu: int = 0, tile_rows: int = 0) -> None:
+        """Launch shape: rays per work grab, persistent waves per CU, image rows per 64-ray
+        tile (0 keeps the current value)."""
+        prm = T.rmh_batch_params(chunk_rays, waves_per_cu, tile_rows)
         _lib.call("rmh_batch_set_params", self._h, ctypes.byref(prm))
 
     def params(self) -> dict:
@@ -292,8 +295,8 @@ class BatchSimulatorHip:
         rays = ctypes.c_uint64()
         chunks = ctypes.c_uint32()
         _lib.call("rmh_batch_get_info", self._h, ctypes.byref(rays), ctypes.byref(chunks), ctypes.byref(prm))
-        return {"chunk_rays": prm.chunk_rays, "waves_per_cu": prm.waves_per_cu, "total_rays": rays.value,
-                "total_chunks": chunks.value}
+        return {"chunk_rays": prm.chunk_rays, "waves_per_cu": prm.waves_per_cu, "tile_rows": prm.tile_rows,
+                "total_rays": rays.value, "total_chunks": chunks.value}
 
     def simulate(self, stream=None) -> list:
         _lib.call("rmh_batch_simulate", self._h, stream_handle(stream))

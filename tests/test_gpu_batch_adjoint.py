@@ -15,6 +15,7 @@ import torch
 
 import oracle
 from oracle import stats as S
+from rmagine_amd import _lib
 from rmagine_amd import adjoint as A
 from rmagine_amd import icp
 from rmagine_amd import math as M
@@ -56,8 +57,9 @@ def test_batch_matches_plain_and_oracle(room):
     rng = np.random.default_rng(1)
     entries = _mixed_entries(Tbm, rng)
     batch = BatchSimulatorHip(hm, entries)
-    for chunk in (256, 64, 1024):
-        batch.set_params(chunk_rays=chunk, waves_per_cu=8 if chunk == 64 else 16)
+    for chunk, rows in ((256, 4), (64, 1), (1024, 16), (256, 64), (64, 8)):
+        batch.set_params(chunk_rays=chunk, waves_per_cu=8 if chunk == 64 else 16, tile_rows=rows)
+        assert batch.params()["tile_rows"] == rows
         bundles = batch.simulate()
         torch.cuda.synchronize()
         for e, b in zip(entries, bundles):
@@ -67,7 +69,11 @@ def test_batch_matches_plain_and_oracle(room):
             torch.cuda.synchronize()
             g, r = b.to_numpy(), ref.to_numpy()
             for k in T.ATTRIBUTES:
-                np.testing.assert_array_equal(g[k].view(np.uint8), r[k].view(np.uint8), err_msg=f"{k} chunk {chunk}")
+                np.testing.assert_array_equal(g[k].view(np.uint8), r[k].view(np.uint8), err_msg=f"{k} chunk {chunk} rows {rows}")
+    for bad in ({"tile_rows": 3}, {"tile_rows": 128}, {"chunk_rays": 100}):
+        with pytest.raises(_lib.RmhError):
+            batch.set_params(**bad)
+    assert batch.params()["tile_rows"] == 8  # a rejected call changes nothing
     # oracle on two entries
     for i in (1, 3):
         e = entries[i]
