@@ -8,6 +8,8 @@
 #include <rmh.h>
 
 #include <memory>
+#include <stdexcept>
+#include <string>
 #include <vector>
 
 #include <rmagine/math/types.h>
@@ -57,11 +59,43 @@ class HipMap {
     return i;
   }
 
+  // Geometry / object id reported for each mesh (empty vector keeps the mesh index).
+  void setMeshIds(const std::vector<uint32_t>& geom_ids, const std::vector<uint32_t>& object_ids) {
+    const uint32_t n = info().n_meshes;
+    if ((!geom_ids.empty() && geom_ids.size() != n) || (!object_ids.empty() && object_ids.size() != n))
+      throw std::runtime_error("HipMap::setMeshIds: one id per mesh expected");
+    RM_RMH_CHECK(rmh_map_set_mesh_ids(m_map, geom_ids.empty() ? nullptr : geom_ids.data(),
+                                      object_ids.empty() ? nullptr : object_ids.data(), n));
+  }
+
  private:
   rmh_map* m_map = nullptr;
   void* m_stream = nullptr;
 };
 
 using HipMapPtr = std::shared_ptr<HipMap>;
+
+// Mesh file (PLY, OBJ, STL, COLLADA) as a map, with the scene graph resolved and the ids
+// assigned as Rmagine's import_embree_map / import_optix_map do
+// (src/rmagine_embree/include/rmagine/map/EmbreeMap.hpp:94-117,
+//  src/rmagine_optix/include/rmagine/map/OptixMap.hpp:88-109).  flags:
+// RMH_SCENE_POLYGONS_FAN triangulates polygons (default: their first three indices).
+static HipMapPtr import_hip_map(const std::string& meshfile, uint32_t flags = 0,
+                                const rmh_bvh_params* params = nullptr) {
+  rmh_scene* sc = nullptr;
+  RM_RMH_CHECK(rmh_scene_load(meshfile.c_str(), flags, &sc));
+  uint32_t nv = 0, nf = 0, nm = 0;
+  (void)rmh_scene_counts(sc, &nv, &nf, &nm);
+  Memory<Vector, RAM> V(nv);
+  Memory<Face, RAM> F(nf);
+  std::vector<uint32_t> off(nm + 1), geom(nm), obj(nm);
+  const int rc = rmh_scene_copy(sc, reinterpret_cast<float*>(V.raw()), reinterpret_cast<uint32_t*>(F.raw()),
+                                off.data(), geom.data(), obj.data());
+  (void)rmh_scene_destroy(sc);
+  RM_RMH_CHECK(rc);
+  auto map = std::make_shared<HipMap>(Memory<Vector, VRAM_HIP>(V), Memory<Face, VRAM_HIP>(F), off, params);
+  map->setMeshIds(geom, obj);
+  return map;
+}
 
 }  // namespace rmagine

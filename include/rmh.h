@@ -137,6 +137,31 @@ int rmh_map_build(const float* vertices_dev, uint32_t n_vertices,
                   const rmh_bvh_params* params, void* stream, rmh_map** out);
 int rmh_map_destroy(rmh_map* map);
 int rmh_map_get_info(const rmh_map* map, rmh_map_info* info);
+/* Per-mesh geometry and object ids (host arrays of n_meshes = the map's mesh count; NULL
+ * keeps that id).  By default both are the mesh index.  The ids a hit reports follow
+ * Embree's: geomID = rtcAttachGeometry order (src/rmagine_embree/src/map/embree/EmbreeScene.cpp:54-71),
+ * object id = instID[0] when the hit went through an instance, else geomID
+ * (src/rmagine_embree/include/rmagine/simulation/SphereSimulatorEmbree.tcc:175-185). */
+int rmh_map_set_mesh_ids(rmh_map* map, const uint32_t* geom_ids, const uint32_t* object_ids, uint32_t n_meshes);
+
+/* ---- map files ----------------------------------------------------------------------------
+ * Replaces import_embree_map / import_optix_map
+ * (src/rmagine_embree/include/rmagine/map/EmbreeMap.hpp:94-117,
+ *  src/rmagine_optix/include/rmagine/map/OptixMap.hpp:88-109) without Assimp.  Formats: PLY
+ * (ascii, binary little/big endian), OBJ, STL (ascii/binary), COLLADA .dae.  The scene graph
+ * is flattened as make_embree_scene + freeze build it (EmbreeScene.cpp:246-290,317-406):
+ * vertices transformed by their node's global transform, meshes in final geometry order,
+ * with the geom and object ids of that scene.  Host memory only. */
+#define RMH_SCENE_POLYGONS_FAN 1u  /* triangulate polygons as fans (default: a polygon keeps
+                                      its first three indices, as EmbreeMesh::init does) */
+typedef struct rmh_scene rmh_scene;
+int rmh_scene_load(const char* path, uint32_t flags, rmh_scene** out);
+int rmh_scene_counts(const rmh_scene* scene, uint32_t* n_vertices, uint32_t* n_faces, uint32_t* n_meshes);
+/* Copies into host arrays of 3*n_vertices floats, 3*n_faces, n_meshes+1, n_meshes and
+ * n_meshes uint32 (any may be NULL). */
+int rmh_scene_copy(const rmh_scene* scene, float* vertices, uint32_t* faces, uint32_t* mesh_face_offsets,
+                   uint32_t* geom_ids, uint32_t* object_ids);
+int rmh_scene_destroy(rmh_scene* scene);
 
 /* ---- sensors ------------------------------------------------------------------------ */
 /* model points at the host struct of `kind` (rmh_spherical_model, rmh_pinhole_model,

@@ -47,6 +47,8 @@ def lib():
         _lib.oracle_map_build.argtypes = [P, ctypes.c_uint32, P, ctypes.c_uint32, P, ctypes.c_uint32]
         _lib.oracle_map_free.restype = None
         _lib.oracle_map_free.argtypes = [P]
+        _lib.oracle_map_set_mesh_ids.restype = None
+        _lib.oracle_map_set_mesh_ids.argtypes = [P, P, P]
         _lib.oracle_simulate.restype = ctypes.c_int
         _lib.oracle_simulate.argtypes = [P, ctypes.c_int, P, P, P, P, P, ctypes.c_uint32,
                                          P, P, P, P, P, P, P, P, ctypes.c_int, ctypes.c_int]
@@ -58,7 +60,7 @@ def _p(a):
 
 
 class OracleMap:
-    def __init__(self, V, F, mesh_face_offsets=None):
+    def __init__(self, V, F, mesh_face_offsets=None, geom_ids=None, object_ids=None):
         self.V = np.ascontiguousarray(np.asarray(V, dtype=np.float32).reshape(-1, 3))
         self.F = np.ascontiguousarray(np.asarray(F, dtype=np.uint32).reshape(-1, 3))
         off = None
@@ -67,6 +69,10 @@ class OracleMap:
             off = np.ascontiguousarray(np.asarray(mesh_face_offsets, dtype=np.uint32))
             n_meshes = len(off) - 1
         self._h = lib().oracle_map_build(_p(self.V), len(self.V), _p(self.F), len(self.F), _p(off), n_meshes)
+        if geom_ids is not None or object_ids is not None:
+            g = None if geom_ids is None else np.ascontiguousarray(np.asarray(geom_ids, dtype=np.uint32))
+            o = None if object_ids is None else np.ascontiguousarray(np.asarray(object_ids, dtype=np.uint32))
+            lib().oracle_map_set_mesh_ids(self._h, _p(g), _p(o))
 
     def __del__(self):
         if getattr(self, "_h", None):

@@ -102,7 +102,7 @@ struct BNode {
 struct Map {
   std::vector<float> V;
   std::vector<uint32_t> F;
-  std::vector<uint32_t> geom, local;
+  std::vector<uint32_t> geom, local, obj, mesh;  // per face: geom id, id within its mesh, object id, mesh index
   std::vector<double> nrm;     // unit normal (FP64), 3 per face
   std::vector<int> order;
   std::vector<BNode> nodes;
@@ -398,12 +398,16 @@ void* oracle_map_build(const float* V, uint32_t nv, const uint32_t* F, uint32_t 
   m->nf = nf;
   m->geom.resize(nf);
   m->local.resize(nf);
+  m->obj.resize(nf);
+  m->mesh.resize(nf);
   std::vector<uint32_t> off;
   if (mesh_off && n_meshes > 0) off.assign(mesh_off, mesh_off + n_meshes + 1);
   else off = {0u, nf};
   for (size_t g = 0; g + 1 < off.size(); g++)
     for (uint32_t f = off[g]; f < off[g + 1]; f++) {
       m->geom[f] = static_cast<uint32_t>(g);
+      m->obj[f] = static_cast<uint32_t>(g);
+      m->mesh[f] = static_cast<uint32_t>(g);
       m->local[f] = f - off[g];
     }
   m->nrm.resize(3 * static_cast<size_t>(nf));
@@ -428,6 +432,16 @@ void* oracle_map_build(const float* V, uint32_t nv, const uint32_t* F, uint32_t 
 }
 
 void oracle_map_free(void* m) { delete static_cast<Map*>(m); }
+
+// Per-mesh geom / object ids (NULL keeps); the object id is instID[0] for a hit through an
+// instance, else the geomID (SphereSimulatorEmbree.tcc:175-185).
+void oracle_map_set_mesh_ids(void* mp, const uint32_t* geom_ids, const uint32_t* object_ids) {
+  Map& m = *static_cast<Map*>(mp);
+  for (uint32_t f = 0; f < m.nf; f++) {
+    if (geom_ids) m.geom[f] = geom_ids[m.mesh[f]];
+    if (object_ids) m.obj[f] = object_ids[m.mesh[f]];
+  }
+}
 
 // model_f: spherical {phi.min, phi.inc, phi.size, theta.min, theta.inc, theta.size, rmin, rmax}
 //          pinhole   {width, height, rmin, rmax, f0, f1, c0, c1}
@@ -512,7 +526,7 @@ int oracle_simulate(void* mp, int kind, const float* model_f, const float* dirs,
           }
           if (face_ids) face_ids[g] = m.local[f];
           if (geom_ids) geom_ids[g] = m.geom[f];
-          if (object_ids) object_ids[g] = m.geom[f];
+          if (object_ids) object_ids[g] = m.obj[f];
         } else {
           if (q.near_t <= M.rmax * (1.0 + kBand)) fl |= 2;
           const float qn = std::numeric_limits<float>::quiet_NaN();

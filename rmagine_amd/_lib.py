@@ -50,6 +50,11 @@ _SIGNATURES = {
                               ctypes.POINTER(T.rmh_bvh_params), c_void_p, ctypes.POINTER(c_void_p)]),
     "rmh_map_destroy": (c_int, [c_void_p]),
     "rmh_map_get_info": (c_int, [c_void_p, ctypes.POINTER(T.rmh_map_info)]),
+    "rmh_map_set_mesh_ids": (c_int, [c_void_p, ctypes.POINTER(c_u32), ctypes.POINTER(c_u32), c_u32]),
+    "rmh_scene_load": (c_int, [ctypes.c_char_p, c_u32, ctypes.POINTER(c_void_p)]),
+    "rmh_scene_counts": (c_int, [c_void_p, ctypes.POINTER(c_u32), ctypes.POINTER(c_u32), ctypes.POINTER(c_u32)]),
+    "rmh_scene_copy": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "rmh_scene_destroy": (c_int, [c_void_p]),
     "rmh_sensor_create": (c_int, [c_int, c_void_p, ctypes.POINTER(c_void_p)]),
     "rmh_sensor_destroy": (c_int, [c_void_p]),
     "rmh_sensor_dims": (c_int, [c_void_p, ctypes.POINTER(c_u32), ctypes.POINTER(c_u32)]),

@@ -2,8 +2,8 @@
 
     python -m rmagine_amd.build          # or rmagine_amd.build.build()
 
-Each csrc/*.hip file is compiled to an object under rmagine_amd/_build/ and linked into
-rmagine_amd/_lib/librmh.so.  Every HIP translation unit is built with
+Each csrc/*.hip file (and each host-only csrc/*.cpp file) is compiled to an object under
+rmagine_amd/_build/ and linked into rmagine_amd/_lib/librmh.so.  Every HIP translation unit is built with
 -ffp-contract=off (the pose/ray algebra must not be contracted into FMAs so that it
 reproduces the reference's FP32 rounding) and with packed FP32 instructions disabled.
 """
@@ -44,9 +44,16 @@ def _hipcc() -> str:
     raise RuntimeError("hipcc not found")
 
 
+# Host-only C++ (map file loaders): plain g++.
+CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", f"-I{INCLUDE}"]
+
+
 def _compile(src: Path, extra: list[str]) -> Path:
     obj = BUILD / (src.stem + ".o")
-    cmd = [_hipcc(), *HIP_FLAGS, *extra, "-c", str(src), "-o", str(obj)]
+    if src.suffix == ".cpp":
+        cmd = [os.environ.get("CXX", "g++"), *CXX_FLAGS, "-c", str(src), "-o", str(obj)]
+    else:
+        cmd = [_hipcc(), *HIP_FLAGS, *extra, "-c", str(src), "-o", str(obj)]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         raise RuntimeError(f"compile failed: {' '.join(cmd)}\n{res.stdout}\n{res.stderr}")
@@ -54,10 +61,11 @@ def _compile(src: Path, extra: list[str]) -> Path:
 
 
 def build(verbose: bool = True, extra_flags: list[str] | None = None) -> Path:
-    """Compile every csrc/*.hip and link librmh.so.  Always rebuilds."""
+    """Compile every csrc/*.hip (hipcc) and csrc/*.cpp (host C++) and link librmh.so.
+    Always rebuilds."""
     BUILD.mkdir(exist_ok=True)
     LIBDIR.mkdir(exist_ok=True)
-    srcs = sorted(CSRC.glob("*.hip"))
+    srcs = sorted(CSRC.glob("*.hip")) + sorted(CSRC.glob("*.cpp"))
     extra = list(extra_flags or [])
     jobs = min(len(srcs), int(os.environ.get("MAX_JOBS", "8")))
     with cf.ThreadPoolExecutor(max_workers=max(1, jobs)) as ex:

@@ -323,9 +323,27 @@ __global__ __launch_bounds__(kBlock) void k_gather(uint32_t nf, const uint32_t* 
   const double bx = (double)x2 - (double)x0, by = (double)y2 - (double)y0, bz = (double)z2 - (double)z0;
   const double nx = ay * bz - az * by, ny = az * bx - ax * bz, nz = ax * by - ay * bx;
   const double len = sqrt(nx * nx + ny * ny + nz * nz);
-  float4 nrm = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-  if (len > 0.0) nrm = make_float4((float)(nx / len), (float)(ny / len), (float)(nz / len), 0.0f);
+  // w: object id of the face (the mesh index until rmh_map_set_mesh_ids says otherwise)
+  float4 nrm = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(g));
+  if (len > 0.0) nrm = make_float4((float)(nx / len), (float)(ny / len), (float)(nz / len), __uint_as_float(g));
   normals[k] = nrm;
+}
+
+// Per-mesh geom / object ids: for every Morton slot, the mesh of its global face (v2.w)
+// selects the ids (NULL table = keep).
+__global__ __launch_bounds__(kBlock) void k_set_ids(uint32_t nf, const uint32_t* __restrict__ mesh_off, uint32_t n_meshes,
+                                                    const uint32_t* __restrict__ geom, const uint32_t* __restrict__ obj,
+                                                    float4* __restrict__ tris, float4* __restrict__ normals) {
+  const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
+  if (k >= nf) return;
+  const uint32_t f = __float_as_uint(tris[3 * static_cast<size_t>(k) + 2].w);
+  uint32_t lo = 0, hi = n_meshes;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (mesh_off[mid] <= f) lo = mid; else hi = mid;
+  }
+  if (geom) tris[3 * static_cast<size_t>(k) + 1].w = __uint_as_float(geom[lo]);
+  if (obj) normals[k].w = __uint_as_float(obj[lo]);
 }
 
 struct DevBuf {
@@ -387,6 +405,7 @@ extern "C" int rmh_map_build(const float* vertices, uint32_t nv, const uint32_t*
   m->info.n_vertices = nv;
   m->info.n_faces = nf;
   m->info.n_meshes = n_meshes;
+  m->mesh_offsets = off;
   if (nf == 0) {  // empty map: every ray misses
     m->root = 0;
     *out = m;
@@ -511,6 +530,33 @@ extern "C" int rmh_map_build(const float* vertices, uint32_t nv, const uint32_t*
 
 extern "C" int rmh_map_destroy(rmh_map* map) {
   map_free(map);
+  return RMH_OK;
+}
+
+extern "C" int rmh_map_set_mesh_ids(rmh_map* map, const uint32_t* geom_ids, const uint32_t* object_ids,
+                                    uint32_t n_meshes) {
+  if (!map) return fail(RMH_ERR_INVALID, "rmh_map_set_mesh_ids: NULL map");
+  if (n_meshes != map->n_meshes)
+    return fail(RMH_ERR_INVALID, "rmh_map_set_mesh_ids: n_meshes differs from the map's mesh count");
+  if (map->n_faces == 0 || (!geom_ids && !object_ids)) return RMH_OK;
+  const uint32_t n = map->n_meshes;
+  std::vector<uint32_t> host;
+  host.reserve(3 * static_cast<size_t>(n) + 1);
+  host.insert(host.end(), map->mesh_offsets.begin(), map->mesh_offsets.end());      // n + 1
+  if (geom_ids) host.insert(host.end(), geom_ids, geom_ids + n);
+  if (object_ids) host.insert(host.end(), object_ids, object_ids + n);
+  uint32_t* dev = nullptr;
+  RMH_HIP_TRY(hipMalloc(&dev, sizeof(uint32_t) * host.size()));
+  hipError_t e = hipMemcpy(dev, host.data(), sizeof(uint32_t) * host.size(), hipMemcpyHostToDevice);
+  if (e == hipSuccess) {
+    const uint32_t* g = geom_ids ? dev + n + 1 : nullptr;
+    const uint32_t* o = object_ids ? dev + n + 1 + (geom_ids ? n : 0) : nullptr;
+    k_set_ids<<<blocks_for(map->n_faces), kBlock>>>(map->n_faces, dev, n, g, o, map->tris, map->normals);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  (void)hipFree(dev);
+  if (e != hipSuccess) return hip_fail(e, "rmh_map_set_mesh_ids");
   return RMH_OK;
 }
 

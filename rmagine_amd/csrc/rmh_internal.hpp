@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 #include <cstdint>
 #include <string>
+#include <vector>
 #include <rmh.h>
 
 #define RMH_HD __host__ __device__ __forceinline__
@@ -129,6 +130,7 @@ struct rmh_map {
   uint32_t n_internal = 0;
   rmh_map_info info{};
   int device = 0;
+  std::vector<uint32_t> mesh_offsets;  // host copy: faces of mesh g are [off[g], off[g+1])
 };
 
 struct rmh_sensor {

@@ -386,9 +386,10 @@ __device__ __forceinline__ void write_outputs(const rmh_outputs& out, uint64_t g
       rmh_vec3 pv; pv.x = p.x; pv.y = p.y; pv.z = p.z;
       out.points[glob] = pv;
     }
+    float4 nn = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (out.normals || out.object_ids) nn = map.normals[h.slot];  // w = object id
     if (out.normals) {
       // face normal into the sensor frame, flipped toward the sensor, normalised
-      const float4 nn = map.normals[h.slot];
       V3 n = qrot(qconj(Tsm.R), v3(nn.x, nn.y, nn.z));
       if (dot(ds, n) > 0.0f) n = scale(n, -1.0f);
       n = normalize(n);
@@ -401,7 +402,8 @@ __device__ __forceinline__ void write_outputs(const rmh_outputs& out, uint64_t g
       const uint32_t geom = __float_as_uint(map.tris[s + 1].w);
       if (out.face_ids) out.face_ids[glob] = face;
       if (out.geom_ids) out.geom_ids[glob] = geom;
-      if (out.object_ids) out.object_ids[glob] = geom;  // no instances: object id = geom id
+      // object id: the instance id for meshes of a multi-mesh instance, else the geom id
+      if (out.object_ids) out.object_ids[glob] = __float_as_uint(nn.w);
     }
   } else {
     const float qnan = __builtin_nanf("");

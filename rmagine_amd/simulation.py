@@ -108,6 +108,19 @@ class HipMap:
     def handle(self) -> ctypes.c_void_p:
         return self._handle
 
+    def set_mesh_ids(self, geom_ids=None, object_ids=None) -> None:
+        """Geometry / object id reported for each mesh (defaults: the mesh index)."""
+        n = self.info()["n_meshes"]
+
+        def arr(x):
+            if x is None:
+                return None
+            a = np.ascontiguousarray(np.asarray(x, dtype=np.uint32))
+            if a.shape != (n,):
+                raise ValueError(f"expected {n} ids")
+            return (ctypes.c_uint32 * n)(*a.tolist())
+        _lib.call("rmh_map_set_mesh_ids", self._handle, arr(geom_ids), arr(object_ids), n)
+
     def info(self) -> dict:
         inf = T.rmh_map_info()
         _lib.call("rmh_map_get_info", self._handle, ctypes.byref(inf))
@@ -121,6 +134,44 @@ class HipMap:
             except Exception:
                 pass
             self._handle = ctypes.c_void_p()
+
+
+def load_map_file(path, polygons: str = "first") -> dict:
+    """Reads a mesh file (PLY, OBJ, STL, COLLADA) into flat host arrays, with the scene graph
+    resolved as Rmagine's import does (import_embree_map: Assimp, make_embree_scene, freeze;
+    src/rmagine_embree/include/rmagine/map/EmbreeMap.hpp:94-117).
+
+    polygons: "first" keeps the first three indices of every polygon (the reference's
+    EmbreeMesh::init), "fan" triangulates polygons as fans.
+    Returns vertices (V,3) float32, faces (F,3) uint32, mesh_face_offsets (M+1,), geom_ids
+    and object_ids (M,) uint32.
+    """
+    if polygons not in ("first", "fan"):
+        raise ValueError("polygons must be 'first' or 'fan'")
+    flags = 1 if polygons == "fan" else 0
+    h = ctypes.c_void_p()
+    _lib.call("rmh_scene_load", str(path).encode(), flags, ctypes.byref(h))
+    try:
+        nv, nf, nm = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint32()
+        _lib.call("rmh_scene_counts", h, ctypes.byref(nv), ctypes.byref(nf), ctypes.byref(nm))
+        out = {"vertices": np.empty((nv.value, 3), np.float32), "faces": np.empty((nf.value, 3), np.uint32),
+               "mesh_face_offsets": np.empty(nm.value + 1, np.uint32), "geom_ids": np.empty(nm.value, np.uint32),
+               "object_ids": np.empty(nm.value, np.uint32)}
+        _lib.call("rmh_scene_copy", h, *(out[k].ctypes.data for k in
+                                         ("vertices", "faces", "mesh_face_offsets", "geom_ids", "object_ids")))
+    finally:
+        _lib.lib().rmh_scene_destroy(h)
+    return out
+
+
+def import_hip_map(path, polygons: str = "first", device="cuda", **bvh) -> HipMap:
+    """HipMap of a mesh file: the counterpart of import_embree_map / import_optix_map
+    (src/rmagine_optix/include/rmagine/map/OptixMap.hpp:88-109).  Hits report the geometry
+    and object ids Rmagine's Embree scene would give the same file."""
+    s = load_map_file(path, polygons)
+    m = HipMap(s["vertices"], s["faces"], s["mesh_face_offsets"], device=device, **bvh)
+    m.set_mesh_ids(s["geom_ids"], s["object_ids"])
+    return m
 
 
 class _SensorHandle:

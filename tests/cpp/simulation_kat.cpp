@@ -149,13 +149,33 @@ static void test_no_map() {
   require(thrown, "simulate without a map throws");
 }
 
-int main() {
+// import_hip_map on the reference's own map files (copies under tests/golden/maps).
+static void test_import(const std::string& dir) {
+  // sphere.ply: icosphere of radius 1 (642 vertices, 1280 faces); every ray from the centre
+  // hits, no farther than 1 and no nearer than the faces' distance to the centre (> 0.98)
+  const HipMapPtr sph = import_hip_map(dir + "/sphere.ply");
+  require(sph->info().n_faces == 1280 && sph->info().n_meshes == 1, "sphere.ply counts");
+  SphereSimulatorHip sim(sph);
+  sim.setModel(example_spherical());
+  auto res = sim.simulate<IntAttrAll<VRAM_HIP>>(Transform::Identity());
+  Memory<uint8_t, RAM> hits = res.hits;
+  Memory<float, RAM> ranges = res.ranges;
+  Memory<unsigned int, RAM> geom = res.geom_ids;
+  for (size_t i = 0; i < ranges.size(); i++)
+    require(hits[i] == 1 && ranges[i] > 0.98f && ranges[i] <= 1.0f + 1e-6f && geom[i] == 0, "sphere.ply ray");
+  // two_cubes.dae: two single-mesh nodes, frozen into geometries 0 and 1 (node order)
+  const HipMapPtr cubes = import_hip_map(dir + "/two_cubes.dae");
+  require(cubes->info().n_meshes == 2 && cubes->info().n_faces == 24, "two_cubes.dae counts");
+}
+
+int main(int argc, char** argv) {
   const HipMapPtr map = make_cube_map();
   test_spherical(map);
   test_o1dn(map);
   test_pinhole(map);
   test_ondn(map);
   test_no_map();
+  if (argc > 1) test_import(argv[1]);
   std::cout << "simulation_kat: all C++ drop-in checks passed" << std::endl;
   return 0;
 }

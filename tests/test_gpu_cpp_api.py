@@ -13,6 +13,7 @@ BUILD = Path(__file__).resolve().parent / "cpp" / "_build"
 def test_cpp_program(prog):
     exe = BUILD / prog
     assert exe.exists(), f"{exe} not built (run __graft_entry__.build())"
-    res = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
+    maps = Path(__file__).resolve().parent / "golden" / "maps"
+    res = subprocess.run([str(exe), str(maps)], capture_output=True, text=True, timeout=120)
     print(res.stdout, res.stderr)
     assert res.returncode == 0, res.stdout + res.stderr
