@@ -220,6 +220,23 @@ int rmh_batch_get_info(const rmh_batch* batch, uint64_t* total_rays, uint32_t* t
 int rmh_batch_simulate(rmh_batch* batch, void* stream);
 int rmh_batch_destroy(rmh_batch* batch);
 
+/* ---- range noise -------------------------------------------------------------------------
+ * Replaces GaussianNoiseCuda / RelGaussianNoiseCuda / UniformDustNoiseCuda::apply
+ * (src/rmagine_cuda/src/noise/GaussianNoiseCuda.cu:7-48, RelGaussianNoiseCuda.cu:7-54,
+ *  UniformDustNoiseCuda.cu:7-69), in place on a device range buffer.  The random numbers are
+ * counter-based (Philox4x32-10 of element index and `epoch` under `seed`): no per-element
+ * state; pass a new epoch for new numbers (the wrappers count calls). */
+#define RMH_NOISE_GAUSSIAN 0u      /* p0 = mean, p1 = stddev                              */
+#define RMH_NOISE_REL_GAUSSIAN 1u  /* p0 = mean, p1 = stddev, p2 = range exponent         */
+#define RMH_NOISE_UNIFORM_DUST 2u  /* p0 = hit probability per metre, p1 = return prob.   */
+typedef struct {
+  uint32_t kind;
+  float p0, p1, p2;
+  float max_range;  /* Gaussian kinds: ranges above are left alone; dust: range cap  */
+  uint32_t seed;
+} rmh_noise_params;
+int rmh_noise_apply(const rmh_noise_params* params, float* ranges_dev, uint64_t n, uint32_t epoch, void* stream);
+
 /* ---- graphs: record a sequence of rmh_* launches on a stream once, replay it ------------
  * No counterpart in the reference.  Capture needs a non-default stream; nothing captured
  * may allocate or synchronise (pass workspaces to the reductions). */

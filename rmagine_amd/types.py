@@ -348,3 +348,11 @@ def make_transforms(t: np.ndarray, q: np.ndarray) -> np.ndarray:
 def transform_to_c(T) -> rmh_transform:
     T = np.asarray(T, dtype=np.float32).reshape(8)
     return rmh_transform(rmh_quat(*map(float, T[0:4])), rmh_vec3(*map(float, T[4:7])), 0)
+
+
+NOISE_GAUSSIAN, NOISE_REL_GAUSSIAN, NOISE_UNIFORM_DUST = 0, 1, 2
+
+
+class rmh_noise_params(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_uint32), ("p0", ctypes.c_float), ("p1", ctypes.c_float), ("p2", ctypes.c_float),
+                ("max_range", ctypes.c_float), ("seed", ctypes.c_uint32)]

@@ -10,6 +10,7 @@
 //   single ray  tests/embree/simulation_single_ray.cpp:24-70 (range 0.5)
 // plus the error convention (no map -> std::runtime_error).  Throws on failure.
 #include <rmagine/map/HipMap.hpp>
+#include <rmagine/noise/NoiseHip.hpp>
 #include <rmagine/simulation/O1DnSimulatorHip.hpp>
 #include <rmagine/simulation/OnDnSimulatorHip.hpp>
 #include <rmagine/simulation/PinholeSimulatorHip.hpp>
@@ -168,6 +169,24 @@ static void test_import(const std::string& dir) {
   require(cubes->info().n_meshes == 2 && cubes->info().n_faces == 24, "two_cubes.dae counts");
 }
 
+// GaussianNoiseHip on simulated ranges (GaussianNoiseCuda usage, e.g. apps' noise demos):
+// stddev 0 makes the result exact: every range <= max_range moves by the mean.
+static void test_noise(const HipMapPtr& map) {
+  SphereSimulatorHip sim(map);
+  sim.setModel(example_spherical());
+  auto res = sim.simulate<Bundle<Ranges<VRAM_HIP>>>(Transform::Identity());
+  Memory<float, RAM> before = res.ranges;
+  GaussianNoiseHip::Options opt;
+  opt.max_range = 50.0f;
+  GaussianNoiseHip noise(0.25f, 0.0f, opt);
+  noise.apply(res.ranges);
+  Memory<float, RAM> after = res.ranges;
+  for (size_t i = 0; i < after.size(); i++) {
+    const float want = before[i] <= 50.0f ? before[i] + 0.25f : before[i];
+    require(std::fabs(after[i] - want) <= 1e-6f, "noise shift");
+  }
+}
+
 int main(int argc, char** argv) {
   const HipMapPtr map = make_cube_map();
   test_spherical(map);
@@ -175,6 +194,7 @@ int main(int argc, char** argv) {
   test_pinhole(map);
   test_ondn(map);
   test_no_map();
+  test_noise(map);
   if (argc > 1) test_import(argv[1]);
   std::cout << "simulation_kat: all C++ drop-in checks passed" << std::endl;
   return 0;
