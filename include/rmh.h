@@ -163,6 +163,13 @@ int rmh_scene_copy(const rmh_scene* scene, float* vertices, uint32_t* faces, uin
                    uint32_t* geom_ids, uint32_t* object_ids);
 int rmh_scene_destroy(rmh_scene* scene);
 
+/* O1Dn model of an Ouster sensor from its metadata JSON: replaces
+ * o1dn_from_ouster_meta_file (src/rmagine_ouster/src/types/ouster_sensors.cpp:8-80).  Fills
+ * width (columns of lidar_mode) and height (beams); with dirs != NULL also the W*H unit
+ * directions, row-major (3 floats each, capacity in floats).  The reference's model has
+ * origin (0,0,0) and range [0.1, 80]. */
+int rmh_ouster_o1dn(const char* path, uint32_t* width, uint32_t* height, float* dirs, size_t capacity);
+
 /* ---- sensors ------------------------------------------------------------------------ */
 /* model points at the host struct of `kind` (rmh_spherical_model, rmh_pinhole_model,
  * rmh_o1dn_model, rmh_ondn_model).  The sensor keeps its own device copy of everything

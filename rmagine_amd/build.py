@@ -45,7 +45,7 @@ def _hipcc() -> str:
 
 
 # Host-only C++ (map file loaders): plain g++.
-CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", f"-I{INCLUDE}"]
+CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", f"-I{INCLUDE}"]
 
 
 def _compile(src: Path, extra: list[str]) -> Path:

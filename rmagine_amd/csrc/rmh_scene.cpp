@@ -823,6 +823,101 @@ Flat flatten(const SceneData& sc, bool fan) {
   return out;
 }
 
+// ---- Ouster sensor metadata (JSON) ------------------------------------------------------------
+struct JVal {
+  enum Kind { NUL, BOOL, NUM, STR, ARR, OBJ } kind = NUL;
+  double num = 0.0;
+  std::string str;
+  std::vector<JVal> arr;
+  std::vector<std::pair<std::string, JVal>> obj;
+  const JVal* get(const std::string& k) const {
+    for (auto& kv : obj) if (kv.first == k) return &kv.second;
+    return nullptr;
+  }
+};
+
+struct JParser {
+  const std::string& s;
+  size_t i = 0;
+  void ws() { while (i < s.size() && std::isspace(static_cast<unsigned char>(s[i]))) i++; }
+  [[noreturn]] void bad() { throw LoadError("JSON: syntax error at byte " + std::to_string(i)); }
+  std::string string() {
+    if (s[i] != '"') bad();
+    std::string out;
+    for (i++; i < s.size() && s[i] != '"'; i++) {
+      if (s[i] == '\\' && i + 1 < s.size()) {
+        i++;
+        const char e = s[i];
+        out += e == 'n' ? '\n' : e == 't' ? '\t' : e == 'u' ? (i += 4, '?') : e;
+      } else {
+        out += s[i];
+      }
+    }
+    if (i >= s.size()) bad();
+    i++;
+    return out;
+  }
+  JVal value(int depth = 0) {
+    if (depth > 64) bad();
+    ws();
+    if (i >= s.size()) bad();
+    JVal v;
+    const char c = s[i];
+    if (c == '{') {
+      v.kind = JVal::OBJ;
+      i++;
+      ws();
+      if (s[i] == '}') { i++; return v; }
+      for (;;) {
+        ws();
+        std::string k = string();
+        ws();
+        if (s[i] != ':') bad();
+        i++;
+        v.obj.emplace_back(k, value(depth + 1));
+        ws();
+        if (s[i] == ',') { i++; continue; }
+        if (s[i] == '}') { i++; return v; }
+        bad();
+      }
+    }
+    if (c == '[') {
+      v.kind = JVal::ARR;
+      i++;
+      ws();
+      if (s[i] == ']') { i++; return v; }
+      for (;;) {
+        v.arr.push_back(value(depth + 1));
+        ws();
+        if (s[i] == ',') { i++; continue; }
+        if (s[i] == ']') { i++; return v; }
+        bad();
+      }
+    }
+    if (c == '"') { v.kind = JVal::STR; v.str = string(); return v; }
+    if (s.compare(i, 4, "true") == 0) { v.kind = JVal::BOOL; v.num = 1; i += 4; return v; }
+    if (s.compare(i, 5, "false") == 0) { v.kind = JVal::BOOL; i += 5; return v; }
+    if (s.compare(i, 4, "null") == 0) { i += 4; return v; }
+    char* e = nullptr;
+    v.num = std::strtod(s.c_str() + i, &e);
+    if (e == s.c_str() + i) bad();
+    v.kind = JVal::NUM;
+    i = static_cast<size_t>(e - s.c_str());
+    return v;
+  }
+};
+
+std::vector<float> float_array(const JVal& root, const char* key) {
+  const JVal* a = root.get(key);
+  if (!a || a->kind != JVal::ARR) throw LoadError(std::string("Ouster metadata without '") + key + "'");
+  std::vector<float> out;
+  for (auto& v : a->arr) {
+    if (v.kind != JVal::NUM) throw LoadError(std::string("Ouster metadata: non-numeric '") + key + "'");
+    out.push_back(static_cast<float>(v.num));  // Json::Value::asFloat
+  }
+  return out;
+}
+
 }  // namespace
 
 struct rmh_scene {
@@ -882,6 +977,52 @@ int rmh_scene_copy(const rmh_scene* s, float* vertices, uint32_t* faces, uint32_
 int rmh_scene_destroy(rmh_scene* s) {
   delete s;
   return RMH_OK;
+}
+
+// o1dn_from_ouster_meta_file (src/rmagine_ouster/src/types/ouster_sensors.cpp:8-80): width
+// from "lidar_mode" ("<columns>x<Hz>"), one row per beam; the direction of (row v, column h)
+// is (cos phi cos theta, cos phi sin theta, sin phi), phi = altitude[v] * DEG_TO_RAD_F,
+// theta = 0 + (2 pi / W) * h + azimuth[v] * DEG_TO_RAD_F, in FP32 as there, with the
+// correctly rounded cos/sin of the FP32 angles.
+int rmh_ouster_o1dn(const char* path, uint32_t* width, uint32_t* height, float* dirs, size_t capacity) {
+  if (!path || !width || !height) return rmh::fail(RMH_ERR_INVALID, "rmh_ouster_o1dn: NULL argument");
+  try {
+    const std::string data = read_file(path);
+    JParser jp{data};
+    const JVal root = jp.value();
+    if (root.kind != JVal::OBJ) throw LoadError("Ouster metadata is not a JSON object");
+    const JVal* mode = root.get("lidar_mode");
+    if (!mode || mode->kind != JVal::STR) throw LoadError("Ouster metadata without 'lidar_mode'");
+    const size_t x = mode->str.find('x');
+    const long w = std::strtol(mode->str.substr(0, x).c_str(), nullptr, 10);
+    if (x == std::string::npos || w <= 0 || w > (1 << 20)) throw LoadError("bad lidar_mode '" + mode->str + "'");
+    const std::vector<float> alt = float_array(root, "beam_altitude_angles");
+    const std::vector<float> az = float_array(root, "beam_azimuth_angles");
+    if (alt.empty() || az.size() < alt.size()) throw LoadError("Ouster metadata: beam angle arrays do not match");
+    const uint32_t W = static_cast<uint32_t>(w), H = static_cast<uint32_t>(alt.size());
+    *width = W;
+    *height = H;
+    if (!dirs) return RMH_OK;
+    if (capacity < 3ull * W * H) return rmh::fail(RMH_ERR_INVALID, "rmh_ouster_o1dn: dirs buffer too small");
+    constexpr float kDegToRad = 0.017453292519943295f;
+    const float hor_angle_inc = static_cast<float>((2.0 * M_PI) / static_cast<float>(W));
+    for (uint32_t v = 0; v < H; v++) {
+      const float phi = alt[v] * kDegToRad;
+      const float cp = static_cast<float>(std::cos(static_cast<double>(phi)));
+      const float sp = static_cast<float>(std::sin(static_cast<double>(phi)));
+      for (uint32_t h = 0; h < W; h++) {
+        const float theta_offset = az[v] * kDegToRad;
+        const float theta = (0.0f + hor_angle_inc * static_cast<float>(h)) + theta_offset;
+        float* d = dirs + 3 * (static_cast<size_t>(v) * W + h);
+        d[0] = cp * static_cast<float>(std::cos(static_cast<double>(theta)));
+        d[1] = cp * static_cast<float>(std::sin(static_cast<double>(theta)));
+        d[2] = sp;
+      }
+    }
+    return RMH_OK;
+  } catch (const std::exception& e) {
+    return rmh::fail(RMH_ERR_INVALID, std::string("rmh_ouster_o1dn: ") + e.what());
+  }
 }
 
 }  // extern "C"

@@ -11,6 +11,7 @@
 // plus the error convention (no map -> std::runtime_error).  Throws on failure.
 #include <rmagine/map/HipMap.hpp>
 #include <rmagine/noise/NoiseHip.hpp>
+#include <rmagine/types/ouster_sensors.h>
 #include <rmagine/simulation/O1DnSimulatorHip.hpp>
 #include <rmagine/simulation/OnDnSimulatorHip.hpp>
 #include <rmagine/simulation/PinholeSimulatorHip.hpp>
@@ -164,6 +165,15 @@ static void test_import(const std::string& dir) {
   Memory<unsigned int, RAM> geom = res.geom_ids;
   for (size_t i = 0; i < ranges.size(); i++)
     require(hits[i] == 1 && ranges[i] > 0.98f && ranges[i] <= 1.0f + 1e-6f && geom[i] == 0, "sphere.ply ray");
+  // the reference's OS0-128 metadata as an O1Dn model, cast from the sphere's centre
+  const O1DnModel os0 = o1dn_from_ouster_meta_file(dir + "/../ouster/os0-128x1024x20-meta.json");
+  require(os0.width == 1024 && os0.height == 128, "ouster model size");
+  O1DnSimulatorHip osim(sph);
+  osim.setModel(os0);
+  auto ores = osim.simulate<Bundle<Hits<VRAM_HIP>, Ranges<VRAM_HIP>>>(Transform::Identity());
+  Memory<uint8_t, RAM> oh = ores.hits;
+  Memory<float, RAM> orr = ores.ranges;
+  for (size_t i = 0; i < orr.size(); i++) require(oh[i] == 1 && orr[i] > 0.98f && orr[i] <= 1.0f + 1e-6f, "ouster ray");
   // two_cubes.dae: two single-mesh nodes, frozen into geometries 0 and 1 (node order)
   const HipMapPtr cubes = import_hip_map(dir + "/two_cubes.dae");
   require(cubes->info().n_meshes == 2 && cubes->info().n_faces == 24, "two_cubes.dae counts");

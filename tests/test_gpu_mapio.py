@@ -12,6 +12,7 @@ import torch
 
 import oracle
 from rmagine_amd import scenes
+from rmagine_amd.ouster import o1dn_from_ouster_meta_file
 from rmagine_amd import types as T
 from rmagine_amd.simulation import import_hip_map, load_map_file, simulator_for
 
@@ -96,3 +97,12 @@ def test_multi_mesh_instance_ids(tmp_path):
     np.testing.assert_array_equal(g["geom_ids"][::3], [0, 0, 1])
     np.testing.assert_array_equal(g["object_ids"][::3], [0, 1, 1])
     np.testing.assert_allclose(g["ranges"][::3], [1.0, 1.3, 0.8], atol=1e-6)
+
+
+def test_ouster_model_in_sphere():
+    """The reference's OS0-128 metadata as an O1Dn model, cast inside sphere.ply: GPU vs oracle."""
+    hm, om, s = _both(MAPS / "sphere.ply")
+    m = o1dn_from_ouster_meta_file(MAPS.parent / "ouster" / "os0-128x1024x20-meta.json")
+    P = _poses(3, -0.2, 0.2, 4)
+    g = _compare(hm, om, m, P, "ouster")
+    assert g["hits"].all()
