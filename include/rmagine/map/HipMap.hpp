@@ -7,6 +7,7 @@
 
 #include <rmh.h>
 
+#include <limits>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -57,6 +58,39 @@ class HipMap {
     rmh_map_info i{};
     RM_RMH_CHECK(rmh_map_get_info(m_map, &i));
     return i;
+  }
+
+  // Nearest map point strictly within max_distance (EmbreeMap::closestPoint,
+  // src/rmagine_embree/src/map/EmbreeMap.cpp:33-37; result {d, p, n, primID, geomID},
+  // d = FLT_MAX and ids 0xFFFFFFFF when nothing is that close).  Blocking, one query; see
+  // rmh_closest_points for batches.
+  struct ClosestPointResult {
+    float d;
+    Vector p;
+    Vector n;
+    unsigned int primID;
+    unsigned int geomID;
+  };
+  ClosestPointResult closestPoint(const Vector& qp, float max_distance = std::numeric_limits<float>::max()) const {
+    Memory<Vector, RAM> q(1);
+    q[0] = qp;
+    Memory<Vector, VRAM_HIP> qd;
+    qd = q;
+    Memory<float, VRAM_HIP> d(1);
+    Memory<Vector, VRAM_HIP> p(1), n(1);
+    Memory<unsigned int, VRAM_HIP> f(1), g(1);
+    rmh_closest_outputs out{};
+    out.dist = d.raw();
+    out.points = reinterpret_cast<rmh_vec3*>(p.raw());
+    out.normals = reinterpret_cast<rmh_vec3*>(n.raw());
+    out.face_ids = f.raw();
+    out.geom_ids = g.raw();
+    RM_RMH_CHECK(rmh_closest_points(m_map, reinterpret_cast<const rmh_vec3*>(qd.raw()), 1, max_distance, &out,
+                                    m_stream));
+    Memory<float, RAM> dh = d;
+    Memory<Vector, RAM> ph = p, nh = n;
+    Memory<unsigned int, RAM> fh = f, gh = g;
+    return ClosestPointResult{dh[0], ph[0], nh[0], fh[0], gh[0]};
   }
 
   // Geometry / object id reported for each mesh (empty vector keeps the mesh index).

@@ -144,6 +144,22 @@ int rmh_map_get_info(const rmh_map* map, rmh_map_info* info);
  * (src/rmagine_embree/include/rmagine/simulation/SphereSimulatorEmbree.tcc:175-185). */
 int rmh_map_set_mesh_ids(rmh_map* map, const uint32_t* geom_ids, const uint32_t* object_ids, uint32_t n_meshes);
 
+/* Closest points: for n query points (device, 12 B each) the nearest point of the map
+ * strictly within max_distance.  Replaces EmbreeMap::closestPoint
+ * (src/rmagine_embree/src/map/EmbreeMap.cpp:33-37, EmbreeScene.cpp:292-315, callback
+ * EmbreeMesh.cpp:22-131) for a batch.  Every output pointer (device) may be NULL.  Misses:
+ * dist FLT_MAX, ids 0xFFFFFFFF, point and normal NaN.  Equal distances: lower global face. */
+typedef struct {
+  float* dist;
+  rmh_vec3* points;
+  rmh_vec3* normals;      /* unit face normal of the closest triangle */
+  uint32_t* face_ids;
+  uint32_t* geom_ids;
+  uint32_t* object_ids;
+} rmh_closest_outputs;
+int rmh_closest_points(const rmh_map* map, const rmh_vec3* queries_dev, uint32_t n, float max_distance,
+                       const rmh_closest_outputs* out, void* stream);
+
 /* ---- map files ----------------------------------------------------------------------------
  * Replaces import_embree_map / import_optix_map
  * (src/rmagine_embree/include/rmagine/map/EmbreeMap.hpp:94-117,

@@ -55,6 +55,8 @@ _SIGNATURES = {
     "rmh_scene_counts": (c_int, [c_void_p, ctypes.POINTER(c_u32), ctypes.POINTER(c_u32), ctypes.POINTER(c_u32)]),
     "rmh_scene_copy": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "rmh_scene_destroy": (c_int, [c_void_p]),
+    "rmh_closest_points": (c_int, [c_void_p, c_void_p, c_u32, ctypes.c_float, ctypes.POINTER(T.rmh_closest_outputs),
+                                   c_void_p]),
     "rmh_ouster_o1dn": (c_int, [ctypes.c_char_p, ctypes.POINTER(c_u32), ctypes.POINTER(c_u32), c_void_p, c_size]),
     "rmh_noise_apply": (c_int, [ctypes.POINTER(T.rmh_noise_params), c_void_p, ctypes.c_uint64, c_u32, c_void_p]),
     "rmh_sensor_create": (c_int, [c_int, c_void_p, ctypes.POINTER(c_void_p)]),

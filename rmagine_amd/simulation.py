@@ -121,6 +121,30 @@ class HipMap:
             return (ctypes.c_uint32 * n)(*a.tolist())
         _lib.call("rmh_map_set_mesh_ids", self._handle, arr(geom_ids), arr(object_ids), n)
 
+    def closest_points(self, queries: torch.Tensor, max_distance: float = float(np.finfo(np.float32).max),
+                       stream=None) -> dict:
+        """Nearest map point of every query ((N,3) float32 device tensor) strictly within
+        max_distance: the batched form of EmbreeMap::closestPoint
+        (src/rmagine_embree/src/map/EmbreeMap.cpp:33-37).  Returns dist (N,), points (N,3),
+        normals (N,3), face_ids, geom_ids, object_ids (N,) device tensors; misses have dist =
+        FLT_MAX, ids 0xFFFFFFFF (as int32 -1) and NaN point/normal."""
+        if not (isinstance(queries, torch.Tensor) and queries.dtype == torch.float32 and queries.is_cuda
+                and queries.is_contiguous() and queries.dim() == 2 and queries.shape[1] == 3):
+            raise ValueError("queries must be a contiguous (N,3) float32 CUDA tensor")
+        n = queries.shape[0]
+        dev = queries.device
+        res = {"dist": torch.empty(n, dtype=torch.float32, device=dev),
+               "points": torch.empty((n, 3), dtype=torch.float32, device=dev),
+               "normals": torch.empty((n, 3), dtype=torch.float32, device=dev),
+               "face_ids": torch.empty(n, dtype=torch.int32, device=dev),
+               "geom_ids": torch.empty(n, dtype=torch.int32, device=dev),
+               "object_ids": torch.empty(n, dtype=torch.int32, device=dev)}
+        out = T.rmh_closest_outputs(*(res[k].data_ptr() for k in ("dist", "points", "normals", "face_ids",
+                                                                   "geom_ids", "object_ids")))
+        _lib.call("rmh_closest_points", self._handle, ctypes.c_void_p(queries.data_ptr()), n,
+                  ctypes.c_float(max_distance), ctypes.byref(out), stream_handle(stream))
+        return res
+
     def info(self) -> dict:
         inf = T.rmh_map_info()
         _lib.call("rmh_map_get_info", self._handle, ctypes.byref(inf))

@@ -356,3 +356,8 @@ NOISE_GAUSSIAN, NOISE_REL_GAUSSIAN, NOISE_UNIFORM_DUST = 0, 1, 2
 class rmh_noise_params(ctypes.Structure):
     _fields_ = [("kind", ctypes.c_uint32), ("p0", ctypes.c_float), ("p1", ctypes.c_float), ("p2", ctypes.c_float),
                 ("max_range", ctypes.c_float), ("seed", ctypes.c_uint32)]
+
+
+class rmh_closest_outputs(ctypes.Structure):
+    _fields_ = [("dist", ctypes.c_void_p), ("points", ctypes.c_void_p), ("normals", ctypes.c_void_p),
+                ("face_ids", ctypes.c_void_p), ("geom_ids", ctypes.c_void_p), ("object_ids", ctypes.c_void_p)]

@@ -1,0 +1,67 @@
+"""rmh_closest_points against the FP64 brute-force oracle (same FP32 map vertices)."""
+import numpy as np
+import pytest
+import torch
+from pathlib import Path
+
+from oracle import closest as C
+from rmagine_amd import scenes
+from rmagine_amd.simulation import HipMap, import_hip_map, load_map_file
+
+pytestmark = pytest.mark.gpu
+MAPS = Path(__file__).resolve().parent / "golden" / "maps"
+
+
+def _check(hm, V, F, Q, max_distance=C.FLT_MAX, label="", offsets=None):
+    r = hm.closest_points(torch.as_tensor(Q, device="cuda"), max_distance)
+    torch.cuda.synchronize()
+    d, p, f = C.closest_points(V, F, Q, max_distance)
+    if offsets is not None:  # the library reports the face index within its mesh (primID)
+        off = np.asarray(offsets)
+        f = np.where(f >= 0, f - off[np.searchsorted(off, f, side="right") - 1], f)
+    gd = r["dist"].cpu().numpy()
+    gf = r["face_ids"].cpu().numpy().astype(np.int64)
+    np.testing.assert_array_equal(gd == C.FLT_MAX, d == C.FLT_MAX, err_msg=label)
+    ok = d != C.FLT_MAX
+    np.testing.assert_allclose(gd[ok], d[ok], rtol=2e-7, atol=1e-7, err_msg=label)
+    np.testing.assert_allclose(r["points"].cpu().numpy()[ok], p[ok], atol=2e-6, err_msg=label)
+    np.testing.assert_array_equal(gf[~ok], -1)
+    same = gf[ok] == f[ok]  # a different face can only be an equal-distance tie
+    assert same.mean() > 0.99, label
+    return r
+
+
+def test_cube_known_answers_reference_scene():
+    """The reference's closest-point test scene (two unit cubes at z = 5, x = 0 and 20)."""
+    V, F = scenes.genCube()
+    V = np.asarray(V, np.float32)
+    V2 = np.concatenate([V + np.float32([0, 0, 5]), V + np.float32([20, 0, 5])])
+    F2 = np.concatenate([F, F + 8]).astype(np.uint32)
+    hm = HipMap(V2, F2, [0, 12, 24])
+    Q = np.array([[0, 0, 0], [50, 50, 0], [50, -50, 20], [0.1, 0.1, 20], [-5, 0, 5]], np.float32)
+    r = _check(hm, V2, F2, Q, label="cubes", offsets=[0, 12, 24])
+    np.testing.assert_allclose(r["points"].cpu().numpy(),
+                               [[0, 0, 4.5], [20.5, 0.5, 4.5], [20.5, -0.5, 5.5], [0.1, 0.1, 5.5], [-0.5, 0, 5]], atol=1e-6)
+    np.testing.assert_array_equal(r["geom_ids"].cpu().numpy(), [0, 1, 1, 0, 0])
+
+
+def test_room_random_queries_and_max_distance():
+    V, F, Tbm = scenes.room_workload(n_sensors=1)
+    hm = HipMap(V, F)
+    rng = np.random.default_rng(9)
+    Q = rng.uniform([-12, -12, -1], [12, 12, 5], size=(2000, 3)).astype(np.float32)
+    _check(hm, V, F, Q, label="room")
+    _check(hm, V, F, Q, max_distance=0.3, label="room max 0.3")
+    r = hm.closest_points(torch.as_tensor(Q, device="cuda"), 0.0)
+    assert (r["dist"].cpu().numpy() == C.FLT_MAX).all() and (r["face_ids"].cpu().numpy() == -1).all()
+
+
+def test_imported_sphere_surface_and_empty():
+    s = load_map_file(MAPS / "sphere.ply")
+    hm = import_hip_map(MAPS / "sphere.ply")
+    rng = np.random.default_rng(2)
+    Q = rng.normal(size=(500, 3)).astype(np.float32) * 2
+    _check(hm, s["vertices"], s["faces"], Q, label="sphere")
+    empty = HipMap(np.zeros((0, 3), np.float32), np.zeros((0, 3), np.uint32))
+    r = empty.closest_points(torch.as_tensor(Q[:5], device="cuda"))
+    assert (r["dist"].cpu().numpy() == C.FLT_MAX).all()
