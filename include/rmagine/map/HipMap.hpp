@@ -93,6 +93,12 @@ class HipMap {
     return ClosestPointResult{dh[0], ph[0], nh[0], fh[0], gh[0]};
   }
 
+  // New vertex positions (same count and faces): updates triangles, normals and BVH boxes
+  // in place (no rebuild).  The role of OptiX's GAS update / EmbreeMesh::apply + commit.
+  void refit(const MemoryView<Vector, VRAM_HIP>& vertices) {
+    RM_RMH_CHECK(rmh_map_refit(m_map, reinterpret_cast<const float*>(vertices.raw()), m_stream));
+  }
+
   // Geometry / object id reported for each mesh (empty vector keeps the mesh index).
   void setMeshIds(const std::vector<uint32_t>& geom_ids, const std::vector<uint32_t>& object_ids) {
     const uint32_t n = info().n_meshes;

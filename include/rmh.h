@@ -143,6 +143,13 @@ int rmh_map_get_info(const rmh_map* map, rmh_map_info* info);
  * object id = instID[0] when the hit went through an instance, else geomID
  * (src/rmagine_embree/include/rmagine/simulation/SphereSimulatorEmbree.tcc:175-185). */
 int rmh_map_set_mesh_ids(rmh_map* map, const uint32_t* geom_ids, const uint32_t* object_ids, uint32_t n_meshes);
+/* Refit: new positions for the map's vertices (device, 3 floats each, same count and faces),
+ * triangles, normals and BVH boxes updated in place without rebuilding the tree (the role of
+ * OptiX's ALLOW_UPDATE rebuild in OptixScene::commit, src/rmagine_optix/src/map/optix/OptixScene.cpp:296-315,
+ * and of EmbreeMesh::apply + commit, src/rmagine_embree/src/map/embree/EmbreeMesh.cpp:297-330).
+ * Results equal those of a fresh build; traversal cost grows as vertices move far from the
+ * layout the tree was built for.  Synchronous. */
+int rmh_map_refit(rmh_map* map, const float* vertices_dev, void* stream);
 
 /* Closest points: for n query points (device, 12 B each) the nearest point of the map
  * strictly within max_distance.  Replaces EmbreeMap::closestPoint

@@ -252,7 +252,8 @@ __global__ __launch_bounds__(kBlock) void k_layout(int n, const uint32_t* __rest
                                                    const int* __restrict__ parent_int,
                                                    const float4* __restrict__ node_lo, const float4* __restrict__ node_hi,
                                                    const uint8_t* __restrict__ collapse, Node2* __restrict__ nodes,
-                                                   uint32_t* stats /* [0]=reachable interior, [1]=max depth */) {
+                                                   uint32_t* stats /* [0]=reachable interior, [1]=max depth */,
+                                                   uint8_t* __restrict__ reach) {
   const int i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= n - 1) return;
   float4 lo[2], hi[2];
@@ -291,6 +292,7 @@ __global__ __launch_bounds__(kBlock) void k_layout(int n, const uint32_t* __rest
     depth++;
     if (collapse[p]) reachable = false;
   }
+  reach[i] = reachable ? 1 : 0;
   if (reachable) {
     atomicAdd(&stats[0], 1u);
     atomicMax(&stats[1], depth + 1u);
@@ -361,6 +363,8 @@ void map_free(rmh_map* m) {
   if (m->tris) (void)hipFree(m->tris);
   if (m->normals) (void)hipFree(m->normals);
   if (m->nodes) (void)hipFree(m->nodes);
+  if (m->parent) (void)hipFree(m->parent);
+  if (m->reach) (void)hipFree(m->reach);
   delete m;
 }
 
@@ -447,6 +451,8 @@ extern "C" int rmh_map_build(const float* vertices, uint32_t nv, const uint32_t*
   BUILD_TRY(hipMalloc(&m->tris, sizeof(float4) * 3 * static_cast<size_t>(nf)));
   BUILD_TRY(hipMalloc(&m->normals, sizeof(float4) * nf));
   BUILD_TRY(hipMalloc(&m->nodes, sizeof(Node2) * (ni + 1)));
+  BUILD_TRY(hipMalloc(&m->parent, sizeof(int) * (ni + 1)));
+  BUILD_TRY(hipMalloc(&m->reach, sizeof(uint8_t) * (ni + 1)));
   BUILD_TRY(hipMemcpy(moff.p, off.data(), sizeof(uint32_t) * (n_meshes + 1), hipMemcpyHostToDevice));
 
   BUILD_TRY(hipEventRecord(ev0, stream));
@@ -485,8 +491,10 @@ extern "C" int rmh_map_build(const float* vertices, uint32_t nv, const uint32_t*
     BUILD_TRY(hipGetLastError());
     k_layout<<<blocks_for(ni), kBlock, 0, stream>>>(n, svals, flo.as<float4>(), fhi.as<float4>(), child.as<uint32_t>(),
                                                     range.as<uint32_t>(), pint.as<int>(), nlo.as<float4>(),
-                                                    nhi.as<float4>(), coll.as<uint8_t>(), m->nodes, stats.as<uint32_t>());
+                                                    nhi.as<float4>(), coll.as<uint8_t>(), m->nodes, stats.as<uint32_t>(),
+                                                    m->reach);
     BUILD_TRY(hipGetLastError());
+    BUILD_TRY(hipMemcpyAsync(m->parent, pint.p, sizeof(int) * ni, hipMemcpyDeviceToDevice, stream));
   }
   k_gather<<<blocks_for(nf), kBlock, 0, stream>>>(nf, svals, m->vertices, m->faces, moff.as<uint32_t>(), n_meshes,
                                                   m->tris, m->normals);
@@ -557,6 +565,124 @@ extern "C" int rmh_map_set_mesh_ids(rmh_map* map, const uint32_t* geom_ids, cons
   if (e == hipSuccess) e = hipDeviceSynchronize();
   (void)hipFree(dev);
   if (e != hipSuccess) return hip_fail(e, "rmh_map_set_mesh_ids");
+  return RMH_OK;
+}
+
+// ---- refit: new vertex positions, same topology -------------------------------------------------
+
+__global__ __launch_bounds__(kBlock) void k_refit_tris(uint32_t nf, const float* __restrict__ V, const uint32_t* __restrict__ F,
+                                                       float4* __restrict__ tris, float4* __restrict__ normals) {
+  const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
+  if (k >= nf) return;
+  float4 t0 = tris[3 * static_cast<size_t>(k)], t1 = tris[3 * static_cast<size_t>(k) + 1], t2 = tris[3 * static_cast<size_t>(k) + 2];
+  const uint32_t f = __float_as_uint(t2.w);
+  const uint32_t i0 = F[3 * f], i1 = F[3 * f + 1], i2 = F[3 * f + 2];
+  t0.x = V[3 * i0]; t0.y = V[3 * i0 + 1]; t0.z = V[3 * i0 + 2];
+  t1.x = V[3 * i1]; t1.y = V[3 * i1 + 1]; t1.z = V[3 * i1 + 2];
+  t2.x = V[3 * i2]; t2.y = V[3 * i2 + 1]; t2.z = V[3 * i2 + 2];
+  tris[3 * static_cast<size_t>(k)] = t0;
+  tris[3 * static_cast<size_t>(k) + 1] = t1;
+  tris[3 * static_cast<size_t>(k) + 2] = t2;
+  const double ax = (double)t1.x - (double)t0.x, ay = (double)t1.y - (double)t0.y, az = (double)t1.z - (double)t0.z;
+  const double bx = (double)t2.x - (double)t0.x, by = (double)t2.y - (double)t0.y, bz = (double)t2.z - (double)t0.z;
+  const double nx = ay * bz - az * by, ny = az * bx - ax * bz, nz = ax * by - ay * bx;
+  const double len = sqrt(nx * nx + ny * ny + nz * nz);
+  float4 nrm = normals[k];
+  nrm.x = len > 0.0 ? (float)(nx / len) : 0.0f;
+  nrm.y = len > 0.0 ? (float)(ny / len) : 0.0f;
+  nrm.z = len > 0.0 ? (float)(nz / len) : 0.0f;
+  normals[k] = nrm;
+}
+
+__global__ __launch_bounds__(kBlock) void k_fill_u32(uint32_t* p, uint32_t n, uint32_t v) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+__device__ inline void set_child_box(Node2& nd, int c, float4 lo, float4 hi) {
+  if (c == 0) {
+    nd.a.x = pad_lo(lo.x); nd.a.y = pad_hi(hi.x); nd.a.z = pad_lo(lo.y); nd.a.w = pad_hi(hi.y);
+    nd.c.x = pad_lo(lo.z); nd.c.y = pad_hi(hi.z);
+  } else {
+    nd.b.x = pad_lo(lo.x); nd.b.y = pad_hi(hi.x); nd.b.z = pad_lo(lo.y); nd.b.w = pad_hi(hi.y);
+    nd.c.z = pad_lo(lo.z); nd.c.w = pad_hi(hi.z);
+  }
+}
+
+// Bottom-up box update of the final tree: leaf slots are boxed from their triangles; the
+// second arrival at a node unions its two (already padded) child boxes, pads once more
+// and moves up.  Boxes stay conservative at every level.
+__global__ __launch_bounds__(kBlock) void k_refit_nodes(uint32_t ni, const uint8_t* __restrict__ reach,
+                                                        const int* __restrict__ parent, const float4* __restrict__ tris,
+                                                        Node2* nodes, uint32_t* arrive) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= ni || !reach[i]) return;
+  Node2 nd = nodes[i];
+  const int refs[2] = {nd.d.x, nd.d.y};
+  uint32_t leaves = 0;
+  // only the words of leaf slots are written here: an interior slot belongs to the thread
+  // that finishes that child, which may already have stored its box
+  for (int c = 0; c < 2; c++) {
+    if (refs[c] >= 0) continue;
+    const uint32_t e = ~static_cast<uint32_t>(refs[c]);
+    const uint32_t first = e >> 3, cnt = (e & 7u) + 1u;
+    float4 lo = make_float4(INFINITY, INFINITY, INFINITY, 0.0f), hi = make_float4(-INFINITY, -INFINITY, -INFINITY, 0.0f);
+    for (uint32_t k = first; k < first + cnt; k++)
+      for (int v = 0; v < 3; v++) {
+        const float4 p = tris[3 * static_cast<size_t>(k) + v];
+        lo.x = fminf(lo.x, p.x); lo.y = fminf(lo.y, p.y); lo.z = fminf(lo.z, p.z);
+        hi.x = fmaxf(hi.x, p.x); hi.y = fmaxf(hi.y, p.y); hi.z = fmaxf(hi.z, p.z);
+      }
+    set_child_box(nd, c, lo, hi);
+    if (c == 0) { nodes[i].a = nd.a; nodes[i].c.x = nd.c.x; nodes[i].c.y = nd.c.y; }
+    else { nodes[i].b = nd.b; nodes[i].c.z = nd.c.z; nodes[i].c.w = nd.c.w; }
+    leaves++;
+  }
+  if (leaves == 0) return;  // both children interior: their threads finish this node
+  __threadfence();
+  uint32_t old = __hip_atomic_fetch_add(&arrive[i], leaves, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+  if (old + leaves != 2u) return;
+  int j = static_cast<int>(i);
+  for (;;) {
+    __threadfence();
+    const Node2 cur = nodes[j];
+    const float4 lo = make_float4(fminf(cur.a.x, cur.b.x), fminf(cur.a.z, cur.b.z), fminf(cur.c.x, cur.c.z), 0.0f);
+    const float4 hi = make_float4(fmaxf(cur.a.y, cur.b.y), fmaxf(cur.a.w, cur.b.w), fmaxf(cur.c.y, cur.c.w), 0.0f);
+    if (j == 0) return;  // the root
+    const int p = parent[j];
+    Node2 pn = nodes[p];
+    const int c = pn.d.x == j ? 0 : 1;
+    set_child_box(pn, c, lo, hi);
+    if (c == 0) { nodes[p].a = pn.a; nodes[p].c.x = pn.c.x; nodes[p].c.y = pn.c.y; }
+    else { nodes[p].b = pn.b; nodes[p].c.z = pn.c.z; nodes[p].c.w = pn.c.w; }
+    __threadfence();
+    old = __hip_atomic_fetch_add(&arrive[p], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    if (old != 1u) return;  // first arrival: the sibling's thread continues
+    j = p;
+  }
+}
+
+extern "C" int rmh_map_refit(rmh_map* map, const float* vertices, void* stream_) {
+  if (!map) return fail(RMH_ERR_INVALID, "rmh_map_refit: NULL map");
+  if (map->n_faces == 0) return RMH_OK;
+  if (!vertices) return fail(RMH_ERR_INVALID, "rmh_map_refit: NULL vertices");
+  hipStream_t stream = static_cast<hipStream_t>(stream_);
+  RMH_HIP_TRY(hipMemcpyAsync(map->vertices, vertices, sizeof(float) * 3 * static_cast<size_t>(map->n_vertices),
+                             hipMemcpyDeviceToDevice, stream));
+  k_refit_tris<<<blocks_for(map->n_faces), kBlock, 0, stream>>>(map->n_faces, map->vertices, map->faces, map->tris,
+                                                                 map->normals);
+  RMH_HIP_TRY(hipGetLastError());
+  if (map->root >= 0 && map->n_internal > 0) {
+    uint32_t* arrive = nullptr;
+    RMH_HIP_TRY(hipMalloc(&arrive, sizeof(uint32_t) * map->n_internal));
+    k_fill_u32<<<blocks_for(map->n_internal), kBlock, 0, stream>>>(arrive, map->n_internal, 0u);
+    k_refit_nodes<<<blocks_for(map->n_internal), kBlock, 0, stream>>>(map->n_internal, map->reach, map->parent,
+                                                                      map->tris, map->nodes, arrive);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    (void)hipFree(arrive);
+    if (e != hipSuccess) return hip_fail(e, "rmh_map_refit");
+  }
   return RMH_OK;
 }
 

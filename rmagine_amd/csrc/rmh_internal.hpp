@@ -131,6 +131,8 @@ struct rmh_map {
   rmh_map_info info{};
   int device = 0;
   std::vector<uint32_t> mesh_offsets;  // host copy: faces of mesh g are [off[g], off[g+1])
+  int* parent = nullptr;               // parent of each interior node (refit), -1 at the root
+  uint8_t* reach = nullptr;            // 1 for the interior nodes of the final tree
 };
 
 struct rmh_sensor {

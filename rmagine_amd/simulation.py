@@ -108,6 +108,15 @@ class HipMap:
     def handle(self) -> ctypes.c_void_p:
         return self._handle
 
+    def refit(self, vertices, stream=None) -> None:
+        """New positions for the map's vertices ((V,3), same faces): triangles, normals and
+        BVH boxes are updated in place without rebuilding the tree."""
+        Vt = _device_tensor(np.asarray(vertices, dtype=np.float32).reshape(-1, 3) if not isinstance(vertices, torch.Tensor)
+                            else vertices, torch.float32, self.device)
+        if Vt.shape != (self.n_vertices, 3):
+            raise ValueError(f"refit needs ({self.n_vertices}, 3) vertices")
+        _lib.call("rmh_map_refit", self._handle, _ptr(Vt), stream_handle(stream))
+
     def set_mesh_ids(self, geom_ids=None, object_ids=None) -> None:
         """Geometry / object id reported for each mesh (defaults: the mesh index)."""
         n = self.info()["n_meshes"]
