@@ -99,6 +99,13 @@ class HipMap {
     RM_RMH_CHECK(rmh_map_refit(m_map, reinterpret_cast<const float*>(vertices.raw()), m_stream));
   }
 
+  // Node layout of the ray kernels: the 4-wide quantised tree (true) or the binary tree
+  // (false, the default).  Returns whether the 4-wide tree is in use.
+  bool setLayout(bool wide) {
+    RM_RMH_CHECK(rmh_map_set_layout(m_map, wide ? 1 : 0, m_stream));
+    return info().n_wide_nodes > 0;
+  }
+
   // Geometry / object id reported for each mesh (empty vector keeps the mesh index).
   void setMeshIds(const std::vector<uint32_t>& geom_ids, const std::vector<uint32_t>& object_ids) {
     const uint32_t n = info().n_meshes;

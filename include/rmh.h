@@ -94,8 +94,13 @@ typedef struct {
   uint32_t max_leaf_size;   /* 1..8, default 4                          */
   float    sah_cost_node;   /* SAH traversal cost, default 1.0          */
   float    sah_cost_tri;    /* SAH intersection cost, default 1.5       */
-  uint32_t flags;           /* reserved, 0                              */
+  uint32_t flags;           /* RMH_BVH_* bits, default 0                */
 } rmh_bvh_params;
+/* Also collapse the binary tree into a 4-wide tree with 8-bit quantised child boxes
+ * (64 B nodes) and let the ray kernels (simulate, batch, pose gradient) traverse that one.
+ * Results are identical; on the measured workloads the binary tree is faster, so it is the
+ * default.  Closest-point queries and refit always use the binary tree. */
+#define RMH_BVH_WIDE 1u
 
 typedef struct {
   uint32_t n_vertices, n_faces, n_meshes;
@@ -105,6 +110,8 @@ typedef struct {
   uint64_t device_bytes;    /* nodes + triangles + normals                */
   float    build_ms;        /* device time of the build                   */
   float    sah_cost;        /* SAH cost of the tree (root-area relative)  */
+  uint32_t n_wide_nodes;    /* nodes of the 4-wide tree (0: binary only)   */
+  uint32_t wide_levels;     /* most 4-wide nodes on a root-to-leaf path    */
 } rmh_map_info;
 
 /* ---- runtime ---------------------------------------------------------------------- */
@@ -150,6 +157,14 @@ int rmh_map_set_mesh_ids(rmh_map* map, const uint32_t* geom_ids, const uint32_t*
  * Results equal those of a fresh build; traversal cost grows as vertices move far from the
  * layout the tree was built for.  Synchronous. */
 int rmh_map_refit(rmh_map* map, const float* vertices_dev, void* stream);
+/* Node layout the ray kernels traverse: wide != 0 builds the 4-wide quantised tree from the
+ * binary one (see RMH_BVH_WIDE), wide == 0 frees it.  No counterpart in the reference (its
+ * acceleration structures are opaque); a layout the autotuner can choose per map.  A map
+ * whose tree cannot be expressed (a single leaf, more than 42 wide levels, coordinates no
+ * 8-bit grid holds) stays binary: rmh_map_info.n_wide_nodes then reads 0.  Synchronous; no
+ * launch that uses the map may be in flight, and graphs captured with the other layout
+ * must be captured again. */
+int rmh_map_set_layout(rmh_map* map, int wide, void* stream);
 
 /* Closest points: for n query points (device, 12 B each) the nearest point of the map
  * strictly within max_distance.  Replaces EmbreeMap::closestPoint

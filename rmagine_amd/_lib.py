@@ -51,6 +51,7 @@ _SIGNATURES = {
     "rmh_map_destroy": (c_int, [c_void_p]),
     "rmh_map_get_info": (c_int, [c_void_p, ctypes.POINTER(T.rmh_map_info)]),
     "rmh_map_refit": (c_int, [c_void_p, c_void_p, c_void_p]),
+    "rmh_map_set_layout": (c_int, [c_void_p, c_int, c_void_p]),
     "rmh_map_set_mesh_ids": (c_int, [c_void_p, ctypes.POINTER(c_u32), ctypes.POINTER(c_u32), c_u32]),
     "rmh_scene_load": (c_int, [ctypes.c_char_p, c_u32, ctypes.POINTER(c_void_p)]),
     "rmh_scene_counts": (c_int, [c_void_p, ctypes.POINTER(c_u32), ctypes.POINTER(c_u32), ctypes.POINTER(c_u32)]),

@@ -41,7 +41,7 @@ struct GradArgs {
 
 constexpr int kWave = 64;
 
-template <int KIND, int STACK>
+template <int KIND, int STACK, bool WIDE>
 __global__ __launch_bounds__(kWave, 8) void k_pose_grad(const GradArgs a) {
   __shared__ int stk[kLdsStack][kWave];
   const uint32_t pose = a.pose_offset + blockIdx.y;
@@ -56,7 +56,7 @@ __global__ __launch_bounds__(kWave, 8) void k_pose_grad(const GradArgs a) {
     const V3 dm = qrot(Tsm.R, ds);
     const V3 om = map_origin<KIND>(Tsm, os);
     Hit h;
-    trace<STACK, kWave>(a.map, om, dm, a.g.range_max, stk, h);
+    trace<STACK, kWave, WIDE>(a.map, om, dm, a.g.range_max, stk, h);
     const size_t ti = static_cast<size_t>(pose) * a.target_stride + lid;
     const float tr = a.target[ti];
     const bool tvalid = a.target_mask ? (a.target_mask[ti] != 0) : (tr >= a.g.range_min && tr <= a.g.range_max);
@@ -105,20 +105,31 @@ __global__ void k_pose_grad_combine(const double* partial, uint32_t chunks, uint
   out[i] = s;
 }
 
-template <int KIND, int STACK>
+template <int KIND, int STACK, bool WIDE>
 hipError_t launch_grad(const GradArgs& a, uint32_t np, hipStream_t stream) {
-  k_pose_grad<KIND, STACK><<<dim3(a.tl.tiles, np), kWave, 0, stream>>>(a);
+  k_pose_grad<KIND, STACK, WIDE><<<dim3(a.tl.tiles, np), kWave, 0, stream>>>(a);
   return hipGetLastError();
 }
 
-template <int STACK>
+template <int STACK, bool WIDE>
 hipError_t launch_grad_stack(int kind, const GradArgs& a, uint32_t np, hipStream_t stream) {
   switch (kind) {
-    case RMH_MODEL_SPHERICAL: return launch_grad<RMH_MODEL_SPHERICAL, STACK>(a, np, stream);
-    case RMH_MODEL_PINHOLE: return launch_grad<RMH_MODEL_PINHOLE, STACK>(a, np, stream);
-    case RMH_MODEL_O1DN: return launch_grad<RMH_MODEL_O1DN, STACK>(a, np, stream);
-    default: return launch_grad<RMH_MODEL_ONDN, STACK>(a, np, stream);
+    case RMH_MODEL_SPHERICAL: return launch_grad<RMH_MODEL_SPHERICAL, STACK, WIDE>(a, np, stream);
+    case RMH_MODEL_PINHOLE: return launch_grad<RMH_MODEL_PINHOLE, STACK, WIDE>(a, np, stream);
+    case RMH_MODEL_O1DN: return launch_grad<RMH_MODEL_O1DN, STACK, WIDE>(a, np, stream);
+    default: return launch_grad<RMH_MODEL_ONDN, STACK, WIDE>(a, np, stream);
   }
+}
+
+hipError_t launch_grad_sel(const TravSel& ts, int kind, const GradArgs& a, uint32_t np, hipStream_t stream) {
+  if (ts.wide) {
+    if (ts.stack == 32) return launch_grad_stack<32, true>(kind, a, np, stream);
+    if (ts.stack == 64) return launch_grad_stack<64, true>(kind, a, np, stream);
+    return launch_grad_stack<128, true>(kind, a, np, stream);
+  }
+  if (ts.stack == 32) return launch_grad_stack<32, false>(kind, a, np, stream);
+  if (ts.stack == 64) return launch_grad_stack<64, false>(kind, a, np, stream);
+  return launch_grad_stack<128, false>(kind, a, np, stream);
 }
 
 struct Scratch {
@@ -146,10 +157,10 @@ int rmh_pose_grad(const rmh_map* map, const rmh_sensor* sensor, const rmh_transf
   if (n_poses > (1u << 28)) return fail(RMH_ERR_UNSUPPORTED, "rmh_pose_grad: too many poses");
   if (!Tbm) return fail(RMH_ERR_INVALID, "rmh_pose_grad: Tbm is NULL");
   if (!(huber_delta > 0.0f)) return fail(RMH_ERR_INVALID, "rmh_pose_grad: huber_delta must be > 0");
-  if (map->info.max_depth > 128) return fail(RMH_ERR_UNSUPPORTED, "rmh_pose_grad: BVH deeper than 128 levels");
+  const TravSel ts = trav_select(map);
+  if (ts.stack < 0) return fail(RMH_ERR_UNSUPPORTED, "rmh_pose_grad: BVH deeper than 128 levels");
   const uint64_t rc = static_cast<uint64_t>(sensor->width) * sensor->height;
   if (rc == 0 || rc > (1ull << 30)) return fail(RMH_ERR_UNSUPPORTED, "rmh_pose_grad: bad ray count");
-  const uint32_t depth = map->info.max_depth;
   GradArgs a;
   a.map = make_mapdata(map);
   a.g = make_raygen(sensor);
@@ -176,10 +187,7 @@ int rmh_pose_grad(const rmh_map* map, const rmh_sensor* sensor, const rmh_transf
   for (uint32_t p0 = 0; p0 < n_poses; p0 += 65535u) {
     const uint32_t np = std::min<uint32_t>(65535u, n_poses - p0);
     a.pose_offset = p0;
-    hipError_t e;
-    if (depth <= 32) e = launch_grad_stack<32>(sensor->kind, a, np, stream);
-    else if (depth <= 64) e = launch_grad_stack<64>(sensor->kind, a, np, stream);
-    else e = launch_grad_stack<128>(sensor->kind, a, np, stream);
+    const hipError_t e = launch_grad_sel(ts, sensor->kind, a, np, stream);
     if (e != hipSuccess) return hip_fail(e, "rmh_pose_grad launch");
   }
   k_pose_grad_combine<<<(n_poses * 8u + 255u) / 256u, 256, 0, stream>>>(a.partial, a.tl.tiles, n_poses, out);

@@ -21,11 +21,13 @@
 #include "rmh_internal.hpp"
 
 #include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
 
 #include <algorithm>
 #include <cfloat>
 #include <cmath>
 #include <new>
+#include <utility>
 #include <vector>
 
 namespace {
@@ -348,6 +350,154 @@ __global__ __launch_bounds__(kBlock) void k_set_ids(uint32_t nf, const uint32_t*
   if (obj) normals[k].w = __uint_as_float(obj[lo]);
 }
 
+// ---- 4-wide quantised tree ------------------------------------------------------------------
+//
+// Collapsed top-down from the binary tree, one launch pair per level: a wide node made
+// from binary node b starts with b's two children and repeatedly opens the interior
+// candidate of largest surface area until it has four children or only leaves.  A
+// level's children are numbered by an exclusive scan of the per-node interior-child
+// counts, so the build is deterministic and each level lies in memory in the order of its
+// parents (Morton order of the binary tree).
+
+constexpr int kEmptyRef = 0x7fffffff;  // unused child slot (masked by the child count)
+
+struct Cand {
+  int ref;
+  float lo[3], hi[3];
+};
+
+__device__ inline Cand child_cand(const Node2& nd, int c) {
+  Cand k;
+  if (c == 0) {
+    k.ref = nd.d.x;
+    k.lo[0] = nd.a.x; k.hi[0] = nd.a.y; k.lo[1] = nd.a.z; k.hi[1] = nd.a.w; k.lo[2] = nd.c.x; k.hi[2] = nd.c.y;
+  } else {
+    k.ref = nd.d.y;
+    k.lo[0] = nd.b.x; k.hi[0] = nd.b.y; k.lo[1] = nd.b.z; k.hi[1] = nd.b.w; k.lo[2] = nd.c.z; k.hi[2] = nd.c.w;
+  }
+  return k;
+}
+
+__device__ inline float cand_area(const Cand& c) {
+  const float dx = c.hi[0] - c.lo[0], dy = c.hi[1] - c.lo[1], dz = c.hi[2] - c.lo[2];
+  return dx * dy + dy * dz + dz * dx;
+}
+
+// Children of the wide node made from binary node b, in left-to-right order.
+__device__ inline int select_children(const Node2* __restrict__ nodes, int b, Cand c[4]) {
+  const Node2 nd = nodes[b];
+  c[0] = child_cand(nd, 0);
+  c[1] = child_cand(nd, 1);
+  int n = 2;
+  while (n < 4) {
+    int best = -1;
+    float best_area = -1.0f;
+    for (int k = 0; k < n; k++)
+      if (c[k].ref >= 0 && cand_area(c[k]) > best_area) { best_area = cand_area(c[k]); best = k; }
+    if (best < 0) break;
+    const Node2 cn = nodes[c[best].ref];
+    for (int k = n; k > best + 1; k--) c[k] = c[k - 1];
+    c[best] = child_cand(cn, 0);
+    c[best + 1] = child_cand(cn, 1);
+    n++;
+  }
+  return n;
+}
+
+// Grid of one axis for the planes in [lo, hi]: the smallest exponent e >= -126 with an
+// origin o = floor(lo / 2^e) 2^e such that o + 255 2^e >= hi and every o + q 2^e
+// (q = 0..255) is an FP32 number (|o| and |o + 255 2^e| below 2^(24+e) and FLT_MAX).
+// All of it is exact in FP64.  False when no exponent up to 127 fits.
+__device__ inline bool quant_frame(float lo, float hi, int& e_out, double& origin_out) {
+  const double L = lo, H = hi;
+  int e = 0;
+  (void)frexp((H - L) / 255.0, &e);
+  e = max(e - 1, -126);
+  for (; e <= 127; e++) {
+    const double s = ldexp(1.0, e);
+    const double o = floor(L / s) * s;
+    const double top = o + 255.0 * s;
+    if (top < H) continue;
+    const double lim = fmin(ldexp(1.0, 24 + e), static_cast<double>(FLT_MAX));
+    if (fabs(o) >= lim || fabs(top) >= lim) continue;
+    e_out = e;
+    origin_out = o;
+    return true;
+  }
+  return false;
+}
+
+// Largest grid plane <= x and smallest grid plane >= x (x inside [o, o + 255 s]); the
+// checks are exact FP64 comparisons, so a rounded quotient cannot move a plane inward.
+__device__ inline uint32_t quant_lo(float x, double o, double s) {
+  double q = fmin(fmax(floor((static_cast<double>(x) - o) / s), 0.0), 255.0);
+  while (q > 0.0 && o + q * s > static_cast<double>(x)) q -= 1.0;
+  return static_cast<uint32_t>(q);
+}
+
+__device__ inline uint32_t quant_hi(float x, double o, double s) {
+  double q = fmin(fmax(ceil((static_cast<double>(x) - o) / s), 0.0), 255.0);
+  while (q < 255.0 && o + q * s < static_cast<double>(x)) q += 1.0;
+  return static_cast<uint32_t>(q);
+}
+
+__global__ __launch_bounds__(kBlock) void k_wide_count(const Node2* __restrict__ nodes, const int* __restrict__ front,
+                                                       uint32_t n, uint32_t* __restrict__ counts) {
+  const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
+  if (j >= n) return;
+  Cand c[4];
+  const int k = select_children(nodes, front[j], c);
+  uint32_t cnt = 0;
+  for (int i = 0; i < k; i++) cnt += c[i].ref >= 0 ? 1u : 0u;
+  counts[j] = cnt;
+}
+
+// Writes the wide nodes of one level (slots base .. base + n - 1) and the binary indices
+// of the next level's nodes (slot next_base + offsets[j] + t for the t-th interior child).
+__global__ __launch_bounds__(kBlock) void k_wide_emit(const Node2* __restrict__ nodes, const int* __restrict__ front,
+                                                      uint32_t n, uint32_t base, uint32_t next_base, uint32_t cap,
+                                                      const uint32_t* __restrict__ offsets, Node4* __restrict__ out,
+                                                      int* __restrict__ next_front, uint32_t* err) {
+  const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
+  if (j >= n) return;
+  if (base + j >= cap) { atomicOr(err, 2u); return; }
+  Cand c[4];
+  const int k = select_children(nodes, front[j], c);
+  Node4 nd;
+  for (int i = 0; i < 16; i++) nd.w[i] = 0u;
+  uint32_t meta = static_cast<uint32_t>(k) << 24;
+  for (int a = 0; a < 3; a++) {
+    float lo = c[0].lo[a], hi = c[0].hi[a];
+    for (int i = 1; i < k; i++) { lo = fminf(lo, c[i].lo[a]); hi = fmaxf(hi, c[i].hi[a]); }
+    int e;
+    double o;
+    if (!quant_frame(lo, hi, e, o)) { atomicOr(err, 1u); return; }
+    const double s = ldexp(1.0, e);
+    nd.w[a] = __float_as_uint(static_cast<float>(o));  // exact: o is an FP32 number
+    meta |= static_cast<uint32_t>(e + 127) << (8 * a);
+    for (int i = 0; i < k; i++) {
+      nd.w[4 + 2 * a] |= quant_lo(c[i].lo[a], o, s) << (8 * i);
+      nd.w[5 + 2 * a] |= quant_hi(c[i].hi[a], o, s) << (8 * i);
+    }
+  }
+  nd.w[3] = meta;
+  uint32_t slot = offsets[j];
+  for (int i = 0; i < 4; i++) {
+    int ref = kEmptyRef;
+    if (i < k) {
+      ref = c[i].ref;
+      if (ref >= 0) {
+        if (next_base + slot >= cap) { atomicOr(err, 2u); return; }
+        next_front[slot] = ref;
+        ref = static_cast<int>(next_base + slot);
+        slot++;
+      }
+    }
+    nd.w[10 + i] = static_cast<uint32_t>(ref);
+  }
+  out[base + j] = nd;
+}
+
 struct DevBuf {
   void* p = nullptr;
   ~DevBuf() { if (p) (void)hipFree(p); }
@@ -365,7 +515,84 @@ void map_free(rmh_map* m) {
   if (m->nodes) (void)hipFree(m->nodes);
   if (m->parent) (void)hipFree(m->parent);
   if (m->reach) (void)hipFree(m->reach);
+  if (m->nodes4) (void)hipFree(m->nodes4);
   delete m;
+}
+
+// Most levels of the 4-wide tree the ray kernels accept: a lane's stack holds at most 3
+// entries per level, and the largest stack template has 128.
+constexpr uint32_t kMaxWideLevels = 42;
+
+void free_wide(rmh_map* m) {
+  if (m->nodes4) (void)hipFree(m->nodes4);
+  m->nodes4 = nullptr;
+  m->wide_capacity = 0;
+  m->root4 = m->root;
+  m->info.n_wide_nodes = 0;
+  m->info.wide_levels = 0;
+}
+
+// (Re)builds the 4-wide tree of m from its binary tree, in place when the existing buffer
+// is large enough (a refit keeps the node count, so the buffer address stays the same).
+// Frees it (the ray kernels then use the binary tree) for a single-leaf tree, for a tree
+// of more than kMaxWideLevels wide levels, and for coordinates no 8-bit grid can hold.
+hipError_t build_wide(rmh_map* m, hipStream_t stream) {
+  const uint32_t cap = m->info.n_nodes;  // one wide node per binary node at most
+  if (m->root < 0 || cap == 0 || m->n_internal == 0) { free_wide(m); return hipSuccess; }
+  DevBuf fa, fb, counts, offsets, err, tmp;
+  hipError_t e;
+  if (!m->nodes4 || m->wide_capacity < cap) {
+    free_wide(m);
+    if ((e = hipMalloc(&m->nodes4, sizeof(Node4) * cap)) != hipSuccess) { m->nodes4 = nullptr; return e; }
+    m->wide_capacity = cap;
+  }
+  if ((e = hipMalloc(&fa.p, sizeof(int) * cap)) != hipSuccess) return e;
+  if ((e = hipMalloc(&fb.p, sizeof(int) * cap)) != hipSuccess) return e;
+  if ((e = hipMalloc(&counts.p, sizeof(uint32_t) * cap)) != hipSuccess) return e;
+  if ((e = hipMalloc(&offsets.p, sizeof(uint32_t) * cap)) != hipSuccess) return e;
+  if ((e = hipMalloc(&err.p, sizeof(uint32_t))) != hipSuccess) return e;
+  size_t tmp_bytes = 0;
+  if ((e = rocprim::exclusive_scan(nullptr, tmp_bytes, counts.as<uint32_t>(), offsets.as<uint32_t>(), 0u, cap,
+                                   rocprim::plus<uint32_t>(), stream)) != hipSuccess) return e;
+  if ((e = hipMalloc(&tmp.p, tmp_bytes)) != hipSuccess) return e;
+  if ((e = hipMemsetAsync(err.p, 0, sizeof(uint32_t), stream)) != hipSuccess) return e;
+  const int root = 0;
+  if ((e = hipMemcpyAsync(fa.p, &root, sizeof(int), hipMemcpyHostToDevice, stream)) != hipSuccess) return e;
+  int* front = fa.as<int>();
+  int* next = fb.as<int>();
+  uint32_t n = 1, base = 0, levels = 0;
+  bool ok = true;
+  while (n > 0) {
+    const uint32_t next_base = base + n;
+    k_wide_count<<<blocks_for(n), kBlock, 0, stream>>>(m->nodes, front, n, counts.as<uint32_t>());
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if ((e = rocprim::exclusive_scan(tmp.p, tmp_bytes, counts.as<uint32_t>(), offsets.as<uint32_t>(), 0u, n,
+                                     rocprim::plus<uint32_t>(), stream)) != hipSuccess) return e;
+    k_wide_emit<<<blocks_for(n), kBlock, 0, stream>>>(m->nodes, front, n, base, next_base, cap, offsets.as<uint32_t>(),
+                                                      m->nodes4, next, err.as<uint32_t>());
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    uint32_t h[3] = {0, 0, 0};
+    if ((e = hipMemcpyAsync(&h[0], offsets.as<uint32_t>() + (n - 1), sizeof(uint32_t), hipMemcpyDeviceToHost,
+                            stream)) != hipSuccess) return e;
+    if ((e = hipMemcpyAsync(&h[1], counts.as<uint32_t>() + (n - 1), sizeof(uint32_t), hipMemcpyDeviceToHost,
+                            stream)) != hipSuccess) return e;
+    if ((e = hipMemcpyAsync(&h[2], err.p, sizeof(uint32_t), hipMemcpyDeviceToHost, stream)) != hipSuccess) return e;
+    if ((e = hipStreamSynchronize(stream)) != hipSuccess) return e;
+    levels++;
+    const uint64_t children = static_cast<uint64_t>(h[0]) + h[1];
+    if (h[2] != 0u || next_base + children > cap || levels > kMaxWideLevels) { ok = false; break; }
+    base = next_base;
+    n = static_cast<uint32_t>(children);
+    std::swap(front, next);
+  }
+  if (!ok) {
+    free_wide(m);
+    return hipSuccess;
+  }
+  m->root4 = 0;
+  m->info.n_wide_nodes = base;
+  m->info.wide_levels = levels;
+  return hipSuccess;
 }
 
 #define BUILD_TRY(expr)                                             \
@@ -530,8 +757,19 @@ extern "C" int rmh_map_build(const float* vertices, uint32_t nv, const uint32_t*
     const float area = dx * dy + dy * dz + dz * dx;
     m->info.sah_cost = (n >= 2 && area > 0.0f) ? root_lo.w / area : 0.0f;
   }
+  m->root4 = m->root;
+  if (prm.flags & RMH_BVH_WIDE) {
+    BUILD_TRY(hipEventRecord(ev0, stream));
+    BUILD_TRY(build_wide(m, stream));
+    BUILD_TRY(hipEventRecord(ev1, stream));
+    BUILD_TRY(hipStreamSynchronize(stream));
+    float wide_ms = 0.0f;
+    BUILD_TRY(hipEventElapsedTime(&wide_ms, ev0, ev1));
+    ms += wide_ms;
+  }
   m->info.build_ms = ms;
-  m->info.device_bytes = sizeof(float4) * 4 * static_cast<uint64_t>(nf) + sizeof(Node2) * static_cast<uint64_t>(ni + 1);
+  m->info.device_bytes = sizeof(float4) * 4 * static_cast<uint64_t>(nf) + sizeof(Node2) * static_cast<uint64_t>(ni + 1) +
+                         sizeof(Node4) * static_cast<uint64_t>(m->info.n_wide_nodes);
   *out = m;
   return RMH_OK;
 }
@@ -682,7 +920,36 @@ extern "C" int rmh_map_refit(rmh_map* map, const float* vertices, void* stream_)
     if (e == hipSuccess) e = hipStreamSynchronize(stream);
     (void)hipFree(arrive);
     if (e != hipSuccess) return hip_fail(e, "rmh_map_refit");
+    // the 4-wide tree is collapsed again from the refitted binary boxes
+    if (map->nodes4) {
+      const uint64_t before = sizeof(Node4) * static_cast<uint64_t>(map->info.n_wide_nodes);
+      e = build_wide(map, stream);
+      if (e != hipSuccess) {
+        free_wide(map);  // never leave a half-written tree for the ray kernels
+        return hip_fail(e, "rmh_map_refit: 4-wide tree");
+      }
+      map->info.device_bytes = map->info.device_bytes - before + sizeof(Node4) * static_cast<uint64_t>(map->info.n_wide_nodes);
+    }
   }
+  return RMH_OK;
+}
+
+extern "C" int rmh_map_set_layout(rmh_map* map, int wide, void* stream_) {
+  if (!map) return fail(RMH_ERR_INVALID, "rmh_map_set_layout: NULL map");
+  const uint64_t before = sizeof(Node4) * static_cast<uint64_t>(map->info.n_wide_nodes);
+  if (wide) {
+    if (!map->nodes4) {
+      const hipError_t e = build_wide(map, static_cast<hipStream_t>(stream_));
+      if (e != hipSuccess) {
+        free_wide(map);
+        return hip_fail(e, "rmh_map_set_layout");
+      }
+    }
+  } else if (map->nodes4) {
+    RMH_HIP_TRY(hipDeviceSynchronize());
+    free_wide(map);
+  }
+  map->info.device_bytes = map->info.device_bytes - before + sizeof(Node4) * static_cast<uint64_t>(map->info.n_wide_nodes);
   return RMH_OK;
 }
 

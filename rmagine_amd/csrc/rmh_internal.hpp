@@ -99,6 +99,20 @@ struct alignas(64) Node2 {
   int4 d;
 };
 
+// Node of the 4-wide BVH, 64 B (16 words), collapsed from the binary tree.  Child boxes are
+// quantised to 8 bits per plane on a per-node grid: plane = origin + q * 2^e per axis.
+// The origin is a multiple of 2^e below 2^(24+e) in magnitude, so every grid plane is an
+// FP32 number and fma(q, 2^e, origin) computes it exactly; planes are rounded outward, so
+// each quantised box contains the padded box the binary node holds for that child.
+//   w[0..2]  origin x, y, z (float bits)
+//   w[3]     byte 0..2: e_x + 127, e_y + 127, e_z + 127; byte 3: number of children (2..4)
+//   w[4..9]  lo x, hi x, lo y, hi y, lo z, hi z: byte k = the plane of child k
+//   w[10..13] child references (as in Node2; interior references index Node4 nodes)
+//   w[14..15] unused
+struct alignas(64) Node4 {
+  uint32_t w[16];
+};
+
 constexpr int kMaxLeaf = 8;
 
 RMH_HD int make_leaf_ref(uint32_t first, uint32_t count) {
@@ -115,6 +129,8 @@ struct MapData {
   const Node2* nodes;
   int root;           // root reference (interior index or leaf)
   uint32_t n_faces;
+  const Node4* nodes4;  // 4-wide tree (NULL when the map has none)
+  int root4;
 };
 
 }  // namespace rmh
@@ -133,6 +149,11 @@ struct rmh_map {
   std::vector<uint32_t> mesh_offsets;  // host copy: faces of mesh g are [off[g], off[g+1])
   int* parent = nullptr;               // parent of each interior node (refit), -1 at the root
   uint8_t* reach = nullptr;            // 1 for the interior nodes of the final tree
+  // 4-wide quantised tree collapsed from the binary one; the ray kernels traverse it when
+  // present (nodes4 != NULL), closest-point queries and refit work on the binary tree
+  rmh::Node4* nodes4 = nullptr;
+  int root4 = 0;
+  uint32_t wide_capacity = 0;          // allocated Node4 slots
 };
 
 struct rmh_sensor {

@@ -62,7 +62,7 @@ __device__ __forceinline__ uint32_t xcd_contiguous(uint32_t b, uint32_t n) {
 
 // Launch bounds ask for 8 waves per SIMD (64 VGPRs); the few values that do not fit are
 // set-up constants spilled outside the node loop.
-template <int KIND, int STACK>
+template <int KIND, int STACK, bool WIDE>
 __global__ __launch_bounds__(kWave, 8) void k_simulate(const SimArgs a) {
   __shared__ int stk[kLdsStack][kWave];
   const uint32_t lb = xcd_contiguous(blockIdx.x, gridDim.x);
@@ -79,7 +79,7 @@ __global__ __launch_bounds__(kWave, 8) void k_simulate(const SimArgs a) {
     const Tf Tsm = tmul(load_tf(a.Tbm[pose]), a.Tsb);  // Tsm = Tbm * Tsb (SphereSimulatorEmbree.tcc:69-70)
     const V3 dm = qrot(Tsm.R, ds);
     const V3 om = map_origin<KIND>(Tsm, os);
-    trace<STACK, kWave>(a.map, om, dm, a.g.range_max, stk, h);
+    trace<STACK, kWave, WIDE>(a.map, om, dm, a.g.range_max, stk, h);
   }
   // The ray set-up is recomputed for the attribute writers instead of being held through
   // the traversal: at 64 VGPRs those values would otherwise be spilled to scratch.
@@ -92,20 +92,31 @@ __global__ __launch_bounds__(kWave, 8) void k_simulate(const SimArgs a) {
   write_outputs<KIND>(a.out, glob, a.g, a.map, Tsm, ds, os, h);
 }
 
-template <int KIND, int STACK>
+template <int KIND, int STACK, bool WIDE>
 hipError_t launch_kind(const SimArgs& a, uint32_t n_poses, hipStream_t stream) {
-  k_simulate<KIND, STACK><<<n_poses * a.tl.tiles, kWave, 0, stream>>>(a);
+  k_simulate<KIND, STACK, WIDE><<<n_poses * a.tl.tiles, kWave, 0, stream>>>(a);
   return hipGetLastError();
 }
 
-template <int STACK>
+template <int STACK, bool WIDE>
 hipError_t launch_stack(int kind, const SimArgs& a, uint32_t n_poses, hipStream_t stream) {
   switch (kind) {
-    case RMH_MODEL_SPHERICAL: return launch_kind<RMH_MODEL_SPHERICAL, STACK>(a, n_poses, stream);
-    case RMH_MODEL_PINHOLE: return launch_kind<RMH_MODEL_PINHOLE, STACK>(a, n_poses, stream);
-    case RMH_MODEL_O1DN: return launch_kind<RMH_MODEL_O1DN, STACK>(a, n_poses, stream);
-    default: return launch_kind<RMH_MODEL_ONDN, STACK>(a, n_poses, stream);
+    case RMH_MODEL_SPHERICAL: return launch_kind<RMH_MODEL_SPHERICAL, STACK, WIDE>(a, n_poses, stream);
+    case RMH_MODEL_PINHOLE: return launch_kind<RMH_MODEL_PINHOLE, STACK, WIDE>(a, n_poses, stream);
+    case RMH_MODEL_O1DN: return launch_kind<RMH_MODEL_O1DN, STACK, WIDE>(a, n_poses, stream);
+    default: return launch_kind<RMH_MODEL_ONDN, STACK, WIDE>(a, n_poses, stream);
   }
+}
+
+hipError_t launch_sim(const TravSel& ts, int kind, const SimArgs& a, uint32_t n_poses, hipStream_t stream) {
+  if (ts.wide) {
+    if (ts.stack == 32) return launch_stack<32, true>(kind, a, n_poses, stream);
+    if (ts.stack == 64) return launch_stack<64, true>(kind, a, n_poses, stream);
+    return launch_stack<128, true>(kind, a, n_poses, stream);
+  }
+  if (ts.stack == 32) return launch_stack<32, false>(kind, a, n_poses, stream);
+  if (ts.stack == 64) return launch_stack<64, false>(kind, a, n_poses, stream);
+  return launch_stack<128, false>(kind, a, n_poses, stream);
 }
 
 // ---- batched, persistent ---------------------------------------------------------------
@@ -144,7 +155,7 @@ __device__ __forceinline__ void batch_gen(const BatchEntryDev& e, uint32_t vid, 
 // walks instead of one per kind keeps the kernel inside the instruction cache).  As in
 // k_simulate, the ray set-up is recomputed for the writers rather than held (spilled)
 // through the traversal.
-template <int STACK, int BLOCK>
+template <int STACK, int BLOCK, bool WIDE>
 __device__ __forceinline__ void batch_ray(const BatchArgs& a, const BatchEntryDev& e, int kind, uint32_t pose,
                                           uint32_t vid, uint32_t hid, int (*stk)[BLOCK]) {
   Hit h;
@@ -158,7 +169,7 @@ __device__ __forceinline__ void batch_ray(const BatchArgs& a, const BatchEntryDe
       case RMH_MODEL_O1DN: batch_gen<RMH_MODEL_O1DN>(e, vid, hid, lid, Tsm, ds, os, dm, om); break;
       default: batch_gen<RMH_MODEL_ONDN>(e, vid, hid, lid, Tsm, ds, os, dm, om); break;
     }
-    trace<STACK, BLOCK>(a.map, om, dm, e.g.range_max, stk, h);
+    trace<STACK, BLOCK, WIDE>(a.map, om, dm, e.g.range_max, stk, h);
   }
   asm volatile("" : "+v"(vid), "+v"(hid));
   const Tf Tsm = tmul(load_tf(e.Tbm[pose]), e.Tsb);
@@ -183,7 +194,7 @@ __global__ void k_reset_counter(uint32_t* counter) {
   if (threadIdx.x == 0) *counter = 0u;  // a kernel store (not a memset node) keeps graph replays correct
 }
 
-template <int STACK, int BLOCK>
+template <int STACK, int BLOCK, bool WIDE>
 __global__ __launch_bounds__(BLOCK, 8) void k_batch(const BatchArgs a) {
   __shared__ int stk[kLdsStack][BLOCK];
   const int lane = threadIdx.x & 63;
@@ -209,15 +220,15 @@ __global__ __launch_bounds__(BLOCK, 8) void k_batch(const BatchArgs a) {
       const uint32_t tile = static_cast<uint32_t>(tg - static_cast<uint64_t>(pose) * e.tl.tiles);
       uint32_t vid, hid;
       if (!tile_ray(e.tl, e.g.grid_w, e.g.grid_h, tile, lane, vid, hid)) continue;
-      batch_ray<STACK, BLOCK>(a, e, kind, pose, vid, hid, stk);
+      batch_ray<STACK, BLOCK, WIDE>(a, e, kind, pose, vid, hid, stk);
     }
   }
 }
 
-template <int STACK>
+template <int STACK, bool WIDE>
 hipError_t launch_batch(const BatchArgs& a, uint32_t blocks, hipStream_t stream) {
   k_reset_counter<<<1, 64, 0, stream>>>(a.counter);
-  k_batch<STACK, kWave><<<blocks, kWave, 0, stream>>>(a);
+  k_batch<STACK, kWave, WIDE><<<blocks, kWave, 0, stream>>>(a);
   return hipGetLastError();
 }
 
@@ -279,7 +290,7 @@ int batch_upload(rmh_batch* b, const std::vector<BatchEntryDev>& host) {
 }
 
 int check_depth(const rmh_map* map) {
-  if (map->info.max_depth > 128) return fail(RMH_ERR_UNSUPPORTED, "BVH deeper than 128 levels");
+  if (trav_select(map).stack < 0) return fail(RMH_ERR_UNSUPPORTED, "BVH deeper than 128 levels");
   return RMH_OK;
 }
 
@@ -299,7 +310,7 @@ int rmh_simulate(const rmh_map* map, const rmh_sensor* sensor, const rmh_transfo
   if (ray_count == 0) return RMH_OK;
   if (ray_count > (1ull << 31)) return fail(RMH_ERR_UNSUPPORTED, "rmh_simulate: more than 2^31 rays per pose");
   if (int rc = check_depth(map)) return rc;
-  const uint32_t depth = map->info.max_depth;
+  const TravSel ts = trav_select(map);
 
   SimArgs a;
   a.map = make_mapdata(map);
@@ -317,10 +328,7 @@ int rmh_simulate(const rmh_map* map, const rmh_sensor* sensor, const rmh_transfo
   for (uint32_t p0 = 0; p0 < n_poses; p0 += poses_per_launch) {
     const uint32_t np = std::min<uint32_t>(poses_per_launch, n_poses - p0);
     a.pose_offset = p0;
-    hipError_t e;
-    if (depth <= 32) e = launch_stack<32>(sensor->kind, a, np, stream);
-    else if (depth <= 64) e = launch_stack<64>(sensor->kind, a, np, stream);
-    else e = launch_stack<128>(sensor->kind, a, np, stream);
+    const hipError_t e = launch_sim(ts, sensor->kind, a, np, stream);
     if (e != hipSuccess) return hip_fail(e, "rmh_simulate launch");
   }
   return RMH_OK;
@@ -427,12 +435,19 @@ int rmh_batch_simulate(rmh_batch* b, void* stream_) {
   a.counter = b->counter;
   const uint32_t waves = b->n_cu * b->params.waves_per_cu;
   const uint32_t blocks = std::max<uint32_t>(1, std::min<uint32_t>(waves, b->total_chunks));
-  const uint32_t depth = b->map->info.max_depth;
+  const TravSel ts = trav_select(b->map);
   hipStream_t stream = static_cast<hipStream_t>(stream_);
   hipError_t e;
-  if (depth <= 32) e = launch_batch<32>(a, blocks, stream);
-  else if (depth <= 64) e = launch_batch<64>(a, blocks, stream);
-  else e = launch_batch<128>(a, blocks, stream);
+  if (ts.stack < 0) return fail(RMH_ERR_UNSUPPORTED, "rmh_batch_simulate: BVH deeper than 128 levels");
+  if (ts.wide) {
+    if (ts.stack == 32) e = launch_batch<32, true>(a, blocks, stream);
+    else if (ts.stack == 64) e = launch_batch<64, true>(a, blocks, stream);
+    else e = launch_batch<128, true>(a, blocks, stream);
+  } else {
+    if (ts.stack == 32) e = launch_batch<32, false>(a, blocks, stream);
+    else if (ts.stack == 64) e = launch_batch<64, false>(a, blocks, stream);
+    else e = launch_batch<128, false>(a, blocks, stream);
+  }
   if (e != hipSuccess) return hip_fail(e, "rmh_batch_simulate launch");
   return RMH_OK;
 }

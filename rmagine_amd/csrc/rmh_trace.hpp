@@ -52,7 +52,25 @@ inline MapData make_mapdata(const rmh_map* m) {
   d.nodes = m->nodes;
   d.root = m->root;
   d.n_faces = m->n_faces;
+  d.nodes4 = m->nodes4;
+  d.root4 = m->root4;
   return d;
+}
+
+// Traversal variant of a map, which the launchers turn into template arguments: the
+// 4-wide tree when the map has one (stack of 3 entries per wide level), else the binary
+// tree (one entry per level).  Stack classes 32 / 64 / 128; -1 = deeper than 128.
+struct TravSel {
+  int stack;
+  bool wide;
+};
+
+inline TravSel trav_select(const rmh_map* m) {
+  const uint32_t need = m->nodes4 ? 3u * m->info.wide_levels : m->info.max_depth;
+  TravSel t;
+  t.wide = m->nodes4 != nullptr;
+  t.stack = need <= 32 ? 32 : (need <= 64 ? 64 : (need <= 128 ? 128 : -1));
+  return t;
 }
 
 __device__ __forceinline__ float sel3(const float4& v, int k) { return k == 0 ? v.x : (k == 1 ? v.y : v.z); }
@@ -220,6 +238,130 @@ __device__ __forceinline__ void walk(const MapData& m, const SlabRay& r, float t
   }
 }
 
+// Entry distances and hit flags of the (up to) four children of a 4-wide node, from its
+// words w[0..9] (Node4).  Each grid plane is computed exactly (fma(q, 2^e, origin) is an
+// FP32 number), then goes through the binary slab test's formula, so the test is
+// conservative in the same way for boxes that contain the binary tree's boxes.
+template <int OCT>
+__device__ __forceinline__ void slab4(const SlabRay& r, float tbf, uint32_t w0, uint32_t w1, uint32_t w2, uint32_t w3,
+                                      uint32_t qlx, uint32_t qhx, uint32_t qly, uint32_t qhy, uint32_t qlz,
+                                      uint32_t qhz, float (&tmin)[4], bool (&hit)[4]) {
+  const float ox = __uint_as_float(w0), oy = __uint_as_float(w1), oz = __uint_as_float(w2);
+  const float sx = __uint_as_float((w3 & 0xffu) << 23);
+  const float sy = __uint_as_float(((w3 >> 8) & 0xffu) << 23);
+  const float sz = __uint_as_float(((w3 >> 16) & 0xffu) << 23);
+  const uint32_t nch = w3 >> 24;
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    const float lx = __fmaf_rn(static_cast<float>((qlx >> (8 * k)) & 0xffu), sx, ox);
+    const float hx = __fmaf_rn(static_cast<float>((qhx >> (8 * k)) & 0xffu), sx, ox);
+    const float ly = __fmaf_rn(static_cast<float>((qly >> (8 * k)) & 0xffu), sy, oy);
+    const float hy = __fmaf_rn(static_cast<float>((qhy >> (8 * k)) & 0xffu), sy, oy);
+    const float lz = __fmaf_rn(static_cast<float>((qlz >> (8 * k)) & 0xffu), sz, oz);
+    const float hz = __fmaf_rn(static_cast<float>((qhz >> (8 * k)) & 0xffu), sz, oz);
+    const float x0 = __fmaf_rn(lx, r.idx, r.lx), x1 = __fmaf_rn(hx, r.idx, r.hx);
+    const float y0 = __fmaf_rn(ly, r.idy, r.ly), y1 = __fmaf_rn(hy, r.idy, r.hy);
+    const float z0 = __fmaf_rn(lz, r.idz, r.lz), z1 = __fmaf_rn(hz, r.idz, r.hz);
+    float t0, t1;
+    if constexpr (OCT < 0) {
+      t0 = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), 0.0f));
+      t1 = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), tbf));
+    } else {
+      constexpr bool nx = OCT & 1, ny = OCT & 2, nz = OCT & 4;
+      t0 = fmaxf(fmaxf(nx ? x1 : x0, ny ? y1 : y0), fmaxf(nz ? z1 : z0, 0.0f));
+      t1 = fminf(fminf(nx ? x0 : x1, ny ? y0 : y1), fminf(nz ? z0 : z1, tbf));
+    }
+    tmin[k] = t0;
+    hit[k] = t0 <= t1 * 1.0000005f && (k < 2 || static_cast<uint32_t>(k) < nch);
+  }
+}
+
+__device__ __forceinline__ void cswap(uint32_t& a, uint32_t& b) {
+  const uint32_t lo = min(a, b), hi = max(a, b);
+  a = lo;
+  b = hi;
+}
+
+__device__ __forceinline__ int pick_ref(uint32_t key, int r0, int r1, int r2, int r3) {
+  const int a = (key & 1u) ? r1 : r0, b = (key & 1u) ? r3 : r2;
+  return (key & 2u) ? b : a;
+}
+
+// The 4-wide form of walk(): the hit children of a node are ordered by entry distance
+// (keys = distance bits with the child slot in the low 2 bits, sorted by a 5-exchange
+// network), the nearest is taken next and the others are pushed farthest first.  The host
+// picks STACK >= 3 x the number of wide levels (at most 3 entries are pushed per level).
+template <int OCT, int STACK, int BLOCK>
+__device__ __forceinline__ void walk4(const MapData& m, const SlabRay& r, float tbf, int (*stk)[BLOCK],
+                                      volatile int* spill, int& ref, int& sp) {
+  const int tid = threadIdx.x;
+  while (ref >= 0) {
+    float tmin[4];
+    bool hit[4];
+    int c0, c1, c2, c3;
+    const int r0 = __builtin_amdgcn_readfirstlane(ref);
+    if (__ballot(ref != r0) == 0) {
+      // wave-uniform node: scalar loads, node words in SGPRs (as in walk())
+      typedef __attribute__((address_space(4))) const uint32_t CUint;
+      const CUint* cw = (const CUint*)(m.nodes4) + static_cast<size_t>(r0) * 16;
+      slab4<OCT>(r, tbf, cw[0], cw[1], cw[2], cw[3], cw[4], cw[5], cw[6], cw[7], cw[8], cw[9], tmin, hit);
+      c0 = static_cast<int>(cw[10]); c1 = static_cast<int>(cw[11]);
+      c2 = static_cast<int>(cw[12]); c3 = static_cast<int>(cw[13]);
+    } else {
+      typedef __attribute__((address_space(1))) const uint32_t GUint;
+      const uint32_t off = static_cast<uint32_t>(ref) << 6;
+      const GUint* gw = (const GUint*)((const GChar*)(m.nodes4) + off);
+      const uint32_t w0 = gw[0], w1 = gw[1], w2 = gw[2], w3 = gw[3];
+      const uint32_t w4 = gw[4], w5 = gw[5], w6 = gw[6], w7 = gw[7];
+      const uint32_t w8 = gw[8], w9 = gw[9];
+      c0 = static_cast<int>(gw[10]); c1 = static_cast<int>(gw[11]);
+      // the last two references: an 8-byte read (see walk())
+      uint32_t roff = off + 48u;
+      asm volatile("" : "+v"(roff));
+      __builtin_assume((roff & 7u) == 0u);
+      const GInt* gr = (const GInt*)((const GChar*)(m.nodes4) + roff);
+      c2 = gr[0];
+      c3 = gr[1];
+      slab4<OCT>(r, tbf, w0, w1, w2, w3, w4, w5, w6, w7, w8, w9, tmin, hit);
+    }
+    uint32_t k0 = hit[0] ? ((__float_as_uint(tmin[0]) & ~3u) | 0u) : 0xFFFFFFFFu;
+    uint32_t k1 = hit[1] ? ((__float_as_uint(tmin[1]) & ~3u) | 1u) : 0xFFFFFFFFu;
+    uint32_t k2 = hit[2] ? ((__float_as_uint(tmin[2]) & ~3u) | 2u) : 0xFFFFFFFFu;
+    uint32_t k3 = hit[3] ? ((__float_as_uint(tmin[3]) & ~3u) | 3u) : 0xFFFFFFFFu;
+    cswap(k0, k1);
+    cswap(k2, k3);
+    cswap(k0, k2);
+    cswap(k1, k3);
+    cswap(k1, k2);
+    if (k0 == 0xFFFFFFFFu) {
+      if (sp == 0) { ref = kDone; return; }
+      --sp;
+      if (sp < kLdsStack) ref = stk[sp][tid];
+      else ref = spill[sp - kLdsStack];
+      continue;
+    }
+    if (k3 != 0xFFFFFFFFu) {
+      const int v = pick_ref(k3, c0, c1, c2, c3);
+      if (sp < kLdsStack) stk[sp][tid] = v;
+      else if (sp < STACK) spill[sp - kLdsStack] = v;
+      sp++;
+    }
+    if (k2 != 0xFFFFFFFFu) {
+      const int v = pick_ref(k2, c0, c1, c2, c3);
+      if (sp < kLdsStack) stk[sp][tid] = v;
+      else if (sp < STACK) spill[sp - kLdsStack] = v;
+      sp++;
+    }
+    if (k1 != 0xFFFFFFFFu) {
+      const int v = pick_ref(k1, c0, c1, c2, c3);
+      if (sp < kLdsStack) stk[sp][tid] = v;
+      else if (sp < STACK) spill[sp - kLdsStack] = v;
+      sp++;
+    }
+    ref = pick_ref(k0, c0, c1, c2, c3);
+  }
+}
+
 // Closest hit along o + t d, 0 <= t <= range_max.  While-while traversal: a lane walks
 // interior nodes (near child first, far child onto its stack) until it reaches a leaf or
 // runs out of work, then the lanes holding leaves test their triangles together.  The
@@ -227,7 +369,7 @@ __device__ __forceinline__ void walk(const MapData& m, const SlabRay& r, float t
 //
 // Waves whose 64 rays share one direction octant (almost all tiles) run the walk
 // specialised for it; mixed waves run the generic one.
-template <int STACK, int BLOCK>
+template <int STACK, int BLOCK, bool WIDE>
 __device__ __forceinline__ void trace(const MapData& m, const V3& o, const V3& d, float range_max,
                                       int (*stk)[BLOCK], Hit& h) {
   h.t = (double)range_max;
@@ -254,19 +396,33 @@ __device__ __forceinline__ void trace(const MapData& m, const V3& o, const V3& d
   // the LDS and scratch pops into one flat load
   volatile int spill[STACK > kLdsStack ? STACK - kLdsStack : 1];
   float tbf = range_max;
-  int ref = m.root;
+  int ref = WIDE ? m.root4 : m.root;
   int sp = 0;
   for (;;) {
-    switch (woct) {
-      case 0: walk<0, STACK, BLOCK>(m, r, tbf, stk, spill, ref, sp); break;
-      case 1: walk<1, STACK, BLOCK>(m, r, tbf, stk, spill, ref, sp); break;
-      case 2: walk<2, STACK, BLOCK>(m, r, tbf, stk, spill, ref, sp); break;
-      case 3: walk<3, STACK, BLOCK>(m, r, tbf, stk, spill, ref, sp); break;
-      case 4: walk<4, STACK, BLOCK>(m, r, tbf, stk, spill, ref, sp); break;
-      case 5: walk<5, STACK, BLOCK>(m, r, tbf, stk, spill, ref, sp); break;
-      case 6: walk<6, STACK, BLOCK>(m, r, tbf, stk, spill, ref, sp); break;
-      case 7: walk<7, STACK, BLOCK>(m, r, tbf, stk, spill, ref, sp); break;
-      default: walk<-1, STACK, BLOCK>(m, r, tbf, stk, spill, ref, sp); break;
+    if constexpr (WIDE) {
+      switch (woct) {
+        case 0: walk4<0, STACK, BLOCK>(m, r, tbf, stk, spill, ref, sp); break;
+        case 1: walk4<1, STACK, BLOCK>(m, r, tbf, stk, spill, ref, sp); break;
+        case 2: walk4<2, STACK, BLOCK>(m, r, tbf, stk, spill, ref, sp); break;
+        case 3: walk4<3, STACK, BLOCK>(m, r, tbf, stk, spill, ref, sp); break;
+        case 4: walk4<4, STACK, BLOCK>(m, r, tbf, stk, spill, ref, sp); break;
+        case 5: walk4<5, STACK, BLOCK>(m, r, tbf, stk, spill, ref, sp); break;
+        case 6: walk4<6, STACK, BLOCK>(m, r, tbf, stk, spill, ref, sp); break;
+        case 7: walk4<7, STACK, BLOCK>(m, r, tbf, stk, spill, ref, sp); break;
+        default: walk4<-1, STACK, BLOCK>(m, r, tbf, stk, spill, ref, sp); break;
+      }
+    } else {
+      switch (woct) {
+        case 0: walk<0, STACK, BLOCK>(m, r, tbf, stk, spill, ref, sp); break;
+        case 1: walk<1, STACK, BLOCK>(m, r, tbf, stk, spill, ref, sp); break;
+        case 2: walk<2, STACK, BLOCK>(m, r, tbf, stk, spill, ref, sp); break;
+        case 3: walk<3, STACK, BLOCK>(m, r, tbf, stk, spill, ref, sp); break;
+        case 4: walk<4, STACK, BLOCK>(m, r, tbf, stk, spill, ref, sp); break;
+        case 5: walk<5, STACK, BLOCK>(m, r, tbf, stk, spill, ref, sp); break;
+        case 6: walk<6, STACK, BLOCK>(m, r, tbf, stk, spill, ref, sp); break;
+        case 7: walk<7, STACK, BLOCK>(m, r, tbf, stk, spill, ref, sp); break;
+        default: walk<-1, STACK, BLOCK>(m, r, tbf, stk, spill, ref, sp); break;
+      }
     }
     if (ref == kDone) break;
     const uint32_t e = ~static_cast<uint32_t>(ref);

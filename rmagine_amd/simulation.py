@@ -85,10 +85,14 @@ class HipMap:
 
     vertices: (V, 3) float32, faces: (F, 3) uint32 indices.  mesh_face_offsets (len
     n_meshes+1) splits the faces into separate geometries (geom ids 0..n_meshes-1).
+    wide=True also collapses the binary BVH into the 4-wide tree with 8-bit quantised
+    child boxes and makes the ray kernels traverse it (same results; the binary default is
+    faster on the measured workloads); set_layout() switches later.
     """
 
     def __init__(self, vertices, faces, mesh_face_offsets=None, max_leaf_size: int = 4,
-                 sah_cost_node: float = 1.0, sah_cost_tri: float = 1.5, device="cuda", stream=None):
+                 sah_cost_node: float = 1.0, sah_cost_tri: float = 1.5, device="cuda", stream=None,
+                 wide: bool = False):
         V = _device_tensor(np.asarray(vertices, dtype=np.float32).reshape(-1, 3), torch.float32, device)
         F = _device_tensor(np.asarray(faces, dtype=np.uint32).reshape(-1, 3).view(np.int32), torch.int32, device)
         self.device = torch.device(device)
@@ -98,7 +102,7 @@ class HipMap:
             off = np.ascontiguousarray(np.asarray(mesh_face_offsets, dtype=np.uint32))
             n_meshes = len(off) - 1
             off_arr = (ctypes.c_uint32 * len(off))(*off.tolist())
-        prm = T.rmh_bvh_params(max_leaf_size, sah_cost_node, sah_cost_tri, 0)
+        prm = T.rmh_bvh_params(max_leaf_size, sah_cost_node, sah_cost_tri, T.BVH_WIDE if wide else 0)
         _lib.call("rmh_map_build", _ptr(V), V.shape[0], _ptr(F), F.shape[0], off_arr, n_meshes,
                   ctypes.byref(prm), stream_handle(stream), ctypes.byref(self._handle))
         self.n_vertices = V.shape[0]
@@ -116,6 +120,17 @@ class HipMap:
         if Vt.shape != (self.n_vertices, 3):
             raise ValueError(f"refit needs ({self.n_vertices}, 3) vertices")
         _lib.call("rmh_map_refit", self._handle, _ptr(Vt), stream_handle(stream))
+
+    def set_layout(self, wide: bool, stream=None) -> bool:
+        """Node layout the ray kernels traverse: the 4-wide quantised tree (wide=True) or the
+        binary one.  Returns whether the 4-wide tree is in use (a tree it cannot express stays
+        binary).  No launch on this map may be in flight; re-capture graphs afterwards."""
+        _lib.call("rmh_map_set_layout", self._handle, 1 if wide else 0, stream_handle(stream))
+        return self.info()["n_wide_nodes"] > 0
+
+    @property
+    def wide(self) -> bool:
+        return self.info()["n_wide_nodes"] > 0
 
     def set_mesh_ids(self, geom_ids=None, object_ids=None) -> None:
         """Geometry / object id reported for each mesh (defaults: the mesh index)."""

@@ -192,6 +192,9 @@ class rmh_outputs(ctypes.Structure):
     _fields_ = [(name, ctypes.c_void_p) for name in ATTRIBUTES]
 
 
+BVH_WIDE = 1  # rmh_bvh_params.flags: also build the 4-wide quantised tree and traverse it
+
+
 class rmh_bvh_params(ctypes.Structure):
     _fields_ = [("max_leaf_size", ctypes.c_uint32), ("sah_cost_node", ctypes.c_float),
                 ("sah_cost_tri", ctypes.c_float), ("flags", ctypes.c_uint32)]
@@ -200,7 +203,8 @@ class rmh_bvh_params(ctypes.Structure):
 class rmh_map_info(ctypes.Structure):
     _fields_ = [("n_vertices", ctypes.c_uint32), ("n_faces", ctypes.c_uint32), ("n_meshes", ctypes.c_uint32),
                 ("n_nodes", ctypes.c_uint32), ("n_leaves", ctypes.c_uint32), ("max_depth", ctypes.c_uint32),
-                ("device_bytes", ctypes.c_uint64), ("build_ms", ctypes.c_float), ("sah_cost", ctypes.c_float)]
+                ("device_bytes", ctypes.c_uint64), ("build_ms", ctypes.c_float), ("sah_cost", ctypes.c_float),
+                ("n_wide_nodes", ctypes.c_uint32), ("wide_levels", ctypes.c_uint32)]
 
 
 class rmh_cloud(ctypes.Structure):
