@@ -60,6 +60,9 @@ def parse():
     ap.add_argument("--no-extra", action="store_true", help="skip the batched / pose-gradient / ICP workloads")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy 5 times (scale factor for HBM counters)")
+    ap.add_argument("--compare-layouts", nargs="*", metavar="WORKLOAD",
+                    help="only time simulate() on the binary and the 4-wide quantised BVH of the same map "
+                         "(workloads: terrain particles icp room; default all) and check the outputs are identical")
     return ap.parse_args()
 
 
@@ -279,6 +282,70 @@ def extra_icp() -> dict:
             "umeyama_fp64_max_abs_err_vs_cpu_oracle": umerr}
 
 
+def compare_layouts(names) -> None:
+    """Binary against 4-wide quantised node layout on the same map: ms per simulate() (min of
+    5 HIP-event timings after 2 warm-ups) and a bit-for-bit comparison of every output."""
+    import torch
+
+    from rmagine_amd import scenes
+    from rmagine_amd import types as T
+    from rmagine_amd.simulation import HipMap, simulator_for
+
+    def bits(x):
+        return x.view(torch.int32) if x.dtype == torch.float32 else x
+
+    def run(name, V, F, model, P, attrs):
+        P = torch.as_tensor(P, device="cuda")
+        out, res = {}, {}
+        for wide in (False, True):
+            hm = HipMap(V, F, wide=wide)
+            sim = simulator_for(model, hm)
+            r = sim.simulate(P, attrs=attrs)
+            sim.simulate(P, r)
+            torch.cuda.synchronize()
+            ms = []
+            for _ in range(5):
+                a, b = _events()
+                a.record()
+                sim.simulate(P, r)
+                b.record()
+                b.synchronize()
+                ms.append(a.elapsed_time(b))
+            inf = hm.info()
+            out["wide" if wide else "binary"] = {"ms": round(min(ms), 3), "build_ms": round(inf["build_ms"], 2),
+                                                 "wide_nodes": inf["n_wide_nodes"], "wide_levels": inf["wide_levels"],
+                                                 "binary_depth": inf["max_depth"]}
+            res[wide] = r
+            del sim, hm
+        same = all(torch.equal(bits(getattr(res[False], n)), bits(getattr(res[True], n))) for n in attrs)
+        rays = P.shape[0] * model.size()
+        print(json.dumps({"workload": name, "rays": rays, "identical": same,
+                          "mrays_s": {k: round(rays / v["ms"] / 1e3, 1) for k, v in out.items()}, **out}), flush=True)
+        del res
+        torch.cuda.empty_cache()
+
+    names = names or ["terrain", "particles", "icp", "room"]
+    for name in names:
+        if name == "terrain":
+            V, F, P = scenes.terrain_workload()
+            run(name, V, F, scenes.ouster_workload_model(), P, T.ATTRIBUTES)
+        elif name == "particles":
+            V, F, _, P = scenes.fine_room_workload()
+            run(name, V, F, scenes.vlp16_workload_model(), P, ("hits", "ranges"))
+        elif name == "icp":
+            V, F = scenes.objects_mesh()
+            rng = np.random.default_rng(4)
+            model = T.O1DnModel(32768, 1, T.Interval(0.1, 100.0), (0.0, 0.0, 0.0), scenes.random_unit_dirs(32768, rng))
+            hyp = np.repeat(T.make_transforms([[0.0, 0.0, 0.0]], [[0.0, 0.0, 0.0, 1.0]]), 2000, 0).copy()
+            hyp[:, 4:7] += rng.normal(scale=0.2, size=(2000, 3)).astype(np.float32)
+            run(name, V, F, model, hyp, ("hits", "ranges", "points", "normals"))
+        elif name == "room":
+            V, F, P = scenes.room_workload()
+            run(name, V, F, scenes.vlp16_workload_model(), P, ("hits", "ranges"))
+        else:
+            raise SystemExit(f"unknown workload {name!r}")
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -295,6 +362,9 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         print(json.dumps({"copy_calibration_bytes": n, "launches": 5}))
+        return
+    if args.compare_layouts is not None:
+        compare_layouts(args.compare_layouts)
         return
 
     from rmagine_amd import scenes
