@@ -124,9 +124,11 @@ def extra_batched(cpu_seconds: float, do_cpu: bool) -> dict:
         mix[{0: "vlp16", 1: "pinhole", 2: "o1dn"}[m.kind]] += 1
     key = autotune.cache_key(len(F), mix)
     tuned = autotune.tune_batch(batch, key=key, chunks=(64, 256, 1024), waves=(16, 32), tile_rows=(2, 4, 8),
-                                reps=2, use_cache=False)
+                                reps=2, use_cache=False, layouts=(False, True))
     ms = autotune.time_batch(batch, reps=5)
-    log(f"batched: {rays} rays, {ms:.2f} ms -> {rays / ms / 1e3:.1f} Mrays/s (tuned {tuned['chunk_rays']}/{tuned['waves_per_cu']}/{tuned['tile_rows']})")
+    layout = "4-wide" if tuned.get("wide") else "binary"
+    log(f"batched: {rays} rays, {ms:.2f} ms -> {rays / ms / 1e3:.1f} Mrays/s (tuned {tuned['chunk_rays']}/"
+        f"{tuned['waves_per_cu']}/{tuned['tile_rows']}, {layout} nodes; layouts {tuned.get('layout_ms')})")
     # the same sensors with one rmh_simulate launch each
     sims = [simulator_for(m, hm) for m, _ in sensors]
     bundles = [None] * len(sims)
@@ -148,7 +150,8 @@ def extra_batched(cpu_seconds: float, do_cpu: bool) -> dict:
                        f"(800 VLP-16, 600 pinhole 640x480 fov 90, 600 O1Dn 1k-50k rays), hits+ranges",
            "rays": rays, "batched_ms": round(ms, 3), "batched_mrays_s": round(rays / ms / 1e3, 1),
            "per_sensor_launches_ms": round(plain_ms, 3), "per_sensor_mrays_s": round(rays / plain_ms / 1e3, 1),
-           "tuned": {k: tuned[k] for k in ("chunk_rays", "waves_per_cu", "tile_rows")},
+           "tuned": {**{k: tuned[k] for k in ("chunk_rays", "waves_per_cu", "tile_rows")},
+                     "layout": "4-wide" if tuned.get("wide") else "binary", "layout_ms": tuned.get("layout_ms")},
            "batched_vs_per_sensor_mismatches": mism}
     if do_cpu:
         import oracle

@@ -8,7 +8,9 @@ sensors better, large ones cost fewer atomics; more waves hide more latency unti
 registers run out; the tile shape decides how coherent a wave's rays are (on a spinning
 lidar wide tiles win because rays of one row end at similar ranges).  The tuner times
 every candidate on the caller's actual batch with HIP events and keeps the fastest:
-first chunk x waves at the default tile, then the tile rows at the best pair.  Results are cached as JSON keyed by device architecture, mesh size (power of
+first chunk x waves at the default tile, then the tile rows at the best pair, then (when
+asked with layouts=(False, True)) the BVH node layout of the batch's map: binary or 4-wide
+with quantised boxes, which changes speed, never results.  Results are cached as JSON keyed by device architecture, mesh size (power of
 two bucket of the face count) and sensor mix, in $RMH_TUNE_CACHE or
 ~/.cache/rmagine_amd/autotune.json.
 
@@ -75,14 +77,20 @@ def time_batch(batch, reps: int = 3, stream=None) -> float:
 
 
 def tune_batch(batch, key: str | None = None, chunks=DEFAULT_CHUNKS, waves=DEFAULT_WAVES,
-               tile_rows=DEFAULT_TILE_ROWS, reps: int = 3, use_cache: bool = True, stream=None) -> dict:
-    """Picks (chunk_rays, waves_per_cu, tile_rows) for `batch`, applies it and returns the record."""
+               tile_rows=DEFAULT_TILE_ROWS, reps: int = 3, use_cache: bool = True, stream=None,
+               layouts=(False,)) -> dict:
+    """Picks (chunk_rays, waves_per_cu, tile_rows) for `batch` and, if `layouts` holds both
+    False and True, the node layout of its map (True = 4-wide quantised); applies the choice
+    and returns the record.  Re-capture graphs of the batch afterwards."""
     path = cache_path()
+    hm = batch.map
     if use_cache and key:
         rec = _load(path).get(key)
         if rec:
             batch.set_params(chunk_rays=rec["chunk_rays"], waves_per_cu=rec["waves_per_cu"],
                              tile_rows=rec.get("tile_rows", 4))
+            if "wide" in rec and hm.wide != rec["wide"]:
+                hm.set_layout(rec["wide"])
             return dict(rec, cached=True)
     trials = []
     rows0 = batch.params()["tile_rows"]
@@ -101,8 +109,23 @@ def tune_batch(batch, key: str | None = None, chunks=DEFAULT_CHUNKS, waves=DEFAU
     batch.set_params(chunk_rays=c, waves_per_cu=w, tile_rows=r)
     rec = {"chunk_rays": c, "waves_per_cu": w, "tile_rows": r, "ms": ms,
            "trials": [{"chunk_rays": tc, "waves_per_cu": tw, "tile_rows": tr, "ms": tm} for tm, tc, tw, tr in trials]}
+    keys = ["chunk_rays", "waves_per_cu", "tile_rows", "ms"]
+    if len(set(layouts)) > 1:
+        start = hm.wide
+        timed = {start: ms}
+        for wide in layouts:
+            if wide in timed:
+                continue
+            if hm.set_layout(wide) != wide:  # this tree has no 4-wide form
+                continue
+            timed[wide] = time_batch(batch, reps, stream)
+        best = min(timed, key=timed.get)
+        if hm.wide != best:
+            hm.set_layout(best)
+        rec.update(wide=best, ms=timed[best], layout_ms={("wide" if k else "binary"): v for k, v in timed.items()})
+        keys.append("wide")
     if key:
         data = _load(path)
-        data[key] = {k: rec[k] for k in ("chunk_rays", "waves_per_cu", "tile_rows", "ms")}
+        data[key] = {k: rec[k] for k in keys}
         _store(path, data)
     return dict(rec, cached=False)

@@ -383,6 +383,10 @@ class BatchSimulatorHip:
         self._h = ctypes.c_void_p()
         _lib.call("rmh_batch_create", map.handle, table, len(entries), ctypes.byref(prm), ctypes.byref(self._h))
 
+    @property
+    def map(self) -> HipMap:
+        return self._map
+
     def set_params(self, chunk_rays: int = 0, waves_per_cu: int = 0, tile_rows: int = 0) -> None:
         """Launch shape: rays per work grab, persistent waves per CU, image rows per 64-ray
         tile (0 keeps the current value)."""

@@ -205,6 +205,13 @@ int main(int argc, char** argv) {
   test_ondn(map);
   test_no_map();
   test_noise(map);
+  // the same known answers through the 4-wide quantised node layout
+  require(map->setLayout(true), "4-wide layout of the cube map");
+  test_spherical(map);
+  test_o1dn(map);
+  test_pinhole(map);
+  test_ondn(map);
+  require(!map->setLayout(false), "back to the binary layout");
   if (argc > 1) test_import(argv[1]);
   std::cout << "simulation_kat: all C++ drop-in checks passed" << std::endl;
   return 0;

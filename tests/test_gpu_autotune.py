@@ -1,0 +1,58 @@
+"""Batch autotuner (rmagine_amd.autotune): it applies the fastest launch shape and node
+layout it timed, records them, re-applies them from its cache, and never changes results
+(every output byte equal before and after tuning)."""
+import json
+
+import numpy as np
+import pytest
+import torch
+
+from rmagine_amd import autotune, scenes
+from rmagine_amd import types as T
+from rmagine_amd.simulation import BatchSimulatorHip, HipMap
+
+pytestmark = pytest.mark.gpu
+
+
+def _outputs(batch):
+    out = [b.to_numpy() for b in batch.simulate()]
+    torch.cuda.synchronize()
+    return out
+
+
+def test_tune_batch_shape_and_layout(tmp_path, monkeypatch):
+    monkeypatch.setenv("RMH_TUNE_CACHE", str(tmp_path / "tune.json"))
+    V, F, P = scenes.room_workload(n_sensors=32)
+    hm = HipMap(V, F)
+    rng = np.random.default_rng(2)
+    d = rng.normal(size=(2048, 3))
+    o1 = T.O1DnModel(2048, 1, T.Interval(0.1, 60.0), (0.0, 0.0, 0.05),
+                     (d / np.linalg.norm(d, axis=1, keepdims=True)).astype(np.float32))
+    entries = [{"model": m, "Tbm": torch.as_tensor(P[i * 16:(i + 1) * 16], device="cuda"), "attrs": T.ATTRIBUTES}
+               for i, m in enumerate((scenes.vlp16_workload_model(), o1))]
+    batch = BatchSimulatorHip(hm, entries)
+    before = _outputs(batch)
+    rec = autotune.tune_batch(batch, key="room-test", chunks=(64, 256), waves=(8, 16), tile_rows=(2, 4),
+                              reps=1, layouts=(False, True))
+    assert not rec["cached"] and rec["wide"] in (False, True)
+    assert set(rec["layout_ms"]) == {"binary", "wide"}
+    assert rec["ms"] == min(rec["layout_ms"].values())
+    p = batch.params()
+    assert (p["chunk_rays"], p["waves_per_cu"], p["tile_rows"]) == (rec["chunk_rays"], rec["waves_per_cu"], rec["tile_rows"])
+    assert hm.wide == rec["wide"]
+    assert len(rec["trials"]) == 2 * 2 + 1
+    after = _outputs(batch)
+    for a, b in zip(before, after):
+        for k in a:
+            np.testing.assert_array_equal(a[k].view(np.uint8), b[k].view(np.uint8), err_msg=k)
+    stored = json.loads((tmp_path / "tune.json").read_text())["room-test"]
+    assert stored["wide"] == rec["wide"] and stored["chunk_rays"] == rec["chunk_rays"]
+    # a fresh batch on a map in the other layout picks the cached choice up
+    hm.set_layout(not rec["wide"])
+    batch.set_params(chunk_rays=1024, waves_per_cu=32, tile_rows=8)
+    again = autotune.tune_batch(batch, key="room-test", layouts=(False, True))
+    assert again["cached"] and hm.wide == rec["wide"]
+    assert batch.params()["chunk_rays"] == rec["chunk_rays"] and batch.params()["tile_rows"] == rec["tile_rows"]
+    for a, b in zip(before, _outputs(batch)):
+        for k in a:
+            np.testing.assert_array_equal(a[k].view(np.uint8), b[k].view(np.uint8), err_msg=k)
