@@ -101,6 +101,9 @@ typedef struct {
  * Results are identical; on the measured workloads the binary tree is faster, so it is the
  * default.  Closest-point queries and refit always use the binary tree. */
 #define RMH_BVH_WIDE 1u
+/* Skip the SAH refinement of the radix tree (three rounds of treelet restructuring,
+ * Karras and Aila, HPG 2013): faster build, slower traversal. */
+#define RMH_BVH_NO_REFINE 2u
 
 typedef struct {
   uint32_t n_vertices, n_faces, n_meshes;

@@ -8,6 +8,9 @@
 //   2. 63-bit Morton codes of the centroids;
 //   3. radix sort of (code, face) pairs (rocPRIM);
 //   4. Karras 2012 radix tree over the sorted codes (index-augmented for duplicates);
+//   4b. SAH refinement: three bottom-up rounds of treelet restructuring (Karras and Aila,
+//      HPG 2013; rmh_treelet.hpp), then depth-first renumbering of the leaves so that
+//      every subtree is one range of the triangle array again;
 //   5. bottom-up pass with one atomic arrival counter per interior node: node boxes,
 //      primitive counts and the area-weighted SAH cost; a subtree of at most
 //      max_leaf_size triangles becomes a leaf when that is cheaper under SAH;
@@ -19,6 +22,7 @@
 // (src/rmagine_embree/src/map/embree/EmbreeMesh.cpp:275-295) and of
 // computeFaceNormals_kernel (src/rmagine_cuda/src/map/mesh_preprocessing.cu:6-20).
 #include "rmh_internal.hpp"
+#include "rmh_treelet.hpp"
 
 #include <rocprim/device/device_radix_sort.hpp>
 #include <rocprim/device/device_scan.hpp>
@@ -35,7 +39,8 @@ namespace {
 using namespace rmh;
 
 constexpr int kBlock = 256;
-constexpr uint32_t kLeafBit = 0x80000000u;
+constexpr uint32_t kLeafBit = kTreeLeafBit;
+constexpr int kTreeletRounds = 3;
 
 __device__ inline void atomic_min_float(float* addr, float v) {
   if (v >= 0.0f) atomicMin(reinterpret_cast<int*>(addr), __float_as_int(v));
@@ -107,16 +112,6 @@ __global__ __launch_bounds__(kBlock) void k_prep(const float* __restrict__ V, ui
   }
 }
 
-__device__ inline uint64_t split3_21(uint32_t x) {
-  uint64_t v = x & 0x1fffffu;
-  v = (v | (v << 32)) & 0x001f00000000ffffull;
-  v = (v | (v << 16)) & 0x001f0000ff0000ffull;
-  v = (v | (v << 8)) & 0x100f00f00f00f00full;
-  v = (v | (v << 4)) & 0x10c30c30c30c30c3ull;
-  v = (v | (v << 2)) & 0x1249249249249249ull;
-  return v;
-}
-
 // Step 2.
 __global__ __launch_bounds__(kBlock) void k_morton(const float4* __restrict__ lo, const float4* __restrict__ hi,
                                                    uint32_t nf, const float* __restrict__ bounds,
@@ -124,62 +119,20 @@ __global__ __launch_bounds__(kBlock) void k_morton(const float4* __restrict__ lo
   const uint32_t f = blockIdx.x * kBlock + threadIdx.x;
   if (f >= nf) return;
   const float4 a = lo[f], b = hi[f];
-  const float c[3] = {0.5f * a.x + 0.5f * b.x, 0.5f * a.y + 0.5f * b.y, 0.5f * a.z + 0.5f * b.z};
-  // One scale for all three axes (a cube around the scene): with a per-axis scale, a flat
-  // scene such as a 100 m x 100 m x 16 m terrain gets as many z splits as x splits and the
-  // top of the tree cuts thin slabs (SAH cost 46.8 instead of 32.5 on that terrain).
-  uint32_t q[3];
+  const float l[3] = {a.x, a.y, a.z}, h[3] = {b.x, b.y, b.z};
+  const float bmin[3] = {bounds[0], bounds[1], bounds[2]};
   const float ext = fmaxf(fmaxf(bounds[3] - bounds[0], bounds[4] - bounds[1]), bounds[5] - bounds[2]);
-  for (int k = 0; k < 3; k++) {
-    float u = ext > 0.0f ? (c[k] - bounds[k]) / ext : 0.5f;
-    u = fminf(fmaxf(u, 0.0f), 1.0f);
-    q[k] = static_cast<uint32_t>(u * 2097151.0f);
-  }
-  keys[f] = (split3_21(q[0]) << 2) | (split3_21(q[1]) << 1) | split3_21(q[2]);
+  keys[f] = morton_key(l, h, bmin, ext);
   vals[f] = f;
 }
 
-__device__ inline int delta(const uint64_t* __restrict__ keys, int n, int i, int j) {
-  if (j < 0 || j >= n) return -1;
-  const uint64_t a = keys[i], b = keys[j];
-  if (a == b) return 64 + __clz(static_cast<uint32_t>(i ^ j));
-  return __clzll(a ^ b);
-}
-
-// Step 4: Karras, "Maximizing Parallelism in the Construction of BVHs, Octrees, and
-// k-d Trees", HPG 2012, Fig. 4.
+// Step 4 (karras_node: Karras 2012, Fig. 4).
 __global__ __launch_bounds__(kBlock) void k_karras(const uint64_t* __restrict__ keys, int n,
                                                    uint32_t* __restrict__ child, uint32_t* __restrict__ range,
                                                    int* __restrict__ parent_int, int* __restrict__ parent_leaf) {
   const int i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= n - 1) return;
-  const int d = (delta(keys, n, i, i + 1) - delta(keys, n, i, i - 1)) >= 0 ? 1 : -1;
-  const int dmin = delta(keys, n, i, i - d);
-  int lmax = 2;
-  while (delta(keys, n, i, i + lmax * d) > dmin) lmax <<= 1;
-  int l = 0;
-  for (int t = lmax >> 1; t >= 1; t >>= 1)
-    if (delta(keys, n, i, i + (l + t) * d) > dmin) l += t;
-  const int j = i + l * d;
-  const int dnode = delta(keys, n, i, j);
-  int s = 0;
-  int t = l;
-  do {
-    t = (t + 1) >> 1;
-    if (delta(keys, n, i, i + (s + t) * d) > dnode) s += t;
-  } while (t > 1);
-  const int gamma = i + s * d + (d < 0 ? -1 : 0);
-  const int first = min(i, j), last = max(i, j);
-  uint32_t left, right;
-  if (first == gamma) { left = static_cast<uint32_t>(gamma) | kLeafBit; parent_leaf[gamma] = i; }
-  else { left = static_cast<uint32_t>(gamma); parent_int[gamma] = i; }
-  if (last == gamma + 1) { right = static_cast<uint32_t>(gamma + 1) | kLeafBit; parent_leaf[gamma + 1] = i; }
-  else { right = static_cast<uint32_t>(gamma + 1); parent_int[gamma + 1] = i; }
-  child[2 * i + 0] = left;
-  child[2 * i + 1] = right;
-  range[2 * i + 0] = static_cast<uint32_t>(first);
-  range[2 * i + 1] = static_cast<uint32_t>(last);
-  if (i == 0) parent_int[0] = -1;
+  karras_node(keys, n, i, child, range, parent_int, parent_leaf);
 }
 
 __device__ inline float half_area(float4 a, float4 b) {
@@ -348,6 +301,109 @@ __global__ __launch_bounds__(kBlock) void k_set_ids(uint32_t nf, const uint32_t*
   }
   if (geom) tris[3 * static_cast<size_t>(k) + 1].w = __uint_as_float(geom[lo]);
   if (obj) normals[k].w = __uint_as_float(obj[lo]);
+}
+
+// ---- SAH refinement: treelet restructuring (rmh_treelet.hpp) ------------------------------------
+
+__global__ __launch_bounds__(kBlock) void k_leaf_boxes(uint32_t nf, const uint32_t* __restrict__ vals,
+                                                       const float4* __restrict__ flo, const float4* __restrict__ fhi,
+                                                       float* __restrict__ leaf_box) {
+  const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
+  if (k >= nf) return;
+  const uint32_t f = vals[k];
+  const float4 a = flo[f], b = fhi[f];
+  float* o = leaf_box + 6 * static_cast<size_t>(k);
+  o[0] = a.x; o[1] = a.y; o[2] = a.z; o[3] = b.x; o[4] = b.y; o[5] = b.z;
+}
+
+// One bottom-up pass with arrival counters (as k_bottom_up): the second thread to reach an
+// interior node sets its box, count and cost from its children, whose subtrees are final,
+// and restructures the treelet below it when at least `gamma` triangles lie below.  A
+// treelet only involves nodes of that subtree, so no other thread touches them.
+__global__ __launch_bounds__(kBlock) void k_treelet(int n, TreeArrays t, uint32_t* arrive, uint32_t gamma,
+                                                    float c_node, float c_tri, uint32_t* err) {
+  const int k = blockIdx.x * kBlock + threadIdx.x;
+  if (k >= n) return;
+  int p = t.parent_leaf[k];
+  for (int steps = 0; p >= 0; steps++) {
+    if (steps >= kMaxTreeWalk || p >= n - 1) { atomicOr(err, 8u); return; }
+    __threadfence();
+    const uint32_t old = __hip_atomic_fetch_add(&arrive[p], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    if (old == 0u) return;  // first arrival: the sibling's thread finishes this node
+    __threadfence();
+    refresh_node(t, p, c_node, c_tri);
+    if (t.count[p] >= gamma) treelet_optimize(t, p, c_node, c_tri);
+    p = t.parent_int[p];
+  }
+}
+
+// Structure check of the refined tree before anything walks it: every child slot points
+// into range at a node whose parent link names the slot's owner, no node is referenced
+// twice (refs[] counts references), and every node reaches the root by parent links within
+// kMaxTreeWalk steps (no cycle).  Any failure sets *err.
+__global__ __launch_bounds__(kBlock) void k_check_tree(int n, TreeArrays t, uint32_t* refs, uint32_t* err) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < n - 1) {
+    const uint32_t c0 = t.child[2 * i], c1 = t.child[2 * i + 1];
+    if (c0 == c1) atomicOr(err, 4u);
+    for (int c = 0; c < 2; c++) {
+      const uint32_t r = c ? c1 : c0;
+      if (r & kTreeLeafBit) {
+        const uint32_t k = r & ~kTreeLeafBit;
+        if (k >= static_cast<uint32_t>(n) || t.parent_leaf[k] != i) { atomicOr(err, 4u); continue; }
+        atomicAdd(&refs[n - 1 + k], 1u);
+      } else {
+        if (r == 0u || r >= static_cast<uint32_t>(n - 1) || t.parent_int[r] != i) { atomicOr(err, 4u); continue; }
+        atomicAdd(&refs[r], 1u);
+      }
+    }
+  }
+  if (i < n) {
+    int p = t.parent_leaf[i];
+    int steps = 0;
+    while (p >= 0 && steps < kMaxTreeWalk) {
+      if (p >= n - 1) { p = -2; break; }
+      p = t.parent_int[p];
+      steps++;
+    }
+    if (p != -1) atomicOr(err, 4u);
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_check_refs(int n, const uint32_t* refs, uint32_t* err) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= 2 * n - 1) return;
+  const uint32_t want = i == 0 ? 0u : 1u;  // the root has no parent slot
+  if (refs[i] != want) atomicOr(err, 4u);
+}
+
+// Depth-first position of every leaf and of the first leaf below every interior node.
+__global__ __launch_bounds__(kBlock) void k_dfs_positions(int n, TreeArrays t, uint32_t* __restrict__ pos,
+                                                          uint32_t* __restrict__ off) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) pos[i] = dfs_offset(t, static_cast<uint32_t>(i) | kTreeLeafBit);
+  if (i < n - 1) off[i] = dfs_offset(t, static_cast<uint32_t>(i));
+}
+
+// Leaves renumbered depth-first: the sorted face list, the leaf parents and the leaf
+// references follow, and every interior node covers the range [off, off + count - 1].
+__global__ __launch_bounds__(kBlock) void k_apply_order(int n, TreeArrays t, const uint32_t* __restrict__ pos,
+                                                        const uint32_t* __restrict__ off, const uint32_t* __restrict__ vals,
+                                                        uint32_t* __restrict__ vals_out, int* __restrict__ pleaf_out,
+                                                        uint32_t* __restrict__ range) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) {
+    vals_out[pos[i]] = vals[i];
+    pleaf_out[pos[i]] = t.parent_leaf[i];
+  }
+  if (i < n - 1) {
+    for (int c = 0; c < 2; c++) {
+      const uint32_t r = t.child[2 * i + c];
+      if (r & kTreeLeafBit) t.child[2 * i + c] = pos[r & ~kTreeLeafBit] | kTreeLeafBit;
+    }
+    range[2 * i] = off[i];
+    range[2 * i + 1] = off[i] + t.count[i] - 1u;
+  }
 }
 
 // ---- 4-wide quantised tree ------------------------------------------------------------------
@@ -706,11 +762,62 @@ extern "C" int rmh_map_build(const float* vertices, uint32_t nv, const uint32_t*
   uint32_t* svals = vals2.as<uint32_t>();
 
   BUILD_TRY(hipMemsetAsync(stats.p, 0, sizeof(uint32_t) * 4, stream));
+  DevBuf lbox, tbox, tcost, tcount, pos, offs, vals3, pleaf2;
   if (n >= 2) {
-    BUILD_TRY(hipMemsetAsync(arrive.p, 0, sizeof(uint32_t) * ni, stream));
     k_karras<<<blocks_for(ni), kBlock, 0, stream>>>(skeys, n, child.as<uint32_t>(), range.as<uint32_t>(),
                                                     pint.as<int>(), pleaf.as<int>());
     BUILD_TRY(hipGetLastError());
+    if (!(prm.flags & RMH_BVH_NO_REFINE) && n >= kTreeletLeaves) {
+      // SAH refinement of the radix tree, then depth-first leaf order so that every
+      // subtree is again one range of the triangle array
+      BUILD_TRY(hipMalloc(&lbox.p, sizeof(float) * 6 * static_cast<size_t>(nf)));
+      BUILD_TRY(hipMalloc(&tbox.p, sizeof(float) * 6 * static_cast<size_t>(ni)));
+      BUILD_TRY(hipMalloc(&tcost.p, sizeof(float) * ni));
+      BUILD_TRY(hipMalloc(&tcount.p, sizeof(uint32_t) * ni));
+      BUILD_TRY(hipMalloc(&pos.p, sizeof(uint32_t) * nf));
+      BUILD_TRY(hipMalloc(&offs.p, sizeof(uint32_t) * ni));
+      BUILD_TRY(hipMalloc(&vals3.p, sizeof(uint32_t) * nf));
+      BUILD_TRY(hipMalloc(&pleaf2.p, sizeof(int) * nf));
+      TreeArrays ta;
+      ta.child = child.as<uint32_t>();
+      ta.parent_int = pint.as<int>();
+      ta.parent_leaf = pleaf.as<int>();
+      ta.box = tbox.as<float>();
+      ta.cost = tcost.as<float>();
+      ta.count = tcount.as<uint32_t>();
+      ta.leaf_box = lbox.as<float>();
+      k_leaf_boxes<<<blocks_for(nf), kBlock, 0, stream>>>(nf, svals, flo.as<float4>(), fhi.as<float4>(), lbox.as<float>());
+      BUILD_TRY(hipGetLastError());
+      BUILD_TRY(hipMemsetAsync(err.p, 0, sizeof(uint32_t), stream));
+      for (int r = 0; r < kTreeletRounds; r++) {
+        BUILD_TRY(hipMemsetAsync(arrive.p, 0, sizeof(uint32_t) * ni, stream));
+        k_treelet<<<blocks_for(nf), kBlock, 0, stream>>>(n, ta, arrive.as<uint32_t>(), static_cast<uint32_t>(kTreeletLeaves),
+                                                         prm.sah_cost_node, prm.sah_cost_tri, err.as<uint32_t>());
+        BUILD_TRY(hipGetLastError());
+      }
+      {
+        DevBuf refs;
+        BUILD_TRY(hipMalloc(&refs.p, sizeof(uint32_t) * (2 * static_cast<size_t>(nf) - 1)));
+        BUILD_TRY(hipMemsetAsync(refs.p, 0, sizeof(uint32_t) * (2 * static_cast<size_t>(nf) - 1), stream));
+        k_check_tree<<<blocks_for(nf), kBlock, 0, stream>>>(n, ta, refs.as<uint32_t>(), err.as<uint32_t>());
+        BUILD_TRY(hipGetLastError());
+        k_check_refs<<<blocks_for(2 * static_cast<uint64_t>(nf)), kBlock, 0, stream>>>(n, refs.as<uint32_t>(),
+                                                                                       err.as<uint32_t>());
+        BUILD_TRY(hipGetLastError());
+        uint32_t terr = 0;
+        BUILD_TRY(hipMemcpyAsync(&terr, err.p, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+        BUILD_TRY(hipStreamSynchronize(stream));
+        if (terr) { map_free(m); return fail(RMH_ERR_UNSUPPORTED, "rmh_map_build: BVH refinement produced an invalid tree"); }
+      }
+      k_dfs_positions<<<blocks_for(nf), kBlock, 0, stream>>>(n, ta, pos.as<uint32_t>(), offs.as<uint32_t>());
+      BUILD_TRY(hipGetLastError());
+      k_apply_order<<<blocks_for(nf), kBlock, 0, stream>>>(n, ta, pos.as<uint32_t>(), offs.as<uint32_t>(), svals,
+                                                           vals3.as<uint32_t>(), pleaf2.as<int>(), range.as<uint32_t>());
+      BUILD_TRY(hipGetLastError());
+      svals = vals3.as<uint32_t>();
+      std::swap(pleaf.p, pleaf2.p);
+    }
+    BUILD_TRY(hipMemsetAsync(arrive.p, 0, sizeof(uint32_t) * ni, stream));
     k_bottom_up<<<blocks_for(nf), kBlock, 0, stream>>>(n, svals, flo.as<float4>(), fhi.as<float4>(), child.as<uint32_t>(),
                                                        pint.as<int>(), pleaf.as<int>(), nlo.as<float4>(), nhi.as<float4>(),
                                                        arrive.as<uint32_t>(), coll.as<uint8_t>(), prm.max_leaf_size,

@@ -87,12 +87,13 @@ class HipMap:
     n_meshes+1) splits the faces into separate geometries (geom ids 0..n_meshes-1).
     wide=True also collapses the binary BVH into the 4-wide tree with 8-bit quantised
     child boxes and makes the ray kernels traverse it (same results; the binary default is
-    faster on the measured workloads); set_layout() switches later.
+    faster on the measured workloads); set_layout() switches later.  refine=False skips the
+    SAH refinement (treelet restructuring) of the radix tree: faster build, slower rays.
     """
 
     def __init__(self, vertices, faces, mesh_face_offsets=None, max_leaf_size: int = 4,
                  sah_cost_node: float = 1.0, sah_cost_tri: float = 1.5, device="cuda", stream=None,
-                 wide: bool = False):
+                 wide: bool = False, refine: bool = True):
         V = _device_tensor(np.asarray(vertices, dtype=np.float32).reshape(-1, 3), torch.float32, device)
         F = _device_tensor(np.asarray(faces, dtype=np.uint32).reshape(-1, 3).view(np.int32), torch.int32, device)
         self.device = torch.device(device)
@@ -102,7 +103,8 @@ class HipMap:
             off = np.ascontiguousarray(np.asarray(mesh_face_offsets, dtype=np.uint32))
             n_meshes = len(off) - 1
             off_arr = (ctypes.c_uint32 * len(off))(*off.tolist())
-        prm = T.rmh_bvh_params(max_leaf_size, sah_cost_node, sah_cost_tri, T.BVH_WIDE if wide else 0)
+        prm = T.rmh_bvh_params(max_leaf_size, sah_cost_node, sah_cost_tri,
+                               (T.BVH_WIDE if wide else 0) | (0 if refine else T.BVH_NO_REFINE))
         _lib.call("rmh_map_build", _ptr(V), V.shape[0], _ptr(F), F.shape[0], off_arr, n_meshes,
                   ctypes.byref(prm), stream_handle(stream), ctypes.byref(self._handle))
         self.n_vertices = V.shape[0]

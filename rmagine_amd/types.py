@@ -193,6 +193,7 @@ class rmh_outputs(ctypes.Structure):
 
 
 BVH_WIDE = 1  # rmh_bvh_params.flags: also build the 4-wide quantised tree and traverse it
+BVH_NO_REFINE = 2  # rmh_bvh_params.flags: skip the treelet SAH refinement of the radix tree
 
 
 class rmh_bvh_params(ctypes.Structure):
