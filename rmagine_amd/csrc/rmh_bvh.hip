@@ -316,24 +316,86 @@ __global__ __launch_bounds__(kBlock) void k_leaf_boxes(uint32_t nf, const uint32
   o[0] = a.x; o[1] = a.y; o[2] = a.z; o[3] = b.x; o[4] = b.y; o[5] = b.z;
 }
 
-// One bottom-up pass with arrival counters (as k_bottom_up): the second thread to reach an
-// interior node sets its box, count and cost from its children, whose subtrees are final,
-// and restructures the treelet below it when at least `gamma` triangles lie below.  A
-// treelet only involves nodes of that subtree, so no other thread touches them.
-__global__ __launch_bounds__(kBlock) void k_treelet(int n, TreeArrays t, uint32_t* arrive, uint32_t gamma,
-                                                    float c_node, float c_tri, uint32_t* err) {
-  const int k = blockIdx.x * kBlock + threadIdx.x;
-  if (k >= n) return;
-  int p = t.parent_leaf[k];
-  for (int steps = 0; p >= 0; steps++) {
-    if (steps >= kMaxTreeWalk || p >= n - 1) { atomicOr(err, 8u); return; }
-    __threadfence();
-    const uint32_t old = __hip_atomic_fetch_add(&arrive[p], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    if (old == 0u) return;  // first arrival: the sibling's thread finishes this node
-    __threadfence();
-    refresh_node(t, p, c_node, c_tri);
-    if (t.count[p] >= gamma) treelet_optimize(t, p, c_node, c_tri);
-    p = t.parent_int[p];
+// Treelet restructuring of `root` by one whole wave (all 64 lanes call it with the same
+// root): every lane forms the same treelet; the areas of the leaf subsets (two per lane)
+// and then, subset size by subset size, their optimal costs and splits are computed in
+// LDS (a subset's splits only use smaller subsets); lane 0 rebuilds the treelet.  The
+// per-subset choices are those of the serial treelet_optimize().
+__device__ void treelet_wave(const TreeArrays& t, int root, float c_node, float c_tri, float* s_area, float* s_copt,
+                             uint8_t* s_part) {
+  const int lane = threadIdx.x;
+  uint32_t leaf[kTreeletLeaves];
+  int inner[kTreeletLeaves - 1];
+  const int nl = treelet_form(t, root, leaf, inner);
+  if (nl < 3) return;
+  float lb[kTreeletLeaves][6];
+  for (int j = 0; j < nl; j++) ref_box(t, leaf[j], lb[j]);
+  const uint32_t full = (1u << nl) - 1u;
+  for (uint32_t S = lane; S <= full; S += 64)
+    if (S >= 1) s_area[S] = subset_area(lb, nl, S);
+  if (lane < nl) {
+    s_copt[1u << lane] = ref_cost(t, leaf[lane], c_tri);
+    s_part[1u << lane] = 0;
+  }
+  __syncthreads();
+  for (int size = 2; size <= nl; size++) {
+    for (uint32_t S = lane; S <= full; S += 64) {
+      if (S >= 1 && popcount7(S) == size) {
+        uint32_t bp;
+        s_copt[S] = best_split(s_copt, S, s_area[S], c_node, &bp);
+        s_part[S] = static_cast<uint8_t>(bp);
+      }
+    }
+    __syncthreads();
+  }
+  const bool better = s_copt[full] < t.cost[root] * (1.0f - 1e-6f);
+  if (better && lane == 0) treelet_rebuild(t, root, leaf, inner, nl, s_part, c_node, c_tri);
+  __syncthreads();
+}
+
+// One bottom-up pass, one wave per workgroup, arrival counters as in k_bottom_up: the
+// second lane to reach an interior node sets its box, count and cost from its children,
+// whose subtrees are final.  Nodes with at least `gamma` triangles below are then
+// restructured one after the other by the whole wave (treelet_wave); lanes that are done
+// keep taking part until every lane of the wave is done.  A treelet only involves nodes of
+// its subtree, so no other wave touches them.
+__global__ __launch_bounds__(64) void k_treelet(int n, TreeArrays t, uint32_t* arrive, uint32_t gamma, float c_node,
+                                                float c_tri, uint32_t* err) {
+  __shared__ float s_area[1 << kTreeletLeaves];
+  __shared__ float s_copt[1 << kTreeletLeaves];
+  __shared__ uint8_t s_part[1 << kTreeletLeaves];
+  const int k = blockIdx.x * 64 + threadIdx.x;
+  bool active = k < n;
+  int p = active ? t.parent_leaf[k] : -1;
+  int steps = 0;
+  while (__any(active)) {
+    bool need = false;
+    if (active) {
+      if (p < 0) {
+        active = false;
+      } else if (steps++ >= kMaxTreeWalk || p >= n - 1) {
+        atomicOr(err, 8u);
+        active = false;
+      } else {
+        __threadfence();
+        const uint32_t old = __hip_atomic_fetch_add(&arrive[p], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        if (old == 0u) {
+          active = false;  // first arrival: the sibling's lane finishes this node
+        } else {
+          __threadfence();
+          refresh_node(t, p, c_node, c_tri);
+          need = t.count[p] >= gamma;
+        }
+      }
+    }
+    uint64_t todo = __ballot(need);
+    while (todo) {
+      const int l = __ffsll(static_cast<unsigned long long>(todo)) - 1;
+      todo &= todo - 1;
+      treelet_wave(t, __shfl(p, l, 64), c_node, c_tri, s_area, s_copt, s_part);
+    }
+    __threadfence();  // the treelet writes of lane 0 before any lane's next arrival
+    if (active) p = t.parent_int[p];
   }
 }
 
@@ -791,7 +853,7 @@ extern "C" int rmh_map_build(const float* vertices, uint32_t nv, const uint32_t*
       BUILD_TRY(hipMemsetAsync(err.p, 0, sizeof(uint32_t), stream));
       for (int r = 0; r < kTreeletRounds; r++) {
         BUILD_TRY(hipMemsetAsync(arrive.p, 0, sizeof(uint32_t) * ni, stream));
-        k_treelet<<<blocks_for(nf), kBlock, 0, stream>>>(n, ta, arrive.as<uint32_t>(), static_cast<uint32_t>(kTreeletLeaves),
+        k_treelet<<<(nf + 63) / 64, 64, 0, stream>>>(n, ta, arrive.as<uint32_t>(), static_cast<uint32_t>(kTreeletLeaves),
                                                          prm.sah_cost_node, prm.sah_cost_tri, err.as<uint32_t>());
         BUILD_TRY(hipGetLastError());
       }

@@ -172,11 +172,9 @@ RMH_TREE_HD int lowest_bit(uint32_t x) {
   return i;
 }
 
-// Restructures the treelet rooted at interior node `root` (whose children are final).
-// Returns true when the topology changed.
-RMH_TREE_HD bool treelet_optimize(const TreeArrays& t, int root, float c_node, float c_tri) {
-  uint32_t leaf[kTreeletLeaves];
-  int inner[kTreeletLeaves - 1];
+// Treelet below `root`: its leaves (subtree references) and interior nodes (inner[0] =
+// root).  Returns the number of leaves (2..7).
+RMH_TREE_HD int treelet_form(const TreeArrays& t, int root, uint32_t* leaf, int* inner) {
   int nl = 2, ni = 1;
   inner[0] = root;
   leaf[0] = t.child[2 * root];
@@ -195,53 +193,47 @@ RMH_TREE_HD bool treelet_optimize(const TreeArrays& t, int root, float c_node, f
     leaf[best] = t.child[2 * x];
     leaf[nl++] = t.child[2 * x + 1];
   }
-  if (nl < 3) return false;  // two leaves have one topology
-  float lb[kTreeletLeaves][6];
-  float lc[kTreeletLeaves];
+  return nl;
+}
+
+// Surface area (half) of the union of the leaf boxes in subset S.
+RMH_TREE_HD float subset_area(const float (*lb)[6], int nl, uint32_t S) {
+  float b[6] = {3.0e38f, 3.0e38f, 3.0e38f, -3.0e38f, -3.0e38f, -3.0e38f};
   for (int j = 0; j < nl; j++) {
-    ref_box(t, leaf[j], lb[j]);
-    lc[j] = ref_cost(t, leaf[j], c_tri);
-  }
-  const uint32_t full = (1u << nl) - 1u;
-  float area[1 << kTreeletLeaves];
-  float copt[1 << kTreeletLeaves];
-  uint8_t part[1 << kTreeletLeaves];
-  for (uint32_t S = 1; S <= full; S++) {
-    float b[6] = {3.0e38f, 3.0e38f, 3.0e38f, -3.0e38f, -3.0e38f, -3.0e38f};
-    for (int j = 0; j < nl; j++) {
-      if (!(S & (1u << j))) continue;
-      for (int k = 0; k < 3; k++) {
-        b[k] = lb[j][k] < b[k] ? lb[j][k] : b[k];
-        b[3 + k] = lb[j][3 + k] > b[3 + k] ? lb[j][3 + k] : b[3 + k];
-      }
+    if (!(S & (1u << j))) continue;
+    for (int k = 0; k < 3; k++) {
+      b[k] = lb[j][k] < b[k] ? lb[j][k] : b[k];
+      b[3 + k] = lb[j][3 + k] > b[3 + k] ? lb[j][3 + k] : b[3 + k];
     }
-    area[S] = half_area6(b);
   }
-  for (uint32_t S = 1; S <= full; S++) {
-    if (popcount7(S) == 1) {
-      copt[S] = lc[lowest_bit(S)];
-      part[S] = 0;
-      continue;
-    }
-    const uint32_t low = S & (0u - S);
-    float best = 3.0e38f;
-    uint32_t bp = low;
-    // proper subsets P that hold the lowest member: each split of S is seen once
-    for (uint32_t P = (S - 1u) & S; P > 0; P = (P - 1u) & S) {
-      if (!(P & low)) continue;
-      const float c = copt[P] + copt[S ^ P];
-      if (c < best) { best = c; bp = P; }
-    }
-    copt[S] = c_node * area[S] + best;
-    part[S] = static_cast<uint8_t>(bp);
+  return half_area6(b);
+}
+
+// Least cost split of subset S (at least two members) from the optimal costs of its proper
+// subsets: every split is seen once, as the part P that holds S's lowest member; ties keep
+// the first P in this order.  Returns the cost of S; *bp = P.
+RMH_TREE_HD float best_split(const float* copt, uint32_t S, float area_S, float c_node, uint32_t* bp) {
+  const uint32_t low = S & (0u - S);
+  float best = 3.0e38f;
+  uint32_t b = low;
+  for (uint32_t P = (S - 1u) & S; P > 0; P = (P - 1u) & S) {
+    if (!(P & low)) continue;
+    const float c = copt[P] + copt[S ^ P];
+    if (c < best) { best = c; b = P; }
   }
-  if (!(copt[full] < t.cost[root] * (1.0f - 1e-6f))) return false;
-  // rebuild top-down; `made` lists the interior nodes in creation order (parents first)
+  *bp = b;
+  return c_node * area_S + best;
+}
+
+// Rebuilds the treelet with the splits part[] (subset -> the part holding its lowest member),
+// reusing its interior nodes, then refreshes their boxes, counts and costs bottom-up.
+RMH_TREE_HD void treelet_rebuild(const TreeArrays& t, int root, const uint32_t* leaf, const int* inner, int nl,
+                                 const uint8_t* part, float c_node, float c_tri) {
   uint32_t stack_set[kTreeletLeaves];
   int stack_node[kTreeletLeaves];
   int made[kTreeletLeaves - 1];
   int sp = 0, nmade = 0, next_inner = 1;
-  stack_set[sp] = full;
+  stack_set[sp] = (1u << nl) - 1u;
   stack_node[sp] = root;
   sp++;
   while (sp > 0) {
@@ -267,6 +259,32 @@ RMH_TREE_HD bool treelet_optimize(const TreeArrays& t, int root, float c_node, f
     }
   }
   for (int m = nmade - 1; m >= 0; m--) refresh_node(t, made[m], c_node, c_tri);
+}
+
+// Restructures the treelet rooted at interior node `root` (whose children are final), with
+// the dynamic program run serially.  Returns true when the topology changed.
+RMH_TREE_HD bool treelet_optimize(const TreeArrays& t, int root, float c_node, float c_tri) {
+  uint32_t leaf[kTreeletLeaves];
+  int inner[kTreeletLeaves - 1];
+  const int nl = treelet_form(t, root, leaf, inner);
+  if (nl < 3) return false;  // two leaves have one topology
+  float lb[kTreeletLeaves][6];
+  float copt[1 << kTreeletLeaves];
+  uint8_t part[1 << kTreeletLeaves];
+  for (int j = 0; j < nl; j++) ref_box(t, leaf[j], lb[j]);
+  const uint32_t full = (1u << nl) - 1u;
+  for (uint32_t S = 1; S <= full; S++) {
+    if (popcount7(S) == 1) {
+      copt[S] = ref_cost(t, leaf[lowest_bit(S)], c_tri);
+      part[S] = 0;
+      continue;
+    }
+    uint32_t bp;
+    copt[S] = best_split(copt, S, subset_area(lb, nl, S), c_node, &bp);
+    part[S] = static_cast<uint8_t>(bp);
+  }
+  if (!(copt[full] < t.cost[root] * (1.0f - 1e-6f))) return false;
+  treelet_rebuild(t, root, leaf, inner, nl, part, c_node, c_tri);
   return true;
 }
 
