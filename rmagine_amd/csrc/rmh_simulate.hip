@@ -11,7 +11,8 @@
 //   * ray set-up in FP32 in the reference's operation order (contraction off);
 //   * while-while traversal of the binary BVH with an LDS stack (one slot column per
 //     lane, so the 64 lanes of a wave hit 64 different banks), near child first,
-//     conservative FP32 slab tests against outward-padded boxes;
+//     conservative FP32 slab tests against outward-padded boxes; speculative walks
+//     (a lane holds its first leaf and walks on) for O1Dn/OnDn ray lists;
 //   * watertight ray/triangle test (Woop, Benthin, Wald, JCGT 2013) evaluated in FP64:
 //     the decision and the distance are exact to FP64 rounding, which makes hit masks,
 //     face ids and ranges agree with an FP64 evaluation of the same predicate; ties at
@@ -79,7 +80,7 @@ __global__ __launch_bounds__(kWave, 8) void k_simulate(const SimArgs a) {
     const Tf Tsm = tmul(load_tf(a.Tbm[pose]), a.Tsb);  // Tsm = Tbm * Tsb (SphereSimulatorEmbree.tcc:69-70)
     const V3 dm = qrot(Tsm.R, ds);
     const V3 om = map_origin<KIND>(Tsm, os);
-    trace<STACK, kWave, WIDE>(a.map, om, dm, a.g.range_max, stk, h);
+    trace<STACK, kWave, WIDE>(a.map, om, dm, a.g.range_max, stk, h, speculative_walk(KIND));
   }
   // The ray set-up is recomputed for the attribute writers instead of being held through
   // the traversal: at 64 VGPRs those values would otherwise be spilled to scratch.
@@ -169,7 +170,7 @@ __device__ __forceinline__ void batch_ray(const BatchArgs& a, const BatchEntryDe
       case RMH_MODEL_O1DN: batch_gen<RMH_MODEL_O1DN>(e, vid, hid, lid, Tsm, ds, os, dm, om); break;
       default: batch_gen<RMH_MODEL_ONDN>(e, vid, hid, lid, Tsm, ds, os, dm, om); break;
     }
-    trace<STACK, BLOCK, WIDE>(a.map, om, dm, e.g.range_max, stk, h);
+    trace<STACK, BLOCK, WIDE>(a.map, om, dm, e.g.range_max, stk, h, speculative_walk(kind));
   }
   asm volatile("" : "+v"(vid), "+v"(hid));
   const Tf Tsm = tmul(load_tf(e.Tbm[pose]), e.Tsb);
