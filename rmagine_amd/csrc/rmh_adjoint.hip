@@ -56,7 +56,7 @@ __global__ __launch_bounds__(kWave, 8) void k_pose_grad(const GradArgs a) {
     const V3 dm = qrot(Tsm.R, ds);
     const V3 om = map_origin<KIND>(Tsm, os);
     Hit h;
-    trace<STACK, kWave, WIDE>(a.map, om, dm, a.g.range_max, stk, h, speculative_walk(KIND));
+    trace<STACK, kWave, WIDE, spec_mode<KIND>()>(a.map, om, dm, a.g.range_max, stk, h);
     const size_t ti = static_cast<size_t>(pose) * a.target_stride + lid;
     const float tr = a.target[ti];
     const bool tvalid = a.target_mask ? (a.target_mask[ti] != 0) : (tr >= a.g.range_min && tr <= a.g.range_max);

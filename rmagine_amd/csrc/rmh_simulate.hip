@@ -80,7 +80,7 @@ __global__ __launch_bounds__(kWave, 8) void k_simulate(const SimArgs a) {
     const Tf Tsm = tmul(load_tf(a.Tbm[pose]), a.Tsb);  // Tsm = Tbm * Tsb (SphereSimulatorEmbree.tcc:69-70)
     const V3 dm = qrot(Tsm.R, ds);
     const V3 om = map_origin<KIND>(Tsm, os);
-    trace<STACK, kWave, WIDE>(a.map, om, dm, a.g.range_max, stk, h, speculative_walk(KIND));
+    trace<STACK, kWave, WIDE, spec_mode<KIND>()>(a.map, om, dm, a.g.range_max, stk, h);
   }
   // The ray set-up is recomputed for the attribute writers instead of being held through
   // the traversal: at 64 VGPRs those values would otherwise be spilled to scratch.
@@ -170,7 +170,7 @@ __device__ __forceinline__ void batch_ray(const BatchArgs& a, const BatchEntryDe
       case RMH_MODEL_O1DN: batch_gen<RMH_MODEL_O1DN>(e, vid, hid, lid, Tsm, ds, os, dm, om); break;
       default: batch_gen<RMH_MODEL_ONDN>(e, vid, hid, lid, Tsm, ds, os, dm, om); break;
     }
-    trace<STACK, BLOCK, WIDE>(a.map, om, dm, e.g.range_max, stk, h, speculative_walk(kind));
+    trace<STACK, BLOCK, WIDE, kSpecRuntime>(a.map, om, dm, e.g.range_max, stk, h, speculative_walk(kind));
   }
   asm volatile("" : "+v"(vid), "+v"(hid));
   const Tf Tsm = tmul(load_tf(e.Tbm[pose]), e.Tsb);

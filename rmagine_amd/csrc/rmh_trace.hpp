@@ -203,6 +203,12 @@ constexpr int kDone = 0x7fffffff;
 
 // Walks interior nodes from `ref` until a leaf is reached (ref < 0) or the stack runs
 // dry (ref = kDone).
+// Speculation modes of trace(): never (classic while-while), always, or per a wave-uniform
+// run-time flag (one kernel for several sensor kinds; it takes the speculative loop form,
+// with the walk's leaf holding switched by the flag: a third, classic form in the same
+// kernel costs registers and made the batched launch 50 % slower).
+constexpr int kSpecOff = 0, kSpecOn = 1, kSpecRuntime = 2;
+
 // Pops the next subtree of a lane, kDone when its stack is empty.
 template <int STACK, int BLOCK>
 __device__ __forceinline__ int pop_ref(int (*stk)[BLOCK], volatile int* spill, int& sp) {
@@ -214,11 +220,13 @@ __device__ __forceinline__ int pop_ref(int (*stk)[BLOCK], volatile int* spill, i
 
 // spec (wave-uniform): speculative walk (Aila and Laine, "Understanding the Efficiency of
 // Ray Traversal on GPUs", HPG 2009): a lane that reaches its first leaf holds it in `leaf`
-// and keeps walking; the wave leaves the loop when every lane holds a leaf.  This fills
-// the node iterations of incoherent waves (direction-sorted ray lists: 25 % faster on 32k
-// random directions); coherent image tiles reach their leaves together and lose by it
-// (5 %: a held leaf shrinks the ray's interval later), so they walk without it.
-template <int OCT, int STACK, int BLOCK>
+// and keeps walking; the wave leaves the loop once a quarter of its walking lanes hold a
+// leaf (on 2,000 x 32k random directions: 25.9 ms when waiting for all lanes, 24.1 for
+// half, 22.2 for a quarter, 22.5 for an eighth; 34.5 ms without speculation).  This fills
+// the node iterations of incoherent waves; coherent image tiles reach their leaves
+// together and lose by it (5 %: a held leaf shrinks the ray's interval later), so they
+// walk without it.
+template <int OCT, int STACK, int BLOCK, int SPEC>
 __device__ __forceinline__ void walk(const MapData& m, const SlabRay& r, float tbf, int (*stk)[BLOCK],
                                      volatile int* spill, int& ref, int& sp, int& leaf, bool spec) {
   const int tid = threadIdx.x;
@@ -267,12 +275,12 @@ __device__ __forceinline__ void walk(const MapData& m, const SlabRay& r, float t
     } else {
       ref = pop_ref<STACK, BLOCK>(stk, spill, sp);
     }
-    if (spec) {
+    if (SPEC == kSpecOn || (SPEC == kSpecRuntime && spec)) {
       if (ref < 0 && leaf == 0) {
         leaf = ref;
         ref = pop_ref<STACK, BLOCK>(stk, spill, sp);
       }
-      if (__all(leaf != 0)) break;
+      if (__popcll(__ballot(leaf != 0)) * 4 >= __popcll(__ballot(1))) break;
     }
   }
 }
@@ -409,9 +417,9 @@ __device__ __forceinline__ void walk4(const MapData& m, const SlabRay& r, float 
 //
 // Waves whose 64 rays share one direction octant (almost all tiles) run the walk
 // specialised for it; mixed waves run the generic one.
-template <int STACK, int BLOCK, bool WIDE>
+template <int STACK, int BLOCK, bool WIDE, int SPEC>
 __device__ __forceinline__ void trace(const MapData& m, const V3& o, const V3& d, float range_max,
-                                      int (*stk)[BLOCK], Hit& h, bool spec) {
+                                      int (*stk)[BLOCK], Hit& h, bool spec = false) {
   h.t = (double)range_max;
   h.slot = 0xFFFFFFFFu;
   h.face = 0xFFFFFFFFu;
@@ -454,18 +462,18 @@ __device__ __forceinline__ void trace(const MapData& m, const V3& o, const V3& d
       }
     } else {
       switch (woct) {
-        case 0: walk<0, STACK, BLOCK>(m, r, tbf, stk, spill, ref, sp, leaf, spec); break;
-        case 1: walk<1, STACK, BLOCK>(m, r, tbf, stk, spill, ref, sp, leaf, spec); break;
-        case 2: walk<2, STACK, BLOCK>(m, r, tbf, stk, spill, ref, sp, leaf, spec); break;
-        case 3: walk<3, STACK, BLOCK>(m, r, tbf, stk, spill, ref, sp, leaf, spec); break;
-        case 4: walk<4, STACK, BLOCK>(m, r, tbf, stk, spill, ref, sp, leaf, spec); break;
-        case 5: walk<5, STACK, BLOCK>(m, r, tbf, stk, spill, ref, sp, leaf, spec); break;
-        case 6: walk<6, STACK, BLOCK>(m, r, tbf, stk, spill, ref, sp, leaf, spec); break;
-        case 7: walk<7, STACK, BLOCK>(m, r, tbf, stk, spill, ref, sp, leaf, spec); break;
-        default: walk<-1, STACK, BLOCK>(m, r, tbf, stk, spill, ref, sp, leaf, spec); break;
+        case 0: walk<0, STACK, BLOCK, SPEC>(m, r, tbf, stk, spill, ref, sp, leaf, spec); break;
+        case 1: walk<1, STACK, BLOCK, SPEC>(m, r, tbf, stk, spill, ref, sp, leaf, spec); break;
+        case 2: walk<2, STACK, BLOCK, SPEC>(m, r, tbf, stk, spill, ref, sp, leaf, spec); break;
+        case 3: walk<3, STACK, BLOCK, SPEC>(m, r, tbf, stk, spill, ref, sp, leaf, spec); break;
+        case 4: walk<4, STACK, BLOCK, SPEC>(m, r, tbf, stk, spill, ref, sp, leaf, spec); break;
+        case 5: walk<5, STACK, BLOCK, SPEC>(m, r, tbf, stk, spill, ref, sp, leaf, spec); break;
+        case 6: walk<6, STACK, BLOCK, SPEC>(m, r, tbf, stk, spill, ref, sp, leaf, spec); break;
+        case 7: walk<7, STACK, BLOCK, SPEC>(m, r, tbf, stk, spill, ref, sp, leaf, spec); break;
+        default: walk<-1, STACK, BLOCK, SPEC>(m, r, tbf, stk, spill, ref, sp, leaf, spec); break;
       }
     }
-    if (!spec) {
+    if constexpr (SPEC == kSpecOff) {
       // classic while-while order: test the leaf the walk stopped at, then pop (popping
       // first would hold one more register through the FP64 tests: 5 % slower on the
       // terrain)
@@ -498,7 +506,9 @@ __device__ __forceinline__ void trace(const MapData& m, const V3& o, const V3& d
 
 // Speculative walks for ray lists (O1Dn/OnDn: direction-sorted, but any directions),
 // classic walks for image sensors (coherent tiles).
-RMH_HD bool speculative_walk(int kind) { return kind == RMH_MODEL_O1DN || kind == RMH_MODEL_ONDN; }
+RMH_HD constexpr bool speculative_walk(int kind) { return kind == RMH_MODEL_O1DN || kind == RMH_MODEL_ONDN; }
+template <int KIND>
+constexpr int spec_mode() { return speculative_walk(KIND) ? kSpecOn : kSpecOff; }
 
 // Ray tiles: each wave casts a th x tw block of one sensor image (th * tw = 64), which keeps
 // the 64 rays of a wave close in direction.  Tiles are 4 rows x 16 columns: rays of one
