@@ -60,6 +60,9 @@ def parse():
     ap.add_argument("--no-extra", action="store_true", help="skip the batched / pose-gradient / ICP workloads")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy 5 times (scale factor for HBM counters)")
+    ap.add_argument("--visit-stats", type=int, metavar="POSES", default=0,
+                    help="only count node visits, leaves and triangle tests per ray of the default workload's "
+                         "kernel over its first POSES sensor poses (a counting build of the same traversal)")
     ap.add_argument("--compare-layouts", nargs="*", metavar="WORKLOAD",
                     help="only time simulate() on the binary and the 4-wide quantised BVH of the same map "
                          "(workloads: terrain particles icp room; default all) and check the outputs are identical")
@@ -81,11 +84,15 @@ def cpu_baseline(V, F, model, Tbm, seconds, gpu_numpy_for_pose):
     rays = 0
     elapsed = 0.0
     poses = 0
+    per_pose = []
     worst = {"range_max_err": 0.0, "id_mismatch": 0, "hit_mismatch": 0, "ambiguous": 0, "rays": 0}
+    oracle.simulate(om, model, Tbm[:1], threads=threads, flags=False)  # warm-up, not timed
     while elapsed < seconds and poses < len(Tbm):
         t = time.perf_counter()
         o = oracle.simulate(om, model, Tbm[poses:poses + 1], threads=threads, flags=False)
-        elapsed += time.perf_counter() - t
+        dt = time.perf_counter() - t
+        elapsed += dt
+        per_pose.append(dt)
         g = gpu_numpy_for_pose(poses)
         worst["hit_mismatch"] += int((g["hits"] != o["hits"]).sum())
         worst["id_mismatch"] += int((g["face_ids"] != o["face_ids"]).sum())
@@ -95,10 +102,46 @@ def cpu_baseline(V, F, model, Tbm, seconds, gpu_numpy_for_pose):
         rays += len(o["ranges"])
         poses += 1
     log(f"cpu: {poses} poses, {rays} rays in {elapsed:.2f} s on {threads} threads")
+    med = float(np.median(per_pose))
     base = {"value": rays / elapsed / 1e6, "unit": "Mrays/s", "cores": threads, "kind": "port",
+            "median": model.size() / med / 1e6, "cpu": cpu_model(),
             "sample": f"{poses} of the {len(Tbm)} sensor poses ({rays} rays), all attributes, oracle "
-                      f"(exact FP64 predicate, binned-SAH BVH) on {threads} host threads, BVH build excluded"}
+                      f"(exact FP64 predicate, binned-SAH BVH) on {threads} host threads after one untimed "
+                      f"pose, BVH build excluded; value = rays / total time, median = per-pose median"}
     return base, worst
+
+
+def measure_hbm_gbs(nbytes: int = 2 << 30, reps: int = 5) -> float:
+    """HBM bandwidth of this GPU as a plain device copy achieves it: bytes read + written
+    per second of the best of `reps` timed copies of `nbytes`."""
+    import torch
+    src = torch.empty(nbytes, dtype=torch.uint8, device="cuda")
+    src.fill_(1)
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    best = None
+    for _ in range(reps):
+        a, b = _events()
+        a.record()
+        dst.copy_(src)
+        b.record()
+        b.synchronize()
+        ms = a.elapsed_time(b)
+        best = ms if best is None else min(best, ms)
+    del src, dst
+    torch.cuda.empty_cache()
+    return 2 * nbytes / (best * 1e-3) / 1e9
+
+
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def _events():
@@ -369,6 +412,21 @@ def main():
     if args.compare_layouts is not None:
         compare_layouts(args.compare_layouts)
         return
+    if args.visit_stats:
+        from rmagine_amd import scenes
+        from rmagine_amd.simulation import HipMap, SphereSimulatorHip
+        V, F, Tbm = scenes.terrain_workload(n_grid=args.grid, n_sensors=args.sensors)
+        sim = SphereSimulatorHip(HipMap(V, F))
+        sim.setModel(scenes.ouster_workload_model())
+        st = sim.visit_stats(torch.as_tensor(Tbm[:args.visit_stats], device="cuda"))
+        st["active_lanes_per_node_iteration"] = ((st["vector_node_visits"] + st["scalar_node_visits"]) /
+                                                 max(1, st["wave_iterations"]))
+        st["uniform_share_of_wave_iterations"] = st["uniform_wave_iterations"] / max(1, st["wave_iterations"])
+        st["vector_node_bytes_per_ray"] = 56 * st["vector_node_visits_per_ray"]  # 3 x 16 B boxes + 8 B refs
+        st["triangle_bytes_per_ray"] = 48 * st["triangle_tests_per_ray"]
+        print(json.dumps({"visit_stats": st, "workload": f"terrain {args.grid}^2, first {args.visit_stats} of "
+                                                        f"{len(Tbm)} Ouster-style poses"}))
+        return
 
     from rmagine_amd import scenes
     from rmagine_amd.simulation import HipMap, SphereSimulatorHip
@@ -422,6 +480,8 @@ def main():
         cpu, accuracy = cpu_baseline(V, F, model, Tbm, args.cpu_seconds, gpu_numpy_for_pose)
 
     traffic = profiled_traffic(args.grid == 1024 and args.sensors == 1000)
+    hbm_measured = measure_hbm_gbs()
+    log(f"hbm: plain device copy {hbm_measured:.0f} GB/s (read + write)")
     out = {
         "metric": "Mrays/s at <=1e-5 m range err vs Embree semantics (simulate, all attributes)",
         "value": round(value, 2),
@@ -452,6 +512,8 @@ def main():
             "kernel": "k_simulate<spherical> (traversal + attribute writers)",
             "kernel_avg_ms": round(avg_kernel_s * 1e3, 4),
             "alg_bytes_per_launch": alg_bytes,
+            "copy_gbs_measured": round(hbm_measured, 1),
+            "frac_of_measured_copy": round(achieved / hbm_measured, 5),
         },
         "cpu_baseline": cpu,
         "accuracy_vs_oracle": accuracy,

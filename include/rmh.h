@@ -230,6 +230,14 @@ int rmh_sensor_dims(const rmh_sensor* sensor, uint32_t* width, uint32_t* height)
 int rmh_simulate(const rmh_map* map, const rmh_sensor* sensor,
                  const rmh_transform* Tsb, const rmh_transform* Tbm_dev, uint32_t n_poses,
                  const rmh_outputs* out, void* stream);
+/* Diagnostic, no counterpart in the reference: the traversal of rmh_simulate's kernel over
+ * the same rays (binary node layout, same speculation mode), counting instead of writing.
+ * counts (host, 6 entries, summed over all rays): [0] node visits read with vector loads,
+ * [1] node visits read through the scalar cache (the wave at one node), [2] leaves,
+ * [3] triangle tests, [4] wave iterations of the node loop, [5] of them wave-uniform.
+ * Synchronous. */
+int rmh_simulate_visit_stats(const rmh_map* map, const rmh_sensor* sensor, const rmh_transform* Tsb,
+                             const rmh_transform* Tbm_dev, uint32_t n_poses, uint64_t* counts, void* stream);
 
 /* ---- batched heterogeneous simulation ---------------------------------------------------
  * No counterpart in the reference (it launches one sensor model per optixLaunch,

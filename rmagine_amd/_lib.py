@@ -64,6 +64,8 @@ _SIGNATURES = {
     "rmh_sensor_create": (c_int, [c_int, c_void_p, ctypes.POINTER(c_void_p)]),
     "rmh_sensor_destroy": (c_int, [c_void_p]),
     "rmh_sensor_dims": (c_int, [c_void_p, ctypes.POINTER(c_u32), ctypes.POINTER(c_u32)]),
+    "rmh_simulate_visit_stats": (c_int, [c_void_p, c_void_p, ctypes.POINTER(T.rmh_transform), c_void_p, c_u32,
+                                         ctypes.POINTER(ctypes.c_uint64), c_void_p]),
     "rmh_simulate": (c_int, [c_void_p, c_void_p, ctypes.POINTER(T.rmh_transform), c_void_p, c_u32,
                              ctypes.POINTER(T.rmh_outputs), c_void_p]),
     "rmh_batch_create": (c_int, [c_void_p, ctypes.POINTER(T.rmh_batch_entry), c_u32,

@@ -120,6 +120,55 @@ hipError_t launch_sim(const TravSel& ts, int kind, const SimArgs& a, uint32_t n_
   return launch_stack<128, false>(kind, a, n_poses, stream);
 }
 
+// ---- traversal counts (diagnostic) ------------------------------------------------------
+
+struct VisitArgs {
+  MapData map;
+  RayGen g;
+  Tf Tsb;
+  const rmh_transform* Tbm;
+  Tiling tl;
+  unsigned long long* counts;  // 6 sums, see VisitCount
+};
+
+// k_simulate's rays and traversal (binary nodes, same speculation mode), counting instead
+// of writing: per-lane counts are summed over the wave and added to the totals.
+template <int KIND, int STACK>
+__global__ __launch_bounds__(kWave, 8) void k_visit_stats(const VisitArgs a) {
+  __shared__ int stk[kLdsStack][kWave];
+  const uint32_t lb = xcd_contiguous(blockIdx.x, gridDim.x);
+  const uint32_t pl = lb / a.tl.tiles;
+  const uint32_t tile = lb - pl * a.tl.tiles;
+  uint32_t vid, hid;
+  VisitCount vc = {0u, 0u, 0u, 0u, 0u, 0u};
+  if (tile_ray(a.tl, a.g.grid_w, a.g.grid_h, tile, threadIdx.x, vid, hid)) {
+    uint32_t lid;
+    V3 ds, os;
+    gen_ray<KIND>(a.g, vid, hid, lid, ds, os);
+    const Tf Tsm = tmul(load_tf(a.Tbm[pl]), a.Tsb);
+    Hit h;
+    trace<STACK, kWave, false, spec_mode<KIND>(), true>(a.map, map_origin<KIND>(Tsm, os), qrot(Tsm.R, ds),
+                                                       a.g.range_max, stk, h, false, &vc);
+  }
+  uint32_t v[6] = {vc.vec_nodes, vc.uni_nodes, vc.leaves, vc.tris, vc.wave_iters, vc.wave_uni_iters};
+  for (int k = 0; k < 6; k++)
+    for (int off = 32; off > 0; off >>= 1) v[k] += __shfl_xor(v[k], off, 64);
+  if (threadIdx.x == 0)
+    for (int k = 0; k < 6; k++) atomicAdd(&a.counts[k], static_cast<unsigned long long>(v[k]));
+}
+
+template <int STACK>
+hipError_t launch_visit(int kind, const VisitArgs& a, uint32_t n_poses, hipStream_t stream) {
+  const uint32_t blocks = n_poses * a.tl.tiles;
+  switch (kind) {
+    case RMH_MODEL_SPHERICAL: k_visit_stats<RMH_MODEL_SPHERICAL, STACK><<<blocks, kWave, 0, stream>>>(a); break;
+    case RMH_MODEL_PINHOLE: k_visit_stats<RMH_MODEL_PINHOLE, STACK><<<blocks, kWave, 0, stream>>>(a); break;
+    case RMH_MODEL_O1DN: k_visit_stats<RMH_MODEL_O1DN, STACK><<<blocks, kWave, 0, stream>>>(a); break;
+    default: k_visit_stats<RMH_MODEL_ONDN, STACK><<<blocks, kWave, 0, stream>>>(a); break;
+  }
+  return hipGetLastError();
+}
+
 // ---- batched, persistent ---------------------------------------------------------------
 
 struct BatchEntryDev {
@@ -332,6 +381,46 @@ int rmh_simulate(const rmh_map* map, const rmh_sensor* sensor, const rmh_transfo
     const hipError_t e = launch_sim(ts, sensor->kind, a, np, stream);
     if (e != hipSuccess) return hip_fail(e, "rmh_simulate launch");
   }
+  return RMH_OK;
+}
+
+int rmh_simulate_visit_stats(const rmh_map* map, const rmh_sensor* sensor, const rmh_transform* Tsb,
+                             const rmh_transform* Tbm, uint32_t n_poses, uint64_t* counts, void* stream_) {
+  if (!map) return fail(RMH_ERR_NOMAP, "rmh_simulate_visit_stats: no map set");
+  if (!sensor || !counts) return fail(RMH_ERR_INVALID, "rmh_simulate_visit_stats: NULL argument");
+  for (int k = 0; k < 6; k++) counts[k] = 0;
+  if (n_poses == 0) return RMH_OK;
+  if (!Tbm) return fail(RMH_ERR_INVALID, "rmh_simulate_visit_stats: Tbm is NULL");
+  const uint64_t ray_count = static_cast<uint64_t>(sensor->width) * sensor->height;
+  if (ray_count == 0) return RMH_OK;
+  const uint32_t depth = map->info.max_depth;
+  if (depth > 128) return fail(RMH_ERR_UNSUPPORTED, "rmh_simulate_visit_stats: BVH deeper than 128 levels");
+  VisitArgs a;
+  a.map = make_mapdata(map);
+  a.g = make_raygen(sensor);
+  rmh_transform I{};
+  I.R.w = 1.0f;
+  a.Tsb = load_tf(Tsb ? *Tsb : I);
+  a.Tbm = Tbm;
+  a.tl = sensor_tiling(sensor);
+  if (static_cast<uint64_t>(n_poses) * a.tl.tiles >= (1ull << 31))
+    return fail(RMH_ERR_UNSUPPORTED, "rmh_simulate_visit_stats: too many rays for one launch");
+  unsigned long long* dev = nullptr;
+  RMH_HIP_TRY(hipMalloc(&dev, sizeof(unsigned long long) * 6));
+  hipStream_t stream = static_cast<hipStream_t>(stream_);
+  hipError_t e = hipMemsetAsync(dev, 0, sizeof(unsigned long long) * 6, stream);
+  a.counts = dev;
+  if (e == hipSuccess) {
+    if (depth <= 32) e = launch_visit<32>(sensor->kind, a, n_poses, stream);
+    else if (depth <= 64) e = launch_visit<64>(sensor->kind, a, n_poses, stream);
+    else e = launch_visit<128>(sensor->kind, a, n_poses, stream);
+  }
+  unsigned long long h[6] = {0, 0, 0, 0, 0, 0};
+  if (e == hipSuccess) e = hipMemcpyAsync(h, dev, sizeof(h), hipMemcpyDeviceToHost, stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(stream);
+  (void)hipFree(dev);
+  if (e != hipSuccess) return hip_fail(e, "rmh_simulate_visit_stats");
+  for (int k = 0; k < 6; k++) counts[k] = h[k];
   return RMH_OK;
 }
 

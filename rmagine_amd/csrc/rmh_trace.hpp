@@ -144,9 +144,9 @@ __device__ __forceinline__ void test_leaf(const MapData& m, const TriRay& tr, in
 }
 
 // Traversal stack: the top kLdsStack entries of every lane live in LDS (one column per
-// lane); deeper entries go to a private per-lane array, which measured rays almost never
-// reach (the 2.1M-triangle terrain uses at most 16 of its 29 levels).  4 KB of LDS per wave
-// leaves room for 8 resident waves per SIMD.
+// lane); deeper entries go to a private per-lane array, which measured rays rarely reach
+// (a lane holds one entry per level only where both children were hit).  4 KB of LDS per
+// wave leaves room for 8 resident waves per SIMD.
 constexpr int kLdsStack = 16;
 
 // Node words read through the constant address space, which the compiler serves with
@@ -209,6 +209,23 @@ constexpr int kDone = 0x7fffffff;
 // kernel costs registers and made the batched launch 50 % slower).
 constexpr int kSpecOff = 0, kSpecOn = 1, kSpecRuntime = 2;
 
+// Per-lane traversal counts of the diagnostic build of trace() (COUNT = true): node
+// visits through the vector and the scalar path, leaves, triangle tests, and (on the
+// first active lane only) wave iterations of the node loop and how many were wave-uniform.
+struct VisitCount {
+  uint32_t vec_nodes, uni_nodes, leaves, tris, wave_iters, wave_uni_iters;
+};
+
+__device__ __forceinline__ void count_visit(VisitCount* vc, bool uniform) {
+  if (uniform) vc->uni_nodes++;
+  else vc->vec_nodes++;
+  const uint64_t act = __ballot(1);
+  if (threadIdx.x % 64 == static_cast<uint32_t>(__ffsll(static_cast<unsigned long long>(act)) - 1)) {
+    vc->wave_iters++;
+    if (uniform) vc->wave_uni_iters++;
+  }
+}
+
 // Pops the next subtree of a lane, kDone when its stack is empty.
 template <int STACK, int BLOCK>
 __device__ __forceinline__ int pop_ref(int (*stk)[BLOCK], volatile int* spill, int& sp) {
@@ -226,14 +243,15 @@ __device__ __forceinline__ int pop_ref(int (*stk)[BLOCK], volatile int* spill, i
 // the node iterations of incoherent waves; coherent image tiles reach their leaves
 // together and lose by it (5 %: a held leaf shrinks the ray's interval later), so they
 // walk without it.
-template <int OCT, int STACK, int BLOCK, int SPEC>
+template <int OCT, int STACK, int BLOCK, int SPEC, bool COUNT>
 __device__ __forceinline__ void walk(const MapData& m, const SlabRay& r, float tbf, int (*stk)[BLOCK],
-                                     volatile int* spill, int& ref, int& sp, int& leaf, bool spec) {
+                                     volatile int* spill, int& ref, int& sp, int& leaf, bool spec, VisitCount* vc) {
   const int tid = threadIdx.x;
   while (ref >= 0 && ref != kDone) {
     float tmin0, tmax0, tmin1, tmax1;
     int2 nr;
     const int r0 = __builtin_amdgcn_readfirstlane(ref);
+    if constexpr (COUNT) count_visit(vc, __ballot(ref != r0) == 0);
     if (__ballot(ref != r0) == 0) {
       // every active lane is at the same node (45% of visits on the terrain): read it
       // through the scalar cache, which keeps the vector memory return path free; the
@@ -417,9 +435,9 @@ __device__ __forceinline__ void walk4(const MapData& m, const SlabRay& r, float 
 //
 // Waves whose 64 rays share one direction octant (almost all tiles) run the walk
 // specialised for it; mixed waves run the generic one.
-template <int STACK, int BLOCK, bool WIDE, int SPEC>
+template <int STACK, int BLOCK, bool WIDE, int SPEC, bool COUNT = false>
 __device__ __forceinline__ void trace(const MapData& m, const V3& o, const V3& d, float range_max,
-                                      int (*stk)[BLOCK], Hit& h, bool spec = false) {
+                                      int (*stk)[BLOCK], Hit& h, bool spec = false, VisitCount* vc = nullptr) {
   h.t = (double)range_max;
   h.slot = 0xFFFFFFFFu;
   h.face = 0xFFFFFFFFu;
@@ -462,15 +480,15 @@ __device__ __forceinline__ void trace(const MapData& m, const V3& o, const V3& d
       }
     } else {
       switch (woct) {
-        case 0: walk<0, STACK, BLOCK, SPEC>(m, r, tbf, stk, spill, ref, sp, leaf, spec); break;
-        case 1: walk<1, STACK, BLOCK, SPEC>(m, r, tbf, stk, spill, ref, sp, leaf, spec); break;
-        case 2: walk<2, STACK, BLOCK, SPEC>(m, r, tbf, stk, spill, ref, sp, leaf, spec); break;
-        case 3: walk<3, STACK, BLOCK, SPEC>(m, r, tbf, stk, spill, ref, sp, leaf, spec); break;
-        case 4: walk<4, STACK, BLOCK, SPEC>(m, r, tbf, stk, spill, ref, sp, leaf, spec); break;
-        case 5: walk<5, STACK, BLOCK, SPEC>(m, r, tbf, stk, spill, ref, sp, leaf, spec); break;
-        case 6: walk<6, STACK, BLOCK, SPEC>(m, r, tbf, stk, spill, ref, sp, leaf, spec); break;
-        case 7: walk<7, STACK, BLOCK, SPEC>(m, r, tbf, stk, spill, ref, sp, leaf, spec); break;
-        default: walk<-1, STACK, BLOCK, SPEC>(m, r, tbf, stk, spill, ref, sp, leaf, spec); break;
+        case 0: walk<0, STACK, BLOCK, SPEC, COUNT>(m, r, tbf, stk, spill, ref, sp, leaf, spec, vc); break;
+        case 1: walk<1, STACK, BLOCK, SPEC, COUNT>(m, r, tbf, stk, spill, ref, sp, leaf, spec, vc); break;
+        case 2: walk<2, STACK, BLOCK, SPEC, COUNT>(m, r, tbf, stk, spill, ref, sp, leaf, spec, vc); break;
+        case 3: walk<3, STACK, BLOCK, SPEC, COUNT>(m, r, tbf, stk, spill, ref, sp, leaf, spec, vc); break;
+        case 4: walk<4, STACK, BLOCK, SPEC, COUNT>(m, r, tbf, stk, spill, ref, sp, leaf, spec, vc); break;
+        case 5: walk<5, STACK, BLOCK, SPEC, COUNT>(m, r, tbf, stk, spill, ref, sp, leaf, spec, vc); break;
+        case 6: walk<6, STACK, BLOCK, SPEC, COUNT>(m, r, tbf, stk, spill, ref, sp, leaf, spec, vc); break;
+        case 7: walk<7, STACK, BLOCK, SPEC, COUNT>(m, r, tbf, stk, spill, ref, sp, leaf, spec, vc); break;
+        default: walk<-1, STACK, BLOCK, SPEC, COUNT>(m, r, tbf, stk, spill, ref, sp, leaf, spec, vc); break;
       }
     }
     if constexpr (SPEC == kSpecOff) {
@@ -478,6 +496,7 @@ __device__ __forceinline__ void trace(const MapData& m, const V3& o, const V3& d
       // first would hold one more register through the FP64 tests: 5 % slower on the
       // terrain)
       if (ref == kDone) break;
+      if constexpr (COUNT) { vc->leaves++; vc->tris += (~static_cast<uint32_t>(ref) & 7u) + 1u; }
       test_leaf(m, tr, ref, h);
       tbf = __double2float_ru(h.t);
       ref = pop_ref<STACK, BLOCK>(stk, spill, sp);
@@ -491,6 +510,7 @@ __device__ __forceinline__ void trace(const MapData& m, const V3& o, const V3& d
       ref = pop_ref<STACK, BLOCK>(stk, spill, sp);
     }
     while (leaf < 0) {
+      if constexpr (COUNT) { vc->leaves++; vc->tris += (~static_cast<uint32_t>(leaf) & 7u) + 1u; }
       test_leaf(m, tr, leaf, h);
       tbf = __double2float_ru(h.t);
       if (ref < 0) {

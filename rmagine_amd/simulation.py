@@ -321,6 +321,33 @@ class SimulatorHip:
         return bundle
 
 
+    def visit_stats(self, Tbm, stream=None) -> dict:
+        """Diagnostic: traversal counts of simulate()'s kernel over the same rays (binary
+        nodes), summed over all rays: node visits through vector loads and through the scalar
+        cache, leaves, triangle tests, wave iterations of the node loop and how many of them
+        were wave-uniform; plus the per-ray means."""
+        if self._map is None:
+            raise _lib.NoMapError(3, "visit_stats: no map set")
+        if self._sensor is None:
+            raise RuntimeError("visit_stats: no sensor model set")
+        Tbm = torch.as_tensor(np.asarray(Tbm, dtype=np.float32).reshape(-1, 8), device=self.device) \
+            if not isinstance(Tbm, torch.Tensor) else Tbm
+        if Tbm.dtype != torch.float32 or Tbm.dim() != 2 or Tbm.shape[1] != 8 or not Tbm.is_contiguous():
+            raise ValueError("Tbm must be a contiguous float32 tensor of shape (N, 8)")
+        counts = (ctypes.c_uint64 * 6)()
+        Tsb = T.transform_to_c(self._Tsb)
+        _lib.call("rmh_simulate_visit_stats", self._map.handle, self._sensor.handle, ctypes.byref(Tsb), _ptr(Tbm),
+                  Tbm.shape[0], counts, stream_handle(stream))
+        names = ("vector_node_visits", "scalar_node_visits", "leaves", "triangle_tests", "wave_iterations",
+                 "uniform_wave_iterations")
+        out = {n: int(c) for n, c in zip(names, counts)}
+        rays = Tbm.shape[0] * self._sensor.W * self._sensor.H
+        out["rays"] = rays
+        for n in names[:4]:
+            out[n + "_per_ray"] = out[n] / rays if rays else 0.0
+        return out
+
+
 class SphereSimulatorHip(SimulatorHip):
     _model_type = T.SphericalModel
 

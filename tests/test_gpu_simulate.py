@@ -228,3 +228,28 @@ def test_repeat_is_deterministic(room):
     b = run_gpu(hm, m, Tbm[:10])
     for k in a:
         np.testing.assert_array_equal(a[k].view(np.uint8), b[k].view(np.uint8))
+
+
+def test_visit_stats_closed_forms(cube, room):
+    """rmh_simulate_visit_stats: a one-triangle map is a single leaf (no node visits, one
+    leaf and one triangle test per ray); on real trees the counts are consistent."""
+    one = HipMap(np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], np.float32), np.array([[0, 1, 2]], np.uint32))
+    sim = SphereSimulatorHip(one)
+    m = T.example_spherical()
+    sim.setModel(m)
+    st = sim.visit_stats(T.identity_transforms(3))
+    rays = 3 * m.size()
+    assert st["rays"] == rays
+    assert st["vector_node_visits"] == 0 and st["scalar_node_visits"] == 0 and st["wave_iterations"] == 0
+    assert st["leaves"] == rays and st["triangle_tests"] == rays
+    V, F, Tbm, hm, _ = room
+    sim = SphereSimulatorHip(hm)
+    sim.setModel(scenes.vlp16_workload_model())
+    st = sim.visit_stats(Tbm[:10])
+    nodes = st["vector_node_visits"] + st["scalar_node_visits"]
+    assert nodes > st["rays"] and st["leaves"] >= 1 and st["triangle_tests"] >= st["leaves"]
+    assert st["triangle_tests"] <= 8 * st["leaves"]
+    assert 0 < st["uniform_wave_iterations"] <= st["wave_iterations"] <= nodes
+    assert st["scalar_node_visits"] <= 64 * st["uniform_wave_iterations"]
+    # a repeated call counts the same
+    assert sim.visit_stats(Tbm[:10]) == st
