@@ -153,10 +153,11 @@ __global__ __launch_bounds__(kBlock) void k_bottom_up(int n, const uint32_t* __r
   if (k >= n) return;
   int p = parent_leaf[k];
   while (p >= 0) {
-    __threadfence();
+    // acquire-release at agent scope: the node data this thread wrote is visible before
+    // the count, and the sibling's data after it (explicit fences around it only
+    // repeated the L2 write-back and invalidate of the atomic itself)
     const uint32_t old = __hip_atomic_fetch_add(&arrive[p], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
     if (old == 0u) return;  // first arrival: the sibling's thread finishes this node
-    __threadfence();
     float4 lo[2], hi[2];
     float cost[2], cnt[2];
     for (int c = 0; c < 2; c++) {
@@ -377,24 +378,24 @@ __global__ __launch_bounds__(64) void k_treelet(int n, TreeArrays t, uint32_t* a
         atomicOr(err, 8u);
         active = false;
       } else {
-        __threadfence();
         const uint32_t old = __hip_atomic_fetch_add(&arrive[p], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
         if (old == 0u) {
           active = false;  // first arrival: the sibling's lane finishes this node
         } else {
-          __threadfence();
           refresh_node(t, p, c_node, c_tri);
           need = t.count[p] >= gamma;
         }
       }
     }
     uint64_t todo = __ballot(need);
-    while (todo) {
-      const int l = __ffsll(static_cast<unsigned long long>(todo)) - 1;
-      todo &= todo - 1;
-      treelet_wave(t, __shfl(p, l, 64), c_node, c_tri, s_area, s_copt, s_part);
+    if (todo) {
+      while (todo) {
+        const int l = __ffsll(static_cast<unsigned long long>(todo)) - 1;
+        todo &= todo - 1;
+        treelet_wave(t, __shfl(p, l, 64), c_node, c_tri, s_area, s_copt, s_part);
+      }
+      __threadfence();  // the treelet writes of lane 0 before any lane's next arrival
     }
-    __threadfence();  // the treelet writes of lane 0 before any lane's next arrival
     if (active) p = t.parent_int[p];
   }
 }
@@ -1046,12 +1047,10 @@ __global__ __launch_bounds__(kBlock) void k_refit_nodes(uint32_t ni, const uint8
     leaves++;
   }
   if (leaves == 0) return;  // both children interior: their threads finish this node
-  __threadfence();
   uint32_t old = __hip_atomic_fetch_add(&arrive[i], leaves, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
   if (old + leaves != 2u) return;
   int j = static_cast<int>(i);
   for (;;) {
-    __threadfence();
     const Node2 cur = nodes[j];
     const float4 lo = make_float4(fminf(cur.a.x, cur.b.x), fminf(cur.a.z, cur.b.z), fminf(cur.c.x, cur.c.z), 0.0f);
     const float4 hi = make_float4(fmaxf(cur.a.y, cur.b.y), fmaxf(cur.a.w, cur.b.w), fmaxf(cur.c.y, cur.c.w), 0.0f);
@@ -1062,7 +1061,6 @@ __global__ __launch_bounds__(kBlock) void k_refit_nodes(uint32_t ni, const uint8
     set_child_box(pn, c, lo, hi);
     if (c == 0) { nodes[p].a = pn.a; nodes[p].c.x = pn.c.x; nodes[p].c.y = pn.c.y; }
     else { nodes[p].b = pn.b; nodes[p].c.z = pn.c.z; nodes[p].c.w = pn.c.w; }
-    __threadfence();
     old = __hip_atomic_fetch_add(&arrive[p], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
     if (old != 1u) return;  // first arrival: the sibling's thread continues
     j = p;
