@@ -87,27 +87,33 @@ def cpu_baseline(V, F, model, Tbm, seconds, gpu_numpy_for_pose):
     per_pose = []
     worst = {"range_max_err": 0.0, "id_mismatch": 0, "hit_mismatch": 0, "ambiguous": 0, "rays": 0}
     oracle.simulate(om, model, Tbm[:1], threads=threads, flags=False)  # warm-up, not timed
-    while elapsed < seconds and poses < len(Tbm):
+    # the reference's benchmark loop (velodyne_benchmark.hpp:92-114): run for at least
+    # `seconds`, report the running mean; poses are cast in order and, once all are done,
+    # again from the first (the accuracy check covers the first pass)
+    while elapsed < seconds:
+        p = poses % len(Tbm)
         t = time.perf_counter()
-        o = oracle.simulate(om, model, Tbm[poses:poses + 1], threads=threads, flags=False)
+        o = oracle.simulate(om, model, Tbm[p:p + 1], threads=threads, flags=False)
         dt = time.perf_counter() - t
         elapsed += dt
         per_pose.append(dt)
-        g = gpu_numpy_for_pose(poses)
-        worst["hit_mismatch"] += int((g["hits"] != o["hits"]).sum())
-        worst["id_mismatch"] += int((g["face_ids"] != o["face_ids"]).sum())
-        worst["range_max_err"] = max(worst["range_max_err"],
-                                     float(np.abs(g["ranges"].astype(np.float64) - o["ranges"]).max()))
-        worst["rays"] += len(o["ranges"])
+        if poses < len(Tbm):
+            g = gpu_numpy_for_pose(p)
+            worst["hit_mismatch"] += int((g["hits"] != o["hits"]).sum())
+            worst["id_mismatch"] += int((g["face_ids"] != o["face_ids"]).sum())
+            worst["range_max_err"] = max(worst["range_max_err"],
+                                         float(np.abs(g["ranges"].astype(np.float64) - o["ranges"]).max()))
+            worst["rays"] += len(o["ranges"])
         rays += len(o["ranges"])
         poses += 1
-    log(f"cpu: {poses} poses, {rays} rays in {elapsed:.2f} s on {threads} threads")
+    log(f"cpu: {poses} pose casts, {rays} rays in {elapsed:.2f} s on {threads} threads")
     med = float(np.median(per_pose))
     base = {"value": rays / elapsed / 1e6, "unit": "Mrays/s", "cores": threads, "kind": "port",
             "median": model.size() / med / 1e6, "cpu": cpu_model(),
-            "sample": f"{poses} of the {len(Tbm)} sensor poses ({rays} rays), all attributes, oracle "
-                      f"(exact FP64 predicate, binned-SAH BVH) on {threads} host threads after one untimed "
-                      f"pose, BVH build excluded; value = rays / total time, median = per-pose median"}
+            "sample": f"{poses} casts of the {len(Tbm)} sensor poses in order ({rays} rays, {elapsed:.1f} s), all "
+                      f"attributes, oracle (exact FP64 predicate, binned-SAH BVH) on {threads} host threads after one "
+                      f"untimed pose, BVH build excluded; value = rays / total time, median = per-pose median; "
+                      f"accuracy over the first {min(poses, len(Tbm))} poses"}
     return base, worst
 
 
