@@ -217,6 +217,15 @@ int rmh_ouster_o1dn(const char* path, uint32_t* width, uint32_t* height, float* 
  * it needs (direction tables, O1Dn/OnDn rays). */
 int rmh_sensor_create(int kind, const void* model, rmh_sensor** out);
 int rmh_sensor_destroy(rmh_sensor* sensor);
+/* Traversal order of an O1Dn/OnDn sensor's rays (no counterpart in the reference, which
+ * casts them in list order): sorted != 0 = by direction (the default: a wave's 64 rays
+ * cover one patch of the sphere), 0 = list order (rays already ordered by the caller, e.g.
+ * a scanner's row order).  Results are identical either way; a tuning choice of the
+ * autotuner.  Rewrites the sensor's device arrays in place (batches and graphs that use
+ * the sensor stay valid); synchronous; no launch using the sensor may be in flight.
+ * Image sensors (spherical, pinhole) ignore it and report 0. */
+int rmh_sensor_set_ray_order(rmh_sensor* sensor, int sorted);
+int rmh_sensor_get_ray_order(const rmh_sensor* sensor, int* sorted);
 int rmh_sensor_dims(const rmh_sensor* sensor, uint32_t* width, uint32_t* height);
 
 /* ---- simulation --------------------------------------------------------------------- */
@@ -260,6 +269,8 @@ typedef struct {
   uint32_t tile_rows;     /* image rows per 64-ray tile of spherical/pinhole sensors:
                              1, 2, 4, 8, 16, 32 or 64, capped by the image height
                              (0 = keep / default 4)                                      */
+  uint32_t lds_stack;     /* traversal stack entries per lane kept in LDS: 8, 16 or 24
+                             (0 = keep / default 16); deeper entries go to scratch       */
 } rmh_batch_params;
 
 typedef struct rmh_batch rmh_batch;

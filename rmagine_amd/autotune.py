@@ -8,10 +8,14 @@ sensors better, large ones cost fewer atomics; more waves hide more latency unti
 registers run out; the tile shape decides how coherent a wave's rays are (on a spinning
 lidar wide tiles win because rays of one row end at similar ranges).  The tuner times
 every candidate on the caller's actual batch with HIP events and keeps the fastest:
-first chunk x waves at the default tile, then the tile rows at the best pair, then (when
-asked with layouts=(False, True)) the BVH node layout of the batch's map: binary or 4-wide
-with quantised boxes, which changes speed, never results.  Results are cached as JSON keyed by device architecture, mesh size (power of
-two bucket of the face count) and sensor mix, in $RMH_TUNE_CACHE or
+first chunk x waves at the default tile, then the tile rows at the best pair, then the
+depth of the traversal stack kept in LDS (`lds_stack`: 8, 16 or 24 entries per ray; a
+shallower one leaves more LDS for resident waves, a deeper one spills less to private
+memory), then the ray order of the O1Dn/OnDn sensors (sorted by direction, or the list
+order the caller gave), then (when asked with layouts=(False, True)) the BVH node layout of
+the batch's map: binary or 4-wide with quantised boxes.  Stack depth, ray order and layout
+change speed, never results.  Results are cached as JSON keyed by device architecture, mesh
+size (power of two bucket of the face count) and sensor mix, in $RMH_TUNE_CACHE or
 ~/.cache/rmagine_amd/autotune.json.
 
 The reference has no counterpart (its launch shape is fixed by optixLaunch).
@@ -28,6 +32,8 @@ import torch
 DEFAULT_CHUNKS = (64, 128, 256, 512, 1024)
 DEFAULT_WAVES = (8, 16, 24, 32)
 DEFAULT_TILE_ROWS = (1, 2, 4, 8, 16)
+DEFAULT_LDS_STACKS = (8, 16, 24)
+DEFAULT_RAY_ORDERS = (True, False)  # True = direction-sorted
 
 
 def cache_path() -> Path:
@@ -78,17 +84,20 @@ def time_batch(batch, reps: int = 3, stream=None) -> float:
 
 def tune_batch(batch, key: str | None = None, chunks=DEFAULT_CHUNKS, waves=DEFAULT_WAVES,
                tile_rows=DEFAULT_TILE_ROWS, reps: int = 3, use_cache: bool = True, stream=None,
-               layouts=(False,)) -> dict:
-    """Picks (chunk_rays, waves_per_cu, tile_rows) for `batch` and, if `layouts` holds both
-    False and True, the node layout of its map (True = 4-wide quantised); applies the choice
-    and returns the record.  Re-capture graphs of the batch afterwards."""
+               layouts=(False,), lds_stacks=DEFAULT_LDS_STACKS, ray_orders=DEFAULT_RAY_ORDERS) -> dict:
+    """Picks (chunk_rays, waves_per_cu, tile_rows, lds_stack, ray order) for `batch` and, if
+    `layouts` holds both False and True, the node layout of its map (True = 4-wide
+    quantised); applies the choice and returns the record.  Re-capture graphs of the batch
+    afterwards."""
     path = cache_path()
     hm = batch.map
     if use_cache and key:
         rec = _load(path).get(key)
         if rec:
             batch.set_params(chunk_rays=rec["chunk_rays"], waves_per_cu=rec["waves_per_cu"],
-                             tile_rows=rec.get("tile_rows", 4))
+                             tile_rows=rec.get("tile_rows", 4), lds_stack=rec.get("lds_stack", 0))
+            if "rays_sorted" in rec:
+                batch.set_ray_order(rec["rays_sorted"])
             if "wide" in rec and hm.wide != rec["wide"]:
                 hm.set_layout(rec["wide"])
             return dict(rec, cached=True)
@@ -110,6 +119,30 @@ def tune_batch(batch, key: str | None = None, chunks=DEFAULT_CHUNKS, waves=DEFAU
     rec = {"chunk_rays": c, "waves_per_cu": w, "tile_rows": r, "ms": ms,
            "trials": [{"chunk_rays": tc, "waves_per_cu": tw, "tile_rows": tr, "ms": tm} for tm, tc, tw, tr in trials]}
     keys = ["chunk_rays", "waves_per_cu", "tile_rows", "ms"]
+    lds0 = batch.params()["lds_stack"]
+    stack_ms = {lds0: ms}
+    for d in lds_stacks:
+        if d not in stack_ms:
+            batch.set_params(lds_stack=d)
+            stack_ms[d] = time_batch(batch, reps, stream)
+    best_d = min(stack_ms, key=stack_ms.get)
+    batch.set_params(lds_stack=best_d)
+    ms = stack_ms[best_d]
+    rec.update(lds_stack=best_d, ms=ms, lds_stack_ms=stack_ms)
+    keys.append("lds_stack")
+    order0 = batch.ray_order_sorted()
+    order_ms = {order0: ms}
+    for o in ray_orders:
+        if bool(o) not in order_ms:
+            batch.set_ray_order(bool(o))
+            if batch.ray_order_sorted() == bool(o):  # image-only batches have one order
+                order_ms[bool(o)] = time_batch(batch, reps, stream)
+    best_o = min(order_ms, key=order_ms.get)
+    if batch.ray_order_sorted() != best_o:
+        batch.set_ray_order(best_o)
+    ms = order_ms[best_o]
+    rec.update(rays_sorted=best_o, ms=ms, ray_order_ms={("sorted" if k else "list"): v for k, v in order_ms.items()})
+    keys.append("rays_sorted")
     if len(set(layouts)) > 1:
         start = hm.wide
         timed = {start: ms}

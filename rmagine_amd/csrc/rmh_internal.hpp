@@ -169,5 +169,11 @@ struct rmh_sensor {
   rmh_vec3* dirs = nullptr;       // W*H
   rmh_vec3* origs = nullptr;      // OnDn only, W*H
   uint32_t* order = nullptr;      // W*H
+  // the rays in list order and the direction-sorted permutation, kept so that the
+  // traversal order can be switched (rmh_sensor_set_ray_order) without the caller's arrays
+  rmh_vec3* dirs_list = nullptr;
+  rmh_vec3* origs_list = nullptr;
+  uint32_t* perm = nullptr;
+  int sorted = 1;
   int device = 0;
 };

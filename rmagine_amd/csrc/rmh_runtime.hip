@@ -144,6 +144,9 @@ static void sensor_free(rmh_sensor* s) {
   if (s->dirs) (void)hipFree(s->dirs);
   if (s->origs) (void)hipFree(s->origs);
   if (s->order) (void)hipFree(s->order);
+  if (s->dirs_list) (void)hipFree(s->dirs_list);
+  if (s->origs_list) (void)hipFree(s->origs_list);
+  if (s->perm) (void)hipFree(s->perm);
   delete s;
 }
 
@@ -186,6 +189,11 @@ __global__ void k_dir_keys(const rmh_vec3* __restrict__ d, uint32_t n, uint32_t*
   idx[i] = i;
 }
 
+__global__ void k_iota(uint32_t* __restrict__ p, uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = i;
+}
+
 __global__ void k_gather_rays(const uint32_t* __restrict__ order, uint32_t n, const rmh_vec3* __restrict__ d_in,
                               const rmh_vec3* __restrict__ o_in, rmh_vec3* __restrict__ d_out,
                               rmh_vec3* __restrict__ o_out) {
@@ -206,6 +214,12 @@ static hipError_t upload_sorted_rays(rmh_sensor* s, const rmh_vec3* dirs_in, con
   hipError_t e = hipMalloc(&s->dirs, n * sizeof(rmh_vec3));
   if (e == hipSuccess && origs_in) e = hipMalloc(&s->origs, n * sizeof(rmh_vec3));
   if (e == hipSuccess) e = hipMalloc(&s->order, n * sizeof(uint32_t));
+  if (e == hipSuccess) e = hipMalloc(&s->perm, n * sizeof(uint32_t));
+  if (e == hipSuccess) e = hipMalloc(&s->dirs_list, n * sizeof(rmh_vec3));
+  if (e == hipSuccess && origs_in) e = hipMalloc(&s->origs_list, n * sizeof(rmh_vec3));
+  if (e == hipSuccess) e = hipMemcpy(s->dirs_list, dirs_in, n * sizeof(rmh_vec3), hipMemcpyDeviceToDevice);
+  if (e == hipSuccess && origs_in)
+    e = hipMemcpy(s->origs_list, origs_in, n * sizeof(rmh_vec3), hipMemcpyDeviceToDevice);
   if (e == hipSuccess) e = hipMalloc(&keys, n * sizeof(uint32_t));
   if (e == hipSuccess) e = hipMalloc(&keys2, n * sizeof(uint32_t));
   if (e == hipSuccess) e = hipMalloc(&idx, n * sizeof(uint32_t));
@@ -216,16 +230,49 @@ static hipError_t upload_sorted_rays(rmh_sensor* s, const rmh_vec3* dirs_in, con
   if (e == hipSuccess) e = rocprim::radix_sort_pairs(nullptr, tmp_bytes, keys, keys2, idx, s->order, n);
   if (e == hipSuccess) e = hipMalloc(&tmp, tmp_bytes > 0 ? tmp_bytes : 1);
   if (e == hipSuccess) e = rocprim::radix_sort_pairs(tmp, tmp_bytes, keys, keys2, idx, s->order, n);
+  if (e == hipSuccess) e = hipMemcpy(s->perm, s->order, n * sizeof(uint32_t), hipMemcpyDeviceToDevice);
   if (e == hipSuccess) {
     k_gather_rays<<<blocks, 256>>>(s->order, n, dirs_in, origs_in, s->dirs, s->origs);
     e = hipGetLastError();
   }
   if (e == hipSuccess) e = hipDeviceSynchronize();
+  s->sorted = 1;
   if (keys) (void)hipFree(keys);
   if (keys2) (void)hipFree(keys2);
   if (idx) (void)hipFree(idx);
   if (tmp) (void)hipFree(tmp);
   return e;
+}
+
+// Rewrites the traversal arrays in place (same buffers, so batch tables and captured
+// graphs that point at them stay valid): sorted = the direction order, else list order.
+int rmh_sensor_set_ray_order(rmh_sensor* s, int sorted) {
+  if (!s) return fail(RMH_ERR_INVALID, "rmh_sensor_set_ray_order: NULL sensor");
+  if (s->kind != RMH_MODEL_O1DN && s->kind != RMH_MODEL_ONDN) return RMH_OK;  // images: no list
+  sorted = sorted ? 1 : 0;
+  if (sorted == s->sorted) return RMH_OK;
+  const uint32_t n = s->width * s->height;
+  const uint32_t blocks = (n + 255) / 256;
+  RMH_HIP_TRY(hipDeviceSynchronize());  // no launch may read the arrays meanwhile
+  if (sorted) {
+    RMH_HIP_TRY(hipMemcpy(s->order, s->perm, n * sizeof(uint32_t), hipMemcpyDeviceToDevice));
+    k_gather_rays<<<blocks, 256>>>(s->order, n, s->dirs_list, s->origs_list, s->dirs, s->origs);
+    RMH_HIP_TRY(hipGetLastError());
+  } else {
+    k_iota<<<blocks, 256>>>(s->order, n);
+    RMH_HIP_TRY(hipGetLastError());
+    RMH_HIP_TRY(hipMemcpy(s->dirs, s->dirs_list, n * sizeof(rmh_vec3), hipMemcpyDeviceToDevice));
+    if (s->origs) RMH_HIP_TRY(hipMemcpy(s->origs, s->origs_list, n * sizeof(rmh_vec3), hipMemcpyDeviceToDevice));
+  }
+  RMH_HIP_TRY(hipDeviceSynchronize());
+  s->sorted = sorted;
+  return RMH_OK;
+}
+
+int rmh_sensor_get_ray_order(const rmh_sensor* s, int* sorted) {
+  if (!s || !sorted) return fail(RMH_ERR_INVALID, "rmh_sensor_get_ray_order: NULL argument");
+  *sorted = (s->kind == RMH_MODEL_O1DN || s->kind == RMH_MODEL_ONDN) ? s->sorted : 0;
+  return RMH_OK;
 }
 
 int rmh_sensor_create(int kind, const void* model, rmh_sensor** out) {

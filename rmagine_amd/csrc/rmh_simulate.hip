@@ -205,7 +205,7 @@ __device__ __forceinline__ void batch_gen(const BatchEntryDev& e, uint32_t vid, 
 // walks instead of one per kind keeps the kernel inside the instruction cache).  As in
 // k_simulate, the ray set-up is recomputed for the writers rather than held (spilled)
 // through the traversal.
-template <int STACK, int BLOCK, bool WIDE>
+template <int STACK, int BLOCK, bool WIDE, int LDS>
 __device__ __forceinline__ void batch_ray(const BatchArgs& a, const BatchEntryDev& e, int kind, uint32_t pose,
                                           uint32_t vid, uint32_t hid, int (*stk)[BLOCK]) {
   Hit h;
@@ -219,7 +219,7 @@ __device__ __forceinline__ void batch_ray(const BatchArgs& a, const BatchEntryDe
       case RMH_MODEL_O1DN: batch_gen<RMH_MODEL_O1DN>(e, vid, hid, lid, Tsm, ds, os, dm, om); break;
       default: batch_gen<RMH_MODEL_ONDN>(e, vid, hid, lid, Tsm, ds, os, dm, om); break;
     }
-    trace<STACK, BLOCK, WIDE, kSpecRuntime>(a.map, om, dm, e.g.range_max, stk, h, speculative_walk(kind));
+    trace<STACK, BLOCK, WIDE, kSpecRuntime, false, LDS>(a.map, om, dm, e.g.range_max, stk, h, speculative_walk(kind));
   }
   asm volatile("" : "+v"(vid), "+v"(hid));
   const Tf Tsm = tmul(load_tf(e.Tbm[pose]), e.Tsb);
@@ -244,9 +244,9 @@ __global__ void k_reset_counter(uint32_t* counter) {
   if (threadIdx.x == 0) *counter = 0u;  // a kernel store (not a memset node) keeps graph replays correct
 }
 
-template <int STACK, int BLOCK, bool WIDE>
+template <int STACK, int BLOCK, bool WIDE, int LDS>
 __global__ __launch_bounds__(BLOCK, 8) void k_batch(const BatchArgs a) {
-  __shared__ int stk[kLdsStack][BLOCK];
+  __shared__ int stk[LDS][BLOCK];
   const int lane = threadIdx.x & 63;
   for (;;) {
     uint32_t c = 0;
@@ -270,17 +270,28 @@ __global__ __launch_bounds__(BLOCK, 8) void k_batch(const BatchArgs a) {
       const uint32_t tile = static_cast<uint32_t>(tg - static_cast<uint64_t>(pose) * e.tl.tiles);
       uint32_t vid, hid;
       if (!tile_ray(e.tl, e.g.grid_w, e.g.grid_h, tile, lane, vid, hid)) continue;
-      batch_ray<STACK, BLOCK, WIDE>(a, e, kind, pose, vid, hid, stk);
+      batch_ray<STACK, BLOCK, WIDE, LDS>(a, e, kind, pose, vid, hid, stk);
     }
   }
 }
 
-template <int STACK, bool WIDE>
+template <int STACK, bool WIDE, int LDS>
 hipError_t launch_batch(const BatchArgs& a, uint32_t blocks, hipStream_t stream) {
   k_reset_counter<<<1, 64, 0, stream>>>(a.counter);
-  k_batch<STACK, kWave, WIDE><<<blocks, kWave, 0, stream>>>(a);
+  k_batch<STACK, kWave, WIDE, LDS><<<blocks, kWave, 0, stream>>>(a);
   return hipGetLastError();
 }
+
+// LDS stack depth of the batched kernel (a launch parameter the autotuner chooses): 8, 16
+// or 24 entries per lane (2, 4 or 6 KiB per wave); deeper entries go to scratch.
+template <int STACK, bool WIDE>
+hipError_t launch_batch_lds(uint32_t lds, const BatchArgs& a, uint32_t blocks, hipStream_t stream) {
+  if (lds == 8) return launch_batch<STACK, WIDE, 8>(a, blocks, stream);
+  if (lds == 24) return launch_batch<STACK, WIDE, 24>(a, blocks, stream);
+  return launch_batch<STACK, WIDE, 16>(a, blocks, stream);
+}
+
+bool valid_lds_stack(uint32_t v) { return v == 8 || v == 16 || v == 24; }
 
 }  // namespace
 
@@ -438,10 +449,12 @@ int rmh_batch_create(const rmh_map* map, const rmh_batch_entry* entries, uint32_
   b->params.chunk_rays = 256;
   b->params.waves_per_cu = 32;
   b->params.tile_rows = kDefaultTileRows;
+  b->params.lds_stack = kLdsStack;
   if (params) {
     if (params->chunk_rays) b->params.chunk_rays = params->chunk_rays;
     if (params->waves_per_cu) b->params.waves_per_cu = params->waves_per_cu;
     if (params->tile_rows) b->params.tile_rows = params->tile_rows;
+    if (params->lds_stack) b->params.lds_stack = params->lds_stack;
   }
   if (b->params.chunk_rays % 64 != 0 || b->params.chunk_rays > (1u << 20)) {
     batch_free(b);
@@ -450,6 +463,10 @@ int rmh_batch_create(const rmh_map* map, const rmh_batch_entry* entries, uint32_
   if (b->params.waves_per_cu > 64 || !valid_tile_rows(b->params.tile_rows)) {
     batch_free(b);
     return fail(RMH_ERR_INVALID, "rmh_batch_create: waves_per_cu must be <= 64 and tile_rows a power of two <= 64");
+  }
+  if (!valid_lds_stack(b->params.lds_stack)) {
+    batch_free(b);
+    return fail(RMH_ERR_INVALID, "rmh_batch_create: lds_stack must be 8, 16 or 24");
   }
   int dev = 0, ncu = 0;
   (void)hipGetDevice(&dev);
@@ -485,6 +502,9 @@ int rmh_batch_set_params(rmh_batch* b, const rmh_batch_params* params) {
   if (params->chunk_rays) p.chunk_rays = params->chunk_rays;
   if (params->waves_per_cu) p.waves_per_cu = params->waves_per_cu;
   if (params->tile_rows) p.tile_rows = params->tile_rows;
+  if (params->lds_stack) p.lds_stack = params->lds_stack;
+  if (!valid_lds_stack(p.lds_stack))
+    return fail(RMH_ERR_INVALID, "rmh_batch_set_params: lds_stack must be 8, 16 or 24");
   if (p.chunk_rays % 64 != 0 || p.chunk_rays > (1u << 20))
     return fail(RMH_ERR_INVALID, "rmh_batch_set_params: chunk_rays must be a multiple of 64 (<= 2^20)");
   if (p.waves_per_cu > 64) return fail(RMH_ERR_INVALID, "rmh_batch_set_params: waves_per_cu must be <= 64");
@@ -529,14 +549,15 @@ int rmh_batch_simulate(rmh_batch* b, void* stream_) {
   hipStream_t stream = static_cast<hipStream_t>(stream_);
   hipError_t e;
   if (ts.stack < 0) return fail(RMH_ERR_UNSUPPORTED, "rmh_batch_simulate: BVH deeper than 128 levels");
+  const uint32_t lds = b->params.lds_stack;
   if (ts.wide) {
-    if (ts.stack == 32) e = launch_batch<32, true>(a, blocks, stream);
-    else if (ts.stack == 64) e = launch_batch<64, true>(a, blocks, stream);
-    else e = launch_batch<128, true>(a, blocks, stream);
+    if (ts.stack == 32) e = launch_batch_lds<32, true>(lds, a, blocks, stream);
+    else if (ts.stack == 64) e = launch_batch_lds<64, true>(lds, a, blocks, stream);
+    else e = launch_batch_lds<128, true>(lds, a, blocks, stream);
   } else {
-    if (ts.stack == 32) e = launch_batch<32, false>(a, blocks, stream);
-    else if (ts.stack == 64) e = launch_batch<64, false>(a, blocks, stream);
-    else e = launch_batch<128, false>(a, blocks, stream);
+    if (ts.stack == 32) e = launch_batch_lds<32, false>(lds, a, blocks, stream);
+    else if (ts.stack == 64) e = launch_batch_lds<64, false>(lds, a, blocks, stream);
+    else e = launch_batch_lds<128, false>(lds, a, blocks, stream);
   }
   if (e != hipSuccess) return hip_fail(e, "rmh_batch_simulate launch");
   return RMH_OK;

@@ -227,12 +227,12 @@ __device__ __forceinline__ void count_visit(VisitCount* vc, bool uniform) {
 }
 
 // Pops the next subtree of a lane, kDone when its stack is empty.
-template <int STACK, int BLOCK>
+template <int STACK, int BLOCK, int LDS>
 __device__ __forceinline__ int pop_ref(int (*stk)[BLOCK], volatile int* spill, int& sp) {
   if (sp == 0) return kDone;
   --sp;
-  if (sp < kLdsStack) return stk[sp][threadIdx.x];
-  return spill[sp - kLdsStack];
+  if (sp < LDS) return stk[sp][threadIdx.x];
+  return spill[sp - LDS];
 }
 
 // spec (wave-uniform): speculative walk (Aila and Laine, "Understanding the Efficiency of
@@ -243,7 +243,7 @@ __device__ __forceinline__ int pop_ref(int (*stk)[BLOCK], volatile int* spill, i
 // the node iterations of incoherent waves; coherent image tiles reach their leaves
 // together and lose by it (5 %: a held leaf shrinks the ray's interval later), so they
 // walk without it.
-template <int OCT, int STACK, int BLOCK, int SPEC, bool COUNT>
+template <int OCT, int STACK, int BLOCK, int SPEC, bool COUNT, int LDS>
 __device__ __forceinline__ void walk(const MapData& m, const SlabRay& r, float tbf, int (*stk)[BLOCK],
                                      volatile int* spill, int& ref, int& sp, int& leaf, bool spec, VisitCount* vc) {
   const int tid = threadIdx.x;
@@ -282,8 +282,8 @@ __device__ __forceinline__ void walk(const MapData& m, const SlabRay& r, float t
     if (h0 && h1) {
       int near = nr.x, far = nr.y;
       if (tmin1 < tmin0) { near = nr.y; far = nr.x; }
-      if (sp < kLdsStack) stk[sp][tid] = far;
-      else if (sp < STACK) spill[sp - kLdsStack] = far;
+      if (sp < LDS) stk[sp][tid] = far;
+      else if (sp < STACK) spill[sp - LDS] = far;
       sp++;
       ref = near;
     } else if (h0) {
@@ -291,12 +291,12 @@ __device__ __forceinline__ void walk(const MapData& m, const SlabRay& r, float t
     } else if (h1) {
       ref = nr.y;
     } else {
-      ref = pop_ref<STACK, BLOCK>(stk, spill, sp);
+      ref = pop_ref<STACK, BLOCK, LDS>(stk, spill, sp);
     }
     if (SPEC == kSpecOn || (SPEC == kSpecRuntime && spec)) {
       if (ref < 0 && leaf == 0) {
         leaf = ref;
-        ref = pop_ref<STACK, BLOCK>(stk, spill, sp);
+        ref = pop_ref<STACK, BLOCK, LDS>(stk, spill, sp);
       }
       if (__popcll(__ballot(leaf != 0)) * 4 >= __popcll(__ballot(1))) break;
     }
@@ -356,7 +356,7 @@ __device__ __forceinline__ int pick_ref(uint32_t key, int r0, int r1, int r2, in
 // (keys = distance bits with the child slot in the low 2 bits, sorted by a 5-exchange
 // network), the nearest is taken next and the others are pushed farthest first.  The host
 // picks STACK >= 3 x the number of wide levels (at most 3 entries are pushed per level).
-template <int OCT, int STACK, int BLOCK>
+template <int OCT, int STACK, int BLOCK, int LDS>
 __device__ __forceinline__ void walk4(const MapData& m, const SlabRay& r, float tbf, int (*stk)[BLOCK],
                                       volatile int* spill, int& ref, int& sp) {
   const int tid = threadIdx.x;
@@ -401,26 +401,26 @@ __device__ __forceinline__ void walk4(const MapData& m, const SlabRay& r, float 
     if (k0 == 0xFFFFFFFFu) {
       if (sp == 0) { ref = kDone; return; }
       --sp;
-      if (sp < kLdsStack) ref = stk[sp][tid];
-      else ref = spill[sp - kLdsStack];
+      if (sp < LDS) ref = stk[sp][tid];
+      else ref = spill[sp - LDS];
       continue;
     }
     if (k3 != 0xFFFFFFFFu) {
       const int v = pick_ref(k3, c0, c1, c2, c3);
-      if (sp < kLdsStack) stk[sp][tid] = v;
-      else if (sp < STACK) spill[sp - kLdsStack] = v;
+      if (sp < LDS) stk[sp][tid] = v;
+      else if (sp < STACK) spill[sp - LDS] = v;
       sp++;
     }
     if (k2 != 0xFFFFFFFFu) {
       const int v = pick_ref(k2, c0, c1, c2, c3);
-      if (sp < kLdsStack) stk[sp][tid] = v;
-      else if (sp < STACK) spill[sp - kLdsStack] = v;
+      if (sp < LDS) stk[sp][tid] = v;
+      else if (sp < STACK) spill[sp - LDS] = v;
       sp++;
     }
     if (k1 != 0xFFFFFFFFu) {
       const int v = pick_ref(k1, c0, c1, c2, c3);
-      if (sp < kLdsStack) stk[sp][tid] = v;
-      else if (sp < STACK) spill[sp - kLdsStack] = v;
+      if (sp < LDS) stk[sp][tid] = v;
+      else if (sp < STACK) spill[sp - LDS] = v;
       sp++;
     }
     ref = pick_ref(k0, c0, c1, c2, c3);
@@ -435,7 +435,7 @@ __device__ __forceinline__ void walk4(const MapData& m, const SlabRay& r, float 
 //
 // Waves whose 64 rays share one direction octant (almost all tiles) run the walk
 // specialised for it; mixed waves run the generic one.
-template <int STACK, int BLOCK, bool WIDE, int SPEC, bool COUNT = false>
+template <int STACK, int BLOCK, bool WIDE, int SPEC, bool COUNT = false, int LDS = kLdsStack>
 __device__ __forceinline__ void trace(const MapData& m, const V3& o, const V3& d, float range_max,
                                       int (*stk)[BLOCK], Hit& h, bool spec = false, VisitCount* vc = nullptr) {
   h.t = (double)range_max;
@@ -460,7 +460,7 @@ __device__ __forceinline__ void trace(const MapData& m, const V3& o, const V3& d
   const TriRay tr = make_tri_ray(o, d);
   // volatile keeps the rare deep entries in scratch and stops the compiler from merging
   // the LDS and scratch pops into one flat load
-  volatile int spill[STACK > kLdsStack ? STACK - kLdsStack : 1];
+  volatile int spill[STACK > LDS ? STACK - LDS : 1];
   float tbf = range_max;
   int ref = WIDE ? m.root4 : m.root;
   int sp = 0;
@@ -468,27 +468,27 @@ __device__ __forceinline__ void trace(const MapData& m, const V3& o, const V3& d
   for (;;) {
     if constexpr (WIDE) {
       switch (woct) {
-        case 0: walk4<0, STACK, BLOCK>(m, r, tbf, stk, spill, ref, sp); break;
-        case 1: walk4<1, STACK, BLOCK>(m, r, tbf, stk, spill, ref, sp); break;
-        case 2: walk4<2, STACK, BLOCK>(m, r, tbf, stk, spill, ref, sp); break;
-        case 3: walk4<3, STACK, BLOCK>(m, r, tbf, stk, spill, ref, sp); break;
-        case 4: walk4<4, STACK, BLOCK>(m, r, tbf, stk, spill, ref, sp); break;
-        case 5: walk4<5, STACK, BLOCK>(m, r, tbf, stk, spill, ref, sp); break;
-        case 6: walk4<6, STACK, BLOCK>(m, r, tbf, stk, spill, ref, sp); break;
-        case 7: walk4<7, STACK, BLOCK>(m, r, tbf, stk, spill, ref, sp); break;
-        default: walk4<-1, STACK, BLOCK>(m, r, tbf, stk, spill, ref, sp); break;
+        case 0: walk4<0, STACK, BLOCK, LDS>(m, r, tbf, stk, spill, ref, sp); break;
+        case 1: walk4<1, STACK, BLOCK, LDS>(m, r, tbf, stk, spill, ref, sp); break;
+        case 2: walk4<2, STACK, BLOCK, LDS>(m, r, tbf, stk, spill, ref, sp); break;
+        case 3: walk4<3, STACK, BLOCK, LDS>(m, r, tbf, stk, spill, ref, sp); break;
+        case 4: walk4<4, STACK, BLOCK, LDS>(m, r, tbf, stk, spill, ref, sp); break;
+        case 5: walk4<5, STACK, BLOCK, LDS>(m, r, tbf, stk, spill, ref, sp); break;
+        case 6: walk4<6, STACK, BLOCK, LDS>(m, r, tbf, stk, spill, ref, sp); break;
+        case 7: walk4<7, STACK, BLOCK, LDS>(m, r, tbf, stk, spill, ref, sp); break;
+        default: walk4<-1, STACK, BLOCK, LDS>(m, r, tbf, stk, spill, ref, sp); break;
       }
     } else {
       switch (woct) {
-        case 0: walk<0, STACK, BLOCK, SPEC, COUNT>(m, r, tbf, stk, spill, ref, sp, leaf, spec, vc); break;
-        case 1: walk<1, STACK, BLOCK, SPEC, COUNT>(m, r, tbf, stk, spill, ref, sp, leaf, spec, vc); break;
-        case 2: walk<2, STACK, BLOCK, SPEC, COUNT>(m, r, tbf, stk, spill, ref, sp, leaf, spec, vc); break;
-        case 3: walk<3, STACK, BLOCK, SPEC, COUNT>(m, r, tbf, stk, spill, ref, sp, leaf, spec, vc); break;
-        case 4: walk<4, STACK, BLOCK, SPEC, COUNT>(m, r, tbf, stk, spill, ref, sp, leaf, spec, vc); break;
-        case 5: walk<5, STACK, BLOCK, SPEC, COUNT>(m, r, tbf, stk, spill, ref, sp, leaf, spec, vc); break;
-        case 6: walk<6, STACK, BLOCK, SPEC, COUNT>(m, r, tbf, stk, spill, ref, sp, leaf, spec, vc); break;
-        case 7: walk<7, STACK, BLOCK, SPEC, COUNT>(m, r, tbf, stk, spill, ref, sp, leaf, spec, vc); break;
-        default: walk<-1, STACK, BLOCK, SPEC, COUNT>(m, r, tbf, stk, spill, ref, sp, leaf, spec, vc); break;
+        case 0: walk<0, STACK, BLOCK, SPEC, COUNT, LDS>(m, r, tbf, stk, spill, ref, sp, leaf, spec, vc); break;
+        case 1: walk<1, STACK, BLOCK, SPEC, COUNT, LDS>(m, r, tbf, stk, spill, ref, sp, leaf, spec, vc); break;
+        case 2: walk<2, STACK, BLOCK, SPEC, COUNT, LDS>(m, r, tbf, stk, spill, ref, sp, leaf, spec, vc); break;
+        case 3: walk<3, STACK, BLOCK, SPEC, COUNT, LDS>(m, r, tbf, stk, spill, ref, sp, leaf, spec, vc); break;
+        case 4: walk<4, STACK, BLOCK, SPEC, COUNT, LDS>(m, r, tbf, stk, spill, ref, sp, leaf, spec, vc); break;
+        case 5: walk<5, STACK, BLOCK, SPEC, COUNT, LDS>(m, r, tbf, stk, spill, ref, sp, leaf, spec, vc); break;
+        case 6: walk<6, STACK, BLOCK, SPEC, COUNT, LDS>(m, r, tbf, stk, spill, ref, sp, leaf, spec, vc); break;
+        case 7: walk<7, STACK, BLOCK, SPEC, COUNT, LDS>(m, r, tbf, stk, spill, ref, sp, leaf, spec, vc); break;
+        default: walk<-1, STACK, BLOCK, SPEC, COUNT, LDS>(m, r, tbf, stk, spill, ref, sp, leaf, spec, vc); break;
       }
     }
     if constexpr (SPEC == kSpecOff) {
@@ -499,7 +499,7 @@ __device__ __forceinline__ void trace(const MapData& m, const V3& o, const V3& d
       if constexpr (COUNT) { vc->leaves++; vc->tris += (~static_cast<uint32_t>(ref) & 7u) + 1u; }
       test_leaf(m, tr, ref, h);
       tbf = __double2float_ru(h.t);
-      ref = pop_ref<STACK, BLOCK>(stk, spill, sp);
+      ref = pop_ref<STACK, BLOCK, LDS>(stk, spill, sp);
       if (ref == kDone) break;
       continue;
     }
@@ -507,7 +507,7 @@ __device__ __forceinline__ void trace(const MapData& m, const V3& o, const V3& d
     // subtree, test the held leaf, then a leaf the walk stopped at after it
     if (ref < 0 && leaf == 0) {
       leaf = ref;
-      ref = pop_ref<STACK, BLOCK>(stk, spill, sp);
+      ref = pop_ref<STACK, BLOCK, LDS>(stk, spill, sp);
     }
     while (leaf < 0) {
       if constexpr (COUNT) { vc->leaves++; vc->tris += (~static_cast<uint32_t>(leaf) & 7u) + 1u; }
@@ -515,7 +515,7 @@ __device__ __forceinline__ void trace(const MapData& m, const V3& o, const V3& d
       tbf = __double2float_ru(h.t);
       if (ref < 0) {
         leaf = ref;
-        ref = pop_ref<STACK, BLOCK>(stk, spill, sp);
+        ref = pop_ref<STACK, BLOCK, LDS>(stk, spill, sp);
       } else {
         leaf = 0;
       }

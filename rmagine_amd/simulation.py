@@ -244,6 +244,14 @@ class _SensorHandle:
         _lib.call("rmh_sensor_create", kind, ctypes.byref(cmodel), ctypes.byref(self.handle))
         self.W, self.H = model.getWidth(), model.getHeight()
 
+    def set_ray_order(self, sorted_: bool) -> None:
+        _lib.call("rmh_sensor_set_ray_order", self.handle, 1 if sorted_ else 0)
+
+    def ray_order_sorted(self) -> bool:
+        v = ctypes.c_int()
+        _lib.call("rmh_sensor_get_ray_order", self.handle, ctypes.byref(v))
+        return bool(v.value)
+
     def __del__(self):
         h = getattr(self, "handle", None)
         if h is not None and h.value:
@@ -282,6 +290,16 @@ class SimulatorHip:
     @property
     def model(self):
         return None if self._sensor is None else self._sensor.model
+
+    def set_ray_order(self, sorted_: bool = True) -> None:
+        """O1Dn/OnDn: cast the rays in direction-sorted order (default) or in list order.
+        The results are identical; only the speed differs.  Image models ignore it."""
+        if self._sensor is None:
+            raise RuntimeError("set_ray_order: no sensor model set")
+        self._sensor.set_ray_order(sorted_)
+
+    def ray_order_sorted(self) -> bool:
+        return self._sensor is not None and self._sensor.ray_order_sorted()
 
     def simulate(self, Tbm, bundle: Bundle | None = None, attrs=IntAttrAll, stream=None) -> Bundle:
         """Casts every ray of the model from every pose in Tbm ((N,8) float32 tensor).
@@ -384,7 +402,7 @@ class BatchSimulatorHip:
     """
 
     def __init__(self, map: HipMap, entries, chunk_rays: int = 0, waves_per_cu: int = 0, tile_rows: int = 0,
-                 device="cuda"):
+                 device="cuda", lds_stack: int = 0):
         self._map = map
         self.device = torch.device(device)
         self._sensors, self._keep, self.bundles = [], [], []
@@ -408,7 +426,7 @@ class BatchSimulatorHip:
             self._keep.append(Tbm)
             self.bundles.append(b)
             self.total_rays += sh.W * sh.H * n
-        prm = T.rmh_batch_params(chunk_rays, waves_per_cu, tile_rows)
+        prm = T.rmh_batch_params(chunk_rays, waves_per_cu, tile_rows, lds_stack)
         self._h = ctypes.c_void_p()
         _lib.call("rmh_batch_create", map.handle, table, len(entries), ctypes.byref(prm), ctypes.byref(self._h))
 
@@ -416,11 +434,22 @@ class BatchSimulatorHip:
     def map(self) -> HipMap:
         return self._map
 
-    def set_params(self, chunk_rays: int = 0, waves_per_cu: int = 0, tile_rows: int = 0) -> None:
+    def set_params(self, chunk_rays: int = 0, waves_per_cu: int = 0, tile_rows: int = 0,
+                   lds_stack: int = 0) -> None:
         """Launch shape: rays per work grab, persistent waves per CU, image rows per 64-ray
-        tile (0 keeps the current value)."""
-        prm = T.rmh_batch_params(chunk_rays, waves_per_cu, tile_rows)
+        tile, traversal-stack entries held in LDS (8, 16 or 24; deeper entries spill to
+        private memory) -- 0 keeps the current value."""
+        prm = T.rmh_batch_params(chunk_rays, waves_per_cu, tile_rows, lds_stack)
         _lib.call("rmh_batch_set_params", self._h, ctypes.byref(prm))
+
+    def set_ray_order(self, sorted_: bool) -> None:
+        """Ray order of every O1Dn/OnDn entry (direction-sorted or list order); results are
+        identical either way.  Not while a launch of this batch is in flight."""
+        for sh in self._sensors:
+            sh.set_ray_order(sorted_)
+
+    def ray_order_sorted(self) -> bool:
+        return any(sh.ray_order_sorted() for sh in self._sensors)
 
     def params(self) -> dict:
         prm = T.rmh_batch_params()
@@ -428,7 +457,7 @@ class BatchSimulatorHip:
         chunks = ctypes.c_uint32()
         _lib.call("rmh_batch_get_info", self._h, ctypes.byref(rays), ctypes.byref(chunks), ctypes.byref(prm))
         return {"chunk_rays": prm.chunk_rays, "waves_per_cu": prm.waves_per_cu, "tile_rows": prm.tile_rows,
-                "total_rays": rays.value, "total_chunks": chunks.value}
+                "lds_stack": prm.lds_stack, "total_rays": rays.value, "total_chunks": chunks.value}
 
     def simulate(self, stream=None) -> list:
         _lib.call("rmh_batch_simulate", self._h, stream_handle(stream))

@@ -1,5 +1,5 @@
-"""Batch autotuner (rmagine_amd.autotune): it applies the fastest launch shape and node
-layout it timed, records them, re-applies them from its cache, and never changes results
+"""Batch autotuner (rmagine_amd.autotune): it applies the fastest launch shape, LDS stack
+depth, ray order and node layout it timed, records them, re-applies them from its cache, and never changes results
 (every output byte equal before and after tuning)."""
 import json
 
@@ -41,18 +41,46 @@ def test_tune_batch_shape_and_layout(tmp_path, monkeypatch):
     assert (p["chunk_rays"], p["waves_per_cu"], p["tile_rows"]) == (rec["chunk_rays"], rec["waves_per_cu"], rec["tile_rows"])
     assert hm.wide == rec["wide"]
     assert len(rec["trials"]) == 2 * 2 + 1
+    assert rec["lds_stack"] in (8, 16, 24) and set(rec["lds_stack_ms"]) == {8, 16, 24}
+    assert p["lds_stack"] == rec["lds_stack"]
+    assert set(rec["ray_order_ms"]) == {"sorted", "list"}
+    assert batch.ray_order_sorted() == rec["rays_sorted"]
     after = _outputs(batch)
     for a, b in zip(before, after):
         for k in a:
             np.testing.assert_array_equal(a[k].view(np.uint8), b[k].view(np.uint8), err_msg=k)
     stored = json.loads((tmp_path / "tune.json").read_text())["room-test"]
     assert stored["wide"] == rec["wide"] and stored["chunk_rays"] == rec["chunk_rays"]
+    assert stored["lds_stack"] == rec["lds_stack"] and stored["rays_sorted"] == rec["rays_sorted"]
     # a fresh batch on a map in the other layout picks the cached choice up
     hm.set_layout(not rec["wide"])
-    batch.set_params(chunk_rays=1024, waves_per_cu=32, tile_rows=8)
+    batch.set_params(chunk_rays=1024, waves_per_cu=32, tile_rows=8, lds_stack=24 if rec["lds_stack"] != 24 else 8)
+    batch.set_ray_order(not rec["rays_sorted"])
     again = autotune.tune_batch(batch, key="room-test", layouts=(False, True))
     assert again["cached"] and hm.wide == rec["wide"]
     assert batch.params()["chunk_rays"] == rec["chunk_rays"] and batch.params()["tile_rows"] == rec["tile_rows"]
+    assert batch.params()["lds_stack"] == rec["lds_stack"] and batch.ray_order_sorted() == rec["rays_sorted"]
     for a, b in zip(before, _outputs(batch)):
         for k in a:
             np.testing.assert_array_equal(a[k].view(np.uint8), b[k].view(np.uint8), err_msg=k)
+
+
+def test_batch_lds_stack_depths_and_validation():
+    """Every LDS stack depth gives the same bytes; unsupported depths are refused."""
+    V, F, P = scenes.room_workload(n_sensors=8)
+    hm = HipMap(V, F)
+    entries = [{"model": scenes.vlp16_workload_model(), "Tbm": torch.as_tensor(P, device="cuda"),
+                "attrs": T.ATTRIBUTES}]
+    batch = BatchSimulatorHip(hm, entries, lds_stack=8)
+    assert batch.params()["lds_stack"] == 8
+    ref = _outputs(batch)
+    for d in (16, 24):
+        batch.set_params(lds_stack=d)
+        assert batch.params()["lds_stack"] == d
+        for a, b in zip(ref, _outputs(batch)):
+            for k in a:
+                np.testing.assert_array_equal(a[k].view(np.uint8), b[k].view(np.uint8), err_msg=f"{k} lds={d}")
+    for bad in (4, 12, 32):
+        with pytest.raises(Exception):
+            batch.set_params(lds_stack=bad)
+        assert batch.params()["lds_stack"] == 24

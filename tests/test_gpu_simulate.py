@@ -143,6 +143,35 @@ def test_ray_list_order_independence(room):
         compare(a, o, "ray list")
 
 
+def test_ray_order_switch(room):
+    """The traversal order of O1Dn/OnDn rays (direction-sorted, or list order) changes no
+    output byte; switching back and forth restores it; image models report list order."""
+    V, F, Tbm, hm, om = room
+    rng = np.random.default_rng(12)
+    n = 2113
+    dirs = rng.normal(size=(n, 3))
+    dirs = (dirs / np.linalg.norm(dirs, axis=1, keepdims=True)).astype(np.float32)
+    origs = rng.uniform(-0.3, 0.3, size=(n, 3)).astype(np.float32)
+    for m in (T.O1DnModel(n, 1, T.Interval(0.1, 60.0), (0.05, -0.02, 0.1), dirs),
+              T.OnDnModel(n, 1, T.Interval(0.1, 60.0), origs, dirs)):
+        sim = simulator_for(m, hm)
+        assert sim.ray_order_sorted()
+        P = torch.as_tensor(Tbm[:5], device="cuda")
+        ref = sim.simulate(P, attrs=T.ATTRIBUTES).to_numpy()
+        for sorted_ in (False, True, False):
+            sim.set_ray_order(sorted_)
+            assert sim.ray_order_sorted() == sorted_
+            got = sim.simulate(P, attrs=T.ATTRIBUTES).to_numpy()
+            torch.cuda.synchronize()
+            for k in ref:
+                np.testing.assert_array_equal(got[k].view(np.uint8), ref[k].view(np.uint8),
+                                              err_msg=f"{type(m).__name__} {k} sorted={sorted_}")
+        compare(got, oracle.simulate(om, m, Tbm[:5]), f"list order/{type(m).__name__}")
+    sim = simulator_for(T.PinholeModel(32, 24, T.Interval(0.05, 80.0), (20.0, 20.0), (15.5, 11.5)), hm)
+    sim.set_ray_order(True)
+    assert not sim.ray_order_sorted()
+
+
 def test_attribute_subsets(room):
     V, F, Tbm, hm, om = room
     m = scenes.vlp16_workload_model()
