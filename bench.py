@@ -177,7 +177,9 @@ def extra_batched(cpu_seconds: float, do_cpu: bool) -> dict:
     ms = autotune.time_batch(batch, reps=5)
     layout = "4-wide" if tuned.get("wide") else "binary"
     log(f"batched: {rays} rays, {ms:.2f} ms -> {rays / ms / 1e3:.1f} Mrays/s (tuned {tuned['chunk_rays']}/"
-        f"{tuned['waves_per_cu']}/{tuned['tile_rows']}, {layout} nodes; layouts {tuned.get('layout_ms')})")
+        f"{tuned['waves_per_cu']}/{tuned['tile_rows']}, LDS stack {tuned['lds_stack']} {tuned['lds_stack_ms']}, "
+        f"rays {'sorted' if tuned['rays_sorted'] else 'list order'} {tuned['ray_order_ms']}, {layout} nodes; "
+        f"layouts {tuned.get('layout_ms')})")
     # the same sensors with one rmh_simulate launch each
     sims = [simulator_for(m, hm) for m, _ in sensors]
     bundles = [None] * len(sims)
@@ -199,7 +201,9 @@ def extra_batched(cpu_seconds: float, do_cpu: bool) -> dict:
                        f"(800 VLP-16, 600 pinhole 640x480 fov 90, 600 O1Dn 1k-50k rays), hits+ranges",
            "rays": rays, "batched_ms": round(ms, 3), "batched_mrays_s": round(rays / ms / 1e3, 1),
            "per_sensor_launches_ms": round(plain_ms, 3), "per_sensor_mrays_s": round(rays / plain_ms / 1e3, 1),
-           "tuned": {**{k: tuned[k] for k in ("chunk_rays", "waves_per_cu", "tile_rows")},
+           "tuned": {**{k: tuned[k] for k in ("chunk_rays", "waves_per_cu", "tile_rows", "lds_stack")},
+                     "lds_stack_ms": {str(k): round(v, 3) for k, v in tuned["lds_stack_ms"].items()},
+                     "ray_order": "sorted" if tuned["rays_sorted"] else "list", "ray_order_ms": tuned["ray_order_ms"],
                      "layout": "4-wide" if tuned.get("wide") else "binary", "layout_ms": tuned.get("layout_ms")},
            "batched_vs_per_sensor_mismatches": mism}
     if do_cpu:
