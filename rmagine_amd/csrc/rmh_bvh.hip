@@ -255,6 +255,74 @@ __global__ __launch_bounds__(kBlock) void k_layout(int n, const uint32_t* __rest
   }
 }
 
+// Step 8: node order.  Interior indices of the radix tree follow split positions, and the
+// subtrees collapsed into leaves leave about half of them unused, so the nodes a ray visits
+// one after another lie far apart.  The final tree is rewritten in depth-first preorder
+// (a node, its first child's subtree, its second child's subtree): a descent through first
+// children reads consecutive 64-B nodes, every subtree is one contiguous range, and the
+// array holds only reachable nodes.
+// 8a: interior nodes per reachable subtree, bottom-up; a node is finished by the arrival
+// of its last interior child (the acquire-release atomic publishes the children's sizes).
+__global__ __launch_bounds__(kBlock) void k_subtree_sizes(uint32_t ni, const uint8_t* __restrict__ reach,
+                                                          const int* __restrict__ parent, const Node2* __restrict__ nodes,
+                                                          uint32_t* size, uint32_t* arrive, uint32_t* err) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= ni || !reach[i]) return;
+  const int4 d0 = nodes[i].d;
+  if (d0.x >= 0 || d0.y >= 0) return;  // a node with interior children is finished by them
+  size[i] = 1u;
+  int j = static_cast<int>(i);
+  for (int steps = 0; j != 0; steps++) {
+    if (steps >= kMaxTreeWalk) { atomicOr(err, 1u); return; }
+    const int p = parent[j];
+    const int4 d = nodes[p].d;
+    const uint32_t need = (d.x >= 0 ? 1u : 0u) + (d.y >= 0 ? 1u : 0u);
+    const uint32_t old = __hip_atomic_fetch_add(&arrive[p], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    if (old + 1u != need) return;  // the other interior child finishes p
+    size[p] = 1u + (d.x >= 0 ? size[d.x] : 0u) + (d.y >= 0 ? size[d.y] : 0u);
+    j = p;
+  }
+}
+
+// 8b: preorder position = sum over the path from the root of 1 per step, plus the size
+// of the first child's subtree wherever the path takes the second child.
+__global__ __launch_bounds__(kBlock) void k_preorder_rank(uint32_t ni, const uint8_t* __restrict__ reach,
+                                                          const int* __restrict__ parent, const Node2* __restrict__ nodes,
+                                                          const uint32_t* __restrict__ size, uint32_t* __restrict__ rank,
+                                                          uint32_t* err) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= ni || !reach[i]) return;
+  uint32_t r = 0;
+  int j = static_cast<int>(i);
+  for (int steps = 0; j != 0; steps++) {
+    if (steps >= kMaxTreeWalk) { atomicOr(err, 1u); return; }
+    const int p = parent[j];
+    const int first = nodes[p].d.x;
+    r += 1u + ((first != j && first >= 0) ? size[first] : 0u);
+    j = p;
+  }
+  rank[i] = r;
+}
+
+// 8c: move every reachable node to its preorder slot, with its child references, parent
+// link and reach flag renumbered.
+__global__ __launch_bounds__(kBlock) void k_preorder_scatter(uint32_t ni, uint32_t n_out, const uint8_t* __restrict__ reach,
+                                                             const int* __restrict__ parent, const Node2* __restrict__ nodes,
+                                                             const uint32_t* __restrict__ rank, Node2* __restrict__ out,
+                                                             int* __restrict__ out_parent, uint8_t* __restrict__ out_reach,
+                                                             uint32_t* err) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= ni || !reach[i]) return;
+  const uint32_t s = rank[i];
+  if (s >= n_out) { atomicOr(err, 1u); return; }
+  Node2 nd = nodes[i];
+  if (nd.d.x >= 0) nd.d.x = static_cast<int>(rank[nd.d.x]);
+  if (nd.d.y >= 0) nd.d.y = static_cast<int>(rank[nd.d.y]);
+  out[s] = nd;
+  out_parent[s] = i == 0 ? -1 : static_cast<int>(rank[parent[i]]);
+  out_reach[s] = 1;
+}
+
 // Step 7.
 __global__ __launch_bounds__(kBlock) void k_gather(uint32_t nf, const uint32_t* __restrict__ vals,
                                                    const float* __restrict__ V, const uint32_t* __restrict__ F,
@@ -722,6 +790,60 @@ hipError_t build_wide(rmh_map* m, hipStream_t stream) {
 
 }  // namespace
 
+// Rewrites m->nodes / m->parent / m->reach (ni radix-tree slots, n_reach of them reachable,
+// root at slot 0) into preorder, in buffers of exactly n_reach entries.
+static hipError_t preorder_nodes(rmh_map* m, uint32_t ni, uint32_t n_reach, hipStream_t stream) {
+  DevBuf size, rank, arrive, err;
+  Node2* nodes = nullptr;
+  int* parent = nullptr;
+  uint8_t* reach = nullptr;
+  hipError_t e = hipMalloc(&size.p, sizeof(uint32_t) * ni);
+  if (e == hipSuccess) e = hipMalloc(&rank.p, sizeof(uint32_t) * ni);
+  if (e == hipSuccess) e = hipMalloc(&arrive.p, sizeof(uint32_t) * ni);
+  if (e == hipSuccess) e = hipMalloc(&err.p, sizeof(uint32_t));
+  if (e == hipSuccess) e = hipMalloc(&nodes, sizeof(Node2) * n_reach);
+  if (e == hipSuccess) e = hipMalloc(&parent, sizeof(int) * n_reach);
+  if (e == hipSuccess) e = hipMalloc(&reach, n_reach);
+  if (e == hipSuccess) e = hipMemsetAsync(arrive.p, 0, sizeof(uint32_t) * ni, stream);
+  if (e == hipSuccess) e = hipMemsetAsync(err.p, 0, sizeof(uint32_t), stream);
+  if (e == hipSuccess) {
+    k_subtree_sizes<<<blocks_for(ni), kBlock, 0, stream>>>(ni, m->reach, m->parent, m->nodes, size.as<uint32_t>(),
+                                                           arrive.as<uint32_t>(), err.as<uint32_t>());
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) {
+    k_preorder_rank<<<blocks_for(ni), kBlock, 0, stream>>>(ni, m->reach, m->parent, m->nodes, size.as<uint32_t>(),
+                                                           rank.as<uint32_t>(), err.as<uint32_t>());
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) {
+    k_preorder_scatter<<<blocks_for(ni), kBlock, 0, stream>>>(ni, n_reach, m->reach, m->parent, m->nodes,
+                                                              rank.as<uint32_t>(), nodes, parent, reach, err.as<uint32_t>());
+    e = hipGetLastError();
+  }
+  uint32_t herr = 0, root_size = 0;
+  if (e == hipSuccess) e = hipMemcpyAsync(&herr, err.p, sizeof(uint32_t), hipMemcpyDeviceToHost, stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(&root_size, size.p, sizeof(uint32_t), hipMemcpyDeviceToHost, stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(stream);
+  // a walk that did not end at the root, or a count that differs from the reachable
+  // nodes, means the links do not form one tree: the build fails
+  if (e == hipSuccess && (herr != 0 || root_size != n_reach)) e = hipErrorInvalidValue;
+  if (e != hipSuccess) {
+    if (nodes) (void)hipFree(nodes);
+    if (parent) (void)hipFree(parent);
+    if (reach) (void)hipFree(reach);
+    return e;
+  }
+  (void)hipFree(m->nodes);
+  (void)hipFree(m->parent);
+  (void)hipFree(m->reach);
+  m->nodes = nodes;
+  m->parent = parent;
+  m->reach = reach;
+  m->n_internal = n_reach;
+  return hipSuccess;
+}
+
 extern "C" int rmh_map_build(const float* vertices, uint32_t nv, const uint32_t* faces, uint32_t nf,
                              const uint32_t* mesh_face_offsets, uint32_t n_meshes,
                              const rmh_bvh_params* params, void* stream_, rmh_map** out) {
@@ -927,6 +1049,17 @@ extern "C" int rmh_map_build(const float* vertices, uint32_t nv, const uint32_t*
     const float area = dx * dy + dy * dz + dz * dx;
     m->info.sah_cost = (n >= 2 && area > 0.0f) ? root_lo.w / area : 0.0f;
   }
+  if (m->root == 0 && hstats[0] > 0) {
+    BUILD_TRY(hipEventRecord(ev0, stream));
+    BUILD_TRY(preorder_nodes(m, ni, hstats[0], stream));
+    BUILD_TRY(hipEventRecord(ev1, stream));
+    BUILD_TRY(hipStreamSynchronize(stream));
+    float order_ms = 0.0f;
+    BUILD_TRY(hipEventElapsedTime(&order_ms, ev0, ev1));
+    ms += order_ms;
+  } else {
+    m->n_internal = 0;
+  }
   m->root4 = m->root;
   if (prm.flags & RMH_BVH_WIDE) {
     BUILD_TRY(hipEventRecord(ev0, stream));
@@ -938,7 +1071,7 @@ extern "C" int rmh_map_build(const float* vertices, uint32_t nv, const uint32_t*
     ms += wide_ms;
   }
   m->info.build_ms = ms;
-  m->info.device_bytes = sizeof(float4) * 4 * static_cast<uint64_t>(nf) + sizeof(Node2) * static_cast<uint64_t>(ni + 1) +
+  m->info.device_bytes = sizeof(float4) * 4 * static_cast<uint64_t>(nf) + sizeof(Node2) * static_cast<uint64_t>(m->n_internal) +
                          sizeof(Node4) * static_cast<uint64_t>(m->info.n_wide_nodes);
   *out = m;
   return RMH_OK;

@@ -33,6 +33,10 @@ def test_refined_equals_radix_tree(scene):
     a, b = HipMap(V, F), HipMap(V, F, refine=False)
     ia, ib = a.info(), b.info()
     assert ia["sah_cost"] < ib["sah_cost"], (ia, ib)
+    # both trees are stored compactly in preorder: the node array holds exactly the
+    # reachable interior nodes (64 B each) beside 4 x 16 B of triangle data per face
+    for i in (ia, ib):
+        assert i["device_bytes"] == 64 * len(F) + 64 * i["n_nodes"], i
     ga, gb = _sim(a, m, P), _sim(b, m, P)
     for k in T.ATTRIBUTES:
         np.testing.assert_array_equal(ga[k].view(np.uint8), gb[k].view(np.uint8), err_msg=k)
