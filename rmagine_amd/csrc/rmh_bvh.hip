@@ -844,9 +844,31 @@ static hipError_t preorder_nodes(rmh_map* m, uint32_t ni, uint32_t n_reach, hipS
   return hipSuccess;
 }
 
+static int map_build(const float* vertices, uint32_t nv, const uint32_t* faces, uint32_t nf,
+                     const uint32_t* mesh_face_offsets, uint32_t n_meshes, const rmh_bvh_params* params,
+                     void* stream_, rmh_map** out);
+
+// The ray kernels hold one stack entry per tree level, at most 128.  The radix tree is at
+// most 95 levels deep (63 key bits, then at most 32 levels of index bits among equal keys),
+// but treelet restructuring may lengthen paths; a refined tree deeper than 128 levels is
+// replaced by the unrefined one, so every map the build accepts can be traced.
 extern "C" int rmh_map_build(const float* vertices, uint32_t nv, const uint32_t* faces, uint32_t nf,
                              const uint32_t* mesh_face_offsets, uint32_t n_meshes,
                              const rmh_bvh_params* params, void* stream_, rmh_map** out) {
+  const int rc = map_build(vertices, nv, faces, nf, mesh_face_offsets, n_meshes, params, stream_, out);
+  if (rc != RMH_OK || (*out)->info.max_depth <= kMaxBinaryDepth) return rc;
+  if (params && (params->flags & RMH_BVH_NO_REFINE)) return rc;  // not reached, by the bound above
+  map_free(*out);
+  *out = nullptr;
+  rmh_bvh_params prm{4u, 1.0f, 1.5f, 0u};
+  if (params) prm = *params;
+  prm.flags |= RMH_BVH_NO_REFINE;
+  return map_build(vertices, nv, faces, nf, mesh_face_offsets, n_meshes, &prm, stream_, out);
+}
+
+static int map_build(const float* vertices, uint32_t nv, const uint32_t* faces, uint32_t nf,
+                     const uint32_t* mesh_face_offsets, uint32_t n_meshes, const rmh_bvh_params* params,
+                     void* stream_, rmh_map** out) {
   if (!out) return fail(RMH_ERR_INVALID, "rmh_map_build: out is NULL");
   *out = nullptr;
   if (nf > 0 && (!vertices || !faces || nv == 0))

@@ -114,6 +114,8 @@ struct alignas(64) Node4 {
 };
 
 constexpr int kMaxLeaf = 8;
+// deepest binary tree the ray kernels trace (one traversal-stack entry per level)
+constexpr uint32_t kMaxBinaryDepth = 128;
 
 RMH_HD int make_leaf_ref(uint32_t first, uint32_t count) {
   return ~static_cast<int>((first << 3) | (count - 1u));

@@ -69,7 +69,7 @@ inline TravSel trav_select(const rmh_map* m) {
   const uint32_t need = m->nodes4 ? 3u * m->info.wide_levels : m->info.max_depth;
   TravSel t;
   t.wide = m->nodes4 != nullptr;
-  t.stack = need <= 32 ? 32 : (need <= 64 ? 64 : (need <= 128 ? 128 : -1));
+  t.stack = need <= 32 ? 32 : (need <= 64 ? 64 : (need <= kMaxBinaryDepth ? 128 : -1));
   return t;
 }
 
