@@ -157,7 +157,8 @@ def _events():
 
 def extra_batched(cpu_seconds: float, do_cpu: bool) -> dict:
     """Heterogeneous batch: 2,000 sensors (40 % VLP-16, 30 % pinhole 640x480, 30 % O1Dn with
-    1k-50k rays) on a 500k-triangle scene, hits + ranges, one persistent launch, tuned."""
+    1k-50k rays) on a 500k-triangle scene, hits + ranges, one batch (two persistent launches:
+    image sensors, then ray lists), tuned."""
     import torch
 
     from rmagine_amd import autotune, scenes

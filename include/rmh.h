@@ -252,8 +252,9 @@ int rmh_simulate_visit_stats(const rmh_map* map, const rmh_sensor* sensor, const
  * No counterpart in the reference (it launches one sensor model per optixLaunch,
  * src/rmagine_optix/src/simulation/SphereSimulatorOptix.cpp:64-81; the closest piece is the
  * model union of src/rmagine_optix/include/rmagine/simulation/optix/sim_program_data.h:32-44).
- * A batch is a device table of sensors (any mix of models, poses and ray counts) cast in ONE
- * persistent launch; every entry writes its own rmh_outputs exactly as rmh_simulate would. */
+ * A batch is a device table of sensors (any mix of models, poses and ray counts) cast by two
+ * persistent launches (image sensors, then O1Dn/OnDn ray lists, which walk the tree
+ * speculatively); every entry writes its own rmh_outputs exactly as rmh_simulate would. */
 typedef struct {
   const rmh_sensor*    sensor;
   rmh_transform        Tsb;        /* sensor -> base (host value)               */
@@ -282,8 +283,9 @@ int rmh_batch_create(const rmh_map* map, const rmh_batch_entry* entries, uint32_
 int rmh_batch_set_params(rmh_batch* batch, const rmh_batch_params* params);
 int rmh_batch_get_info(const rmh_batch* batch, uint64_t* total_rays, uint32_t* total_chunks,
                        rmh_batch_params* params);
-/* One counter-reset kernel + one persistent kernel on `stream`; capturable into a graph.
- * Launches of one batch must not overlap (they share its work counter). */
+/* Per group present (image sensors, ray lists): one counter-reset kernel + one persistent
+ * kernel, in stream order on `stream`; capturable into a graph.  Launches of one batch must
+ * not overlap (they share its work counters). */
 int rmh_batch_simulate(rmh_batch* batch, void* stream);
 int rmh_batch_destroy(rmh_batch* batch);
 

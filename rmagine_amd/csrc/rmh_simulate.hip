@@ -21,11 +21,12 @@
 //
 // Two launch forms:
 //   rmh_simulate        one sensor model, n poses, one workgroup per (pose, tile);
-//   rmh_batch_simulate  many sensors (any mix of models, poses and ray counts) in ONE
-//                       persistent launch: waves grab chunks of tiles of one sensor from
-//                       an atomic counter until the work is exhausted.  All pointers and
-//                       sizes live in a device table built once, so the launch (counter
-//                       reset kernel + persistent kernel) can be captured into a graph.
+//   rmh_batch_simulate  many sensors (any mix of models, poses and ray counts) in two
+//                       persistent launches, image sensors then ray lists (their walks
+//                       differ): waves grab chunks of tiles of one sensor from an atomic
+//                       counter until the work is exhausted.  All pointers and sizes live
+//                       in a device table built once, so the launches (counter reset
+//                       kernel + persistent kernel, per group) can be captured into a graph.
 // The reference has no batched form; its closest piece is the per-launch model union
 // (src/rmagine_optix/include/rmagine/simulation/optix/sim_program_data.h:32-44).
 #include "rmh_trace.hpp"
@@ -148,7 +149,7 @@ __global__ __launch_bounds__(kWave, 8) void k_visit_stats(const VisitArgs a) {
     const Tf Tsm = tmul(load_tf(a.Tbm[pl]), a.Tsb);
     Hit h;
     trace<STACK, kWave, false, spec_mode<KIND>(), true>(a.map, map_origin<KIND>(Tsm, os), qrot(Tsm.R, ds),
-                                                       a.g.range_max, stk, h, false, &vc);
+                                                       a.g.range_max, stk, h, &vc);
   }
   uint32_t v[6] = {vc.vec_nodes, vc.uni_nodes, vc.leaves, vc.tris, vc.wave_iters, vc.wave_uni_iters};
   for (int k = 0; k < 6; k++)
@@ -181,12 +182,14 @@ struct BatchEntryDev {
   uint64_t n_slots;  // n_poses * tiles * 64 ray slots, enumerated tile by tile
 };
 
+// One persistent launch covers the entries [entry_lo, entry_hi), whose chunks are
+// [chunk_prefix[entry_lo], chunk_prefix[entry_hi]).
 struct BatchArgs {
   MapData map;
   const BatchEntryDev* entries;
   const uint32_t* chunk_prefix;  // n_entries + 1
-  uint32_t n_entries;
-  uint32_t total_chunks;
+  uint32_t entry_lo, entry_hi;
+  uint32_t chunk_lo, chunk_hi;
   uint32_t chunk_rays;           // multiple of 64
   uint32_t* counter;
 };
@@ -200,61 +203,54 @@ __device__ __forceinline__ void batch_gen(const BatchEntryDev& e, uint32_t vid, 
   om = map_origin<KIND>(Tsm, os);
 }
 
-// One batch slot: ray generation and attribute writing depend on the sensor kind, the
-// traversal does not, so it is instantiated once (one copy of the octant-specialised
-// walks instead of one per kind keeps the kernel inside the instruction cache).  As in
+// One batch slot of kind K0 or K1 (the two image models or the two ray-list models, which
+// share the walk of their launch).  Ray generation and attribute writing depend on the
+// kind, the traversal does not, so it is instantiated once per launch (one copy of the
+// octant-specialised walks keeps the kernel inside the instruction cache).  As in
 // k_simulate, the ray set-up is recomputed for the writers rather than held (spilled)
 // through the traversal.
-template <int STACK, int BLOCK, bool WIDE, int LDS>
-__device__ __forceinline__ void batch_ray(const BatchArgs& a, const BatchEntryDev& e, int kind, uint32_t pose,
+template <int STACK, int BLOCK, bool WIDE, int LDS, int SPEC, int K0, int K1>
+__device__ __forceinline__ void batch_ray(const BatchArgs& a, const BatchEntryDev& e, bool k0, uint32_t pose,
                                           uint32_t vid, uint32_t hid, int (*stk)[BLOCK]) {
   Hit h;
   {
     const Tf Tsm = tmul(load_tf(e.Tbm[pose]), e.Tsb);
     uint32_t lid;
     V3 ds, os, dm, om;
-    switch (kind) {
-      case RMH_MODEL_SPHERICAL: batch_gen<RMH_MODEL_SPHERICAL>(e, vid, hid, lid, Tsm, ds, os, dm, om); break;
-      case RMH_MODEL_PINHOLE: batch_gen<RMH_MODEL_PINHOLE>(e, vid, hid, lid, Tsm, ds, os, dm, om); break;
-      case RMH_MODEL_O1DN: batch_gen<RMH_MODEL_O1DN>(e, vid, hid, lid, Tsm, ds, os, dm, om); break;
-      default: batch_gen<RMH_MODEL_ONDN>(e, vid, hid, lid, Tsm, ds, os, dm, om); break;
-    }
-    trace<STACK, BLOCK, WIDE, kSpecRuntime, false, LDS>(a.map, om, dm, e.g.range_max, stk, h, speculative_walk(kind));
+    if (k0) batch_gen<K0>(e, vid, hid, lid, Tsm, ds, os, dm, om);
+    else batch_gen<K1>(e, vid, hid, lid, Tsm, ds, os, dm, om);
+    trace<STACK, BLOCK, WIDE, SPEC, false, LDS>(a.map, om, dm, e.g.range_max, stk, h);
   }
   asm volatile("" : "+v"(vid), "+v"(hid));
   const Tf Tsm = tmul(load_tf(e.Tbm[pose]), e.Tsb);
   uint32_t lid;
   V3 ds, os;
-  switch (kind) {
-    case RMH_MODEL_SPHERICAL: gen_ray<RMH_MODEL_SPHERICAL>(e.g, vid, hid, lid, ds, os); break;
-    case RMH_MODEL_PINHOLE: gen_ray<RMH_MODEL_PINHOLE>(e.g, vid, hid, lid, ds, os); break;
-    case RMH_MODEL_O1DN: gen_ray<RMH_MODEL_O1DN>(e.g, vid, hid, lid, ds, os); break;
-    default: gen_ray<RMH_MODEL_ONDN>(e.g, vid, hid, lid, ds, os); break;
-  }
+  if (k0) gen_ray<K0>(e.g, vid, hid, lid, ds, os);
+  else gen_ray<K1>(e.g, vid, hid, lid, ds, os);
   const uint64_t r = static_cast<uint64_t>(pose) * e.g.ray_count + lid;
-  switch (kind) {
-    case RMH_MODEL_SPHERICAL: write_outputs<RMH_MODEL_SPHERICAL>(e.out, r, e.g, a.map, Tsm, ds, os, h); break;
-    case RMH_MODEL_PINHOLE: write_outputs<RMH_MODEL_PINHOLE>(e.out, r, e.g, a.map, Tsm, ds, os, h); break;
-    case RMH_MODEL_O1DN: write_outputs<RMH_MODEL_O1DN>(e.out, r, e.g, a.map, Tsm, ds, os, h); break;
-    default: write_outputs<RMH_MODEL_ONDN>(e.out, r, e.g, a.map, Tsm, ds, os, h); break;
-  }
+  if (k0) write_outputs<K0>(e.out, r, e.g, a.map, Tsm, ds, os, h);
+  else write_outputs<K1>(e.out, r, e.g, a.map, Tsm, ds, os, h);
 }
 
 __global__ void k_reset_counter(uint32_t* counter) {
   if (threadIdx.x == 0) *counter = 0u;  // a kernel store (not a memset node) keeps graph replays correct
 }
 
-template <int STACK, int BLOCK, bool WIDE, int LDS>
+// Persistent kernel over one group of entries: image sensors (spherical, pinhole) walk the
+// tree classically, ray lists (O1Dn, OnDn) speculatively, as they do in k_simulate.
+template <int STACK, int BLOCK, bool WIDE, int LDS, int SPEC>
 __global__ __launch_bounds__(BLOCK, 8) void k_batch(const BatchArgs a) {
+  constexpr int K0 = SPEC == kSpecOff ? RMH_MODEL_SPHERICAL : RMH_MODEL_O1DN;
+  constexpr int K1 = SPEC == kSpecOff ? RMH_MODEL_PINHOLE : RMH_MODEL_ONDN;
   __shared__ int stk[LDS][BLOCK];
   const int lane = threadIdx.x & 63;
   for (;;) {
     uint32_t c = 0;
     if (lane == 0) c = atomicAdd(a.counter, 1u);
     c = __shfl(c, 0, 64);
-    c = __builtin_amdgcn_readfirstlane(c);
-    if (c >= a.total_chunks) break;  // every wave leaves once the counter passes the end
-    uint32_t lo = 0, hi = a.n_entries;
+    c = __builtin_amdgcn_readfirstlane(c) + a.chunk_lo;
+    if (c >= a.chunk_hi) break;  // every wave leaves once the counter passes the end
+    uint32_t lo = a.entry_lo, hi = a.entry_hi;
     while (hi - lo > 1) {
       const uint32_t mid = (lo + hi) >> 1;
       if (a.chunk_prefix[mid] <= c) lo = mid; else hi = mid;
@@ -262,7 +258,7 @@ __global__ __launch_bounds__(BLOCK, 8) void k_batch(const BatchArgs a) {
     const BatchEntryDev& e = a.entries[lo];
     const uint64_t tile0 = static_cast<uint64_t>(c - a.chunk_prefix[lo]) * (a.chunk_rays / 64);
     const uint64_t n_tiles = static_cast<uint64_t>(e.n_poses) * e.tl.tiles;
-    const int kind = e.g.kind;
+    const bool k0 = e.g.kind == K0;
     for (uint32_t k = 0; k < a.chunk_rays / 64; k++) {
       const uint64_t tg = tile0 + k;  // wave-uniform
       if (tg >= n_tiles) break;
@@ -270,25 +266,38 @@ __global__ __launch_bounds__(BLOCK, 8) void k_batch(const BatchArgs a) {
       const uint32_t tile = static_cast<uint32_t>(tg - static_cast<uint64_t>(pose) * e.tl.tiles);
       uint32_t vid, hid;
       if (!tile_ray(e.tl, e.g.grid_w, e.g.grid_h, tile, lane, vid, hid)) continue;
-      batch_ray<STACK, BLOCK, WIDE, LDS>(a, e, kind, pose, vid, hid, stk);
+      batch_ray<STACK, BLOCK, WIDE, LDS, SPEC, K0, K1>(a, e, k0, pose, vid, hid, stk);
     }
   }
 }
 
 template <int STACK, bool WIDE, int LDS>
-hipError_t launch_batch(const BatchArgs& a, uint32_t blocks, hipStream_t stream) {
+hipError_t launch_batch(const BatchArgs& a, bool lists, uint32_t blocks, hipStream_t stream) {
   k_reset_counter<<<1, 64, 0, stream>>>(a.counter);
-  k_batch<STACK, kWave, WIDE, LDS><<<blocks, kWave, 0, stream>>>(a);
+  if (lists) k_batch<STACK, kWave, WIDE, LDS, kSpecOn><<<blocks, kWave, 0, stream>>>(a);
+  else k_batch<STACK, kWave, WIDE, LDS, kSpecOff><<<blocks, kWave, 0, stream>>>(a);
   return hipGetLastError();
 }
 
 // LDS stack depth of the batched kernel (a launch parameter the autotuner chooses): 8, 16
 // or 24 entries per lane (2, 4 or 6 KiB per wave); deeper entries go to scratch.
 template <int STACK, bool WIDE>
-hipError_t launch_batch_lds(uint32_t lds, const BatchArgs& a, uint32_t blocks, hipStream_t stream) {
-  if (lds == 8) return launch_batch<STACK, WIDE, 8>(a, blocks, stream);
-  if (lds == 24) return launch_batch<STACK, WIDE, 24>(a, blocks, stream);
-  return launch_batch<STACK, WIDE, 16>(a, blocks, stream);
+hipError_t launch_batch_lds(uint32_t lds, const BatchArgs& a, bool lists, uint32_t blocks, hipStream_t stream) {
+  if (lds == 8) return launch_batch<STACK, WIDE, 8>(a, lists, blocks, stream);
+  if (lds == 24) return launch_batch<STACK, WIDE, 24>(a, lists, blocks, stream);
+  return launch_batch<STACK, WIDE, 16>(a, lists, blocks, stream);
+}
+
+hipError_t launch_batch_sel(const TravSel& ts, uint32_t lds, const BatchArgs& a, bool lists, uint32_t blocks,
+                            hipStream_t stream) {
+  if (ts.wide) {
+    if (ts.stack == 32) return launch_batch_lds<32, true>(lds, a, lists, blocks, stream);
+    if (ts.stack == 64) return launch_batch_lds<64, true>(lds, a, lists, blocks, stream);
+    return launch_batch_lds<128, true>(lds, a, lists, blocks, stream);
+  }
+  if (ts.stack == 32) return launch_batch_lds<32, false>(lds, a, lists, blocks, stream);
+  if (ts.stack == 64) return launch_batch_lds<64, false>(lds, a, lists, blocks, stream);
+  return launch_batch_lds<128, false>(lds, a, lists, blocks, stream);
 }
 
 bool valid_lds_stack(uint32_t v) { return v == 8 || v == 16 || v == 24; }
@@ -299,8 +308,10 @@ struct rmh_batch {
   const rmh_map* map = nullptr;
   BatchEntryDev* entries = nullptr;
   uint32_t* chunk_prefix = nullptr;
-  uint32_t* counter = nullptr;
+  uint32_t* counter = nullptr;        // one work counter per launch group
   uint32_t n_entries = 0;
+  uint32_t n_image_entries = 0;       // the table holds the image entries first, then the ray lists
+  uint32_t image_chunks = 0;
   uint32_t total_chunks = 0;
   uint64_t total_rays = 0;
   rmh_batch_params params{};
@@ -344,6 +355,7 @@ int batch_upload(rmh_batch* b, const std::vector<BatchEntryDev>& host) {
   }
   prefix[b->n_entries] = static_cast<uint32_t>(chunks);
   b->total_chunks = static_cast<uint32_t>(chunks);
+  b->image_chunks = prefix[b->n_image_entries];
   RMH_HIP_TRY(hipMemcpy(b->chunk_prefix, prefix.data(), sizeof(uint32_t) * prefix.size(), hipMemcpyHostToDevice));
   if (!host.empty())
     RMH_HIP_TRY(hipMemcpy(b->entries, host.data(), sizeof(BatchEntryDev) * host.size(), hipMemcpyHostToDevice));
@@ -475,9 +487,19 @@ int rmh_batch_create(const rmh_map* map, const rmh_batch_entry* entries, uint32_
   b->host.resize(n_entries);
   b->sensors.resize(n_entries);
   b->entry_rays.resize(n_entries);
+  // device table order: image sensors first, then ray lists (each entry keeps its own
+  // outputs, so the order is not visible to the caller)
+  std::vector<uint32_t> order;
+  order.reserve(n_entries);
+  for (int lists = 0; lists < 2; lists++)
+    for (uint32_t i = 0; i < n_entries; i++) {
+      if (!entries[i].sensor) { batch_free(b); return fail(RMH_ERR_INVALID, "rmh_batch_create: entry without sensor"); }
+      if (speculative_walk(entries[i].sensor->kind) == (lists == 1)) order.push_back(i);
+    }
+  for (uint32_t i = 0; i < n_entries; i++)
+    if (!speculative_walk(entries[i].sensor->kind)) b->n_image_entries++;
   for (uint32_t i = 0; i < n_entries; i++) {
-    const rmh_batch_entry& e = entries[i];
-    if (!e.sensor) { batch_free(b); return fail(RMH_ERR_INVALID, "rmh_batch_create: entry without sensor"); }
+    const rmh_batch_entry& e = entries[order[i]];
     if (e.n_poses > 0 && !e.Tbm_dev) { batch_free(b); return fail(RMH_ERR_INVALID, "rmh_batch_create: entry without poses"); }
     b->sensors[i] = e.sensor;
     b->host[i].g = make_raygen(e.sensor);
@@ -489,7 +511,7 @@ int rmh_batch_create(const rmh_map* map, const rmh_batch_entry* entries, uint32_
   batch_tile(b);
   hipError_t err = hipMalloc(&b->entries, sizeof(BatchEntryDev) * std::max<uint32_t>(1, n_entries));
   if (err == hipSuccess) err = hipMalloc(&b->chunk_prefix, sizeof(uint32_t) * (n_entries + 1));
-  if (err == hipSuccess) err = hipMalloc(&b->counter, sizeof(uint32_t));
+  if (err == hipSuccess) err = hipMalloc(&b->counter, 2 * sizeof(uint32_t));
   if (err != hipSuccess) { batch_free(b); return hip_fail(err, "rmh_batch_create"); }
   if (int rc = batch_upload(b, b->host)) { batch_free(b); return rc; }
   *out = b;
@@ -535,31 +557,27 @@ int rmh_batch_get_info(const rmh_batch* b, uint64_t* total_rays, uint32_t* total
 int rmh_batch_simulate(rmh_batch* b, void* stream_) {
   if (!b) return fail(RMH_ERR_INVALID, "rmh_batch_simulate: NULL batch");
   if (b->total_chunks == 0) return RMH_OK;
-  BatchArgs a;
-  a.map = make_mapdata(b->map);
-  a.entries = b->entries;
-  a.chunk_prefix = b->chunk_prefix;
-  a.n_entries = b->n_entries;
-  a.total_chunks = b->total_chunks;
-  a.chunk_rays = b->params.chunk_rays;
-  a.counter = b->counter;
-  const uint32_t waves = b->n_cu * b->params.waves_per_cu;
-  const uint32_t blocks = std::max<uint32_t>(1, std::min<uint32_t>(waves, b->total_chunks));
   const TravSel ts = trav_select(b->map);
-  hipStream_t stream = static_cast<hipStream_t>(stream_);
-  hipError_t e;
   if (ts.stack < 0) return fail(RMH_ERR_UNSUPPORTED, "rmh_batch_simulate: BVH deeper than 128 levels");
-  const uint32_t lds = b->params.lds_stack;
-  if (ts.wide) {
-    if (ts.stack == 32) e = launch_batch_lds<32, true>(lds, a, blocks, stream);
-    else if (ts.stack == 64) e = launch_batch_lds<64, true>(lds, a, blocks, stream);
-    else e = launch_batch_lds<128, true>(lds, a, blocks, stream);
-  } else {
-    if (ts.stack == 32) e = launch_batch_lds<32, false>(lds, a, blocks, stream);
-    else if (ts.stack == 64) e = launch_batch_lds<64, false>(lds, a, blocks, stream);
-    else e = launch_batch_lds<128, false>(lds, a, blocks, stream);
+  hipStream_t stream = static_cast<hipStream_t>(stream_);
+  const uint32_t waves = b->n_cu * b->params.waves_per_cu;
+  // two persistent launches in stream order: the image entries, then the ray lists
+  for (int lists = 0; lists < 2; lists++) {
+    BatchArgs a;
+    a.map = make_mapdata(b->map);
+    a.entries = b->entries;
+    a.chunk_prefix = b->chunk_prefix;
+    a.entry_lo = lists ? b->n_image_entries : 0u;
+    a.entry_hi = lists ? b->n_entries : b->n_image_entries;
+    a.chunk_lo = lists ? b->image_chunks : 0u;
+    a.chunk_hi = lists ? b->total_chunks : b->image_chunks;
+    a.chunk_rays = b->params.chunk_rays;
+    a.counter = b->counter + lists;
+    if (a.chunk_hi == a.chunk_lo) continue;
+    const uint32_t blocks = std::max<uint32_t>(1, std::min<uint32_t>(waves, a.chunk_hi - a.chunk_lo));
+    const hipError_t e = launch_batch_sel(ts, b->params.lds_stack, a, lists == 1, blocks, stream);
+    if (e != hipSuccess) return hip_fail(e, "rmh_batch_simulate launch");
   }
-  if (e != hipSuccess) return hip_fail(e, "rmh_batch_simulate launch");
   return RMH_OK;
 }
 

@@ -203,11 +203,10 @@ constexpr int kDone = 0x7fffffff;
 
 // Walks interior nodes from `ref` until a leaf is reached (ref < 0) or the stack runs
 // dry (ref = kDone).
-// Speculation modes of trace(): never (classic while-while), always, or per a wave-uniform
-// run-time flag (one kernel for several sensor kinds; it takes the speculative loop form,
-// with the walk's leaf holding switched by the flag: a third, classic form in the same
-// kernel costs registers and made the batched launch 50 % slower).
-constexpr int kSpecOff = 0, kSpecOn = 1, kSpecRuntime = 2;
+// Speculation modes of trace(): never (classic while-while) or always.  It is a template
+// argument: holding both loop forms in one kernel costs registers (a batched kernel that did
+// so was 50 % slower), so the batched launch runs one kernel per mode.
+constexpr int kSpecOff = 0, kSpecOn = 1;
 
 // Per-lane traversal counts of the diagnostic build of trace() (COUNT = true): node
 // visits through the vector and the scalar path, leaves, triangle tests, and (on the
@@ -235,7 +234,7 @@ __device__ __forceinline__ int pop_ref(int (*stk)[BLOCK], volatile int* spill, i
   return spill[sp - LDS];
 }
 
-// spec (wave-uniform): speculative walk (Aila and Laine, "Understanding the Efficiency of
+// SPEC == kSpecOn: speculative walk (Aila and Laine, "Understanding the Efficiency of
 // Ray Traversal on GPUs", HPG 2009): a lane that reaches its first leaf holds it in `leaf`
 // and keeps walking; the wave leaves the loop once a quarter of its walking lanes hold a
 // leaf (on 2,000 x 32k random directions: 25.9 ms when waiting for all lanes, 24.1 for
@@ -245,7 +244,7 @@ __device__ __forceinline__ int pop_ref(int (*stk)[BLOCK], volatile int* spill, i
 // walk without it.
 template <int OCT, int STACK, int BLOCK, int SPEC, bool COUNT, int LDS>
 __device__ __forceinline__ void walk(const MapData& m, const SlabRay& r, float tbf, int (*stk)[BLOCK],
-                                     volatile int* spill, int& ref, int& sp, int& leaf, bool spec, VisitCount* vc) {
+                                     volatile int* spill, int& ref, int& sp, int& leaf, VisitCount* vc) {
   const int tid = threadIdx.x;
   while (ref >= 0 && ref != kDone) {
     float tmin0, tmax0, tmin1, tmax1;
@@ -293,7 +292,7 @@ __device__ __forceinline__ void walk(const MapData& m, const SlabRay& r, float t
     } else {
       ref = pop_ref<STACK, BLOCK, LDS>(stk, spill, sp);
     }
-    if (SPEC == kSpecOn || (SPEC == kSpecRuntime && spec)) {
+    if (SPEC == kSpecOn) {
       if (ref < 0 && leaf == 0) {
         leaf = ref;
         ref = pop_ref<STACK, BLOCK, LDS>(stk, spill, sp);
@@ -430,14 +429,14 @@ __device__ __forceinline__ void walk4(const MapData& m, const SlabRay& r, float 
 // Closest hit along o + t d, 0 <= t <= range_max.  While-while traversal: a lane walks
 // interior nodes (near child first, far child onto its stack) until it reaches a leaf or
 // runs out of work, then the lanes holding leaves test their triangles together; with
-// spec (wave-uniform) the walk is speculative (see walk()).  The stack never overflows:
+// SPEC == kSpecOn the walk is speculative (see walk()).  The stack never overflows:
 // the host picks STACK >= the depth of the BVH.
 //
 // Waves whose 64 rays share one direction octant (almost all tiles) run the walk
 // specialised for it; mixed waves run the generic one.
 template <int STACK, int BLOCK, bool WIDE, int SPEC, bool COUNT = false, int LDS = kLdsStack>
 __device__ __forceinline__ void trace(const MapData& m, const V3& o, const V3& d, float range_max,
-                                      int (*stk)[BLOCK], Hit& h, bool spec = false, VisitCount* vc = nullptr) {
+                                      int (*stk)[BLOCK], Hit& h, VisitCount* vc = nullptr) {
   h.t = (double)range_max;
   h.slot = 0xFFFFFFFFu;
   h.face = 0xFFFFFFFFu;
@@ -480,15 +479,15 @@ __device__ __forceinline__ void trace(const MapData& m, const V3& o, const V3& d
       }
     } else {
       switch (woct) {
-        case 0: walk<0, STACK, BLOCK, SPEC, COUNT, LDS>(m, r, tbf, stk, spill, ref, sp, leaf, spec, vc); break;
-        case 1: walk<1, STACK, BLOCK, SPEC, COUNT, LDS>(m, r, tbf, stk, spill, ref, sp, leaf, spec, vc); break;
-        case 2: walk<2, STACK, BLOCK, SPEC, COUNT, LDS>(m, r, tbf, stk, spill, ref, sp, leaf, spec, vc); break;
-        case 3: walk<3, STACK, BLOCK, SPEC, COUNT, LDS>(m, r, tbf, stk, spill, ref, sp, leaf, spec, vc); break;
-        case 4: walk<4, STACK, BLOCK, SPEC, COUNT, LDS>(m, r, tbf, stk, spill, ref, sp, leaf, spec, vc); break;
-        case 5: walk<5, STACK, BLOCK, SPEC, COUNT, LDS>(m, r, tbf, stk, spill, ref, sp, leaf, spec, vc); break;
-        case 6: walk<6, STACK, BLOCK, SPEC, COUNT, LDS>(m, r, tbf, stk, spill, ref, sp, leaf, spec, vc); break;
-        case 7: walk<7, STACK, BLOCK, SPEC, COUNT, LDS>(m, r, tbf, stk, spill, ref, sp, leaf, spec, vc); break;
-        default: walk<-1, STACK, BLOCK, SPEC, COUNT, LDS>(m, r, tbf, stk, spill, ref, sp, leaf, spec, vc); break;
+        case 0: walk<0, STACK, BLOCK, SPEC, COUNT, LDS>(m, r, tbf, stk, spill, ref, sp, leaf, vc); break;
+        case 1: walk<1, STACK, BLOCK, SPEC, COUNT, LDS>(m, r, tbf, stk, spill, ref, sp, leaf, vc); break;
+        case 2: walk<2, STACK, BLOCK, SPEC, COUNT, LDS>(m, r, tbf, stk, spill, ref, sp, leaf, vc); break;
+        case 3: walk<3, STACK, BLOCK, SPEC, COUNT, LDS>(m, r, tbf, stk, spill, ref, sp, leaf, vc); break;
+        case 4: walk<4, STACK, BLOCK, SPEC, COUNT, LDS>(m, r, tbf, stk, spill, ref, sp, leaf, vc); break;
+        case 5: walk<5, STACK, BLOCK, SPEC, COUNT, LDS>(m, r, tbf, stk, spill, ref, sp, leaf, vc); break;
+        case 6: walk<6, STACK, BLOCK, SPEC, COUNT, LDS>(m, r, tbf, stk, spill, ref, sp, leaf, vc); break;
+        case 7: walk<7, STACK, BLOCK, SPEC, COUNT, LDS>(m, r, tbf, stk, spill, ref, sp, leaf, vc); break;
+        default: walk<-1, STACK, BLOCK, SPEC, COUNT, LDS>(m, r, tbf, stk, spill, ref, sp, leaf, vc); break;
       }
     }
     if constexpr (SPEC == kSpecOff) {

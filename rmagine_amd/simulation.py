@@ -392,7 +392,8 @@ def simulator_for(model, map: HipMap | None = None, device="cuda") -> SimulatorH
 
 
 class BatchSimulatorHip:
-    """Many sensors -- any mix of models, poses and ray counts -- cast in ONE persistent launch.
+    """Many sensors -- any mix of models, poses and ray counts -- cast by two persistent
+    launches (image sensors, then ray lists).
 
     entries: list of dicts {"model": sensor model, "Tbm": (N,8) device tensor,
     "attrs": attribute names (default hits+ranges), "Tsb": optional (8,) pose}.
