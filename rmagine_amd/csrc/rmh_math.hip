@@ -218,6 +218,13 @@ struct M3d {
   double m[9];  // row-major rotation of q p q^-1 (valid for any q, |q|^2 scaled)
 };
 
+__device__ __forceinline__ double uniform_f64(double v) {
+  const uint64_t u = __double_as_longlong(v);
+  const uint32_t lo = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(u));
+  const uint32_t hi = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(u >> 32));
+  return __longlong_as_double(static_cast<long long>((static_cast<uint64_t>(hi) << 32) | lo));
+}
+
 __device__ __forceinline__ M3d quat_matrix(const rmh_quat& qf) {
   const double x = qf.x, y = qf.y, z = qf.z, w = qf.w;
   M3d r;
@@ -246,37 +253,53 @@ __device__ void finalize_stats(rmh_cross_stats* out, const Acc<kMom>& m, const d
   *out = r;
 }
 
-__global__ __launch_bounds__(kBlk) void k_stats(const StatArgs a) {
+// IDX = uint32_t when every segment's offsets fit 32 bits (the usual case): the segment's
+// arrays are then addressed as block-uniform base pointer + 32-bit lane offset; uint64_t
+// otherwise.  The kernel streams its inputs once, so resident waves are what hides the load
+// latency: it is held to 80 VGPRs for six waves per SIMD (a few bytes per lane spill; on
+// 2,000 x 32,768 point-to-plane correspondences 0.490 ms at five waves, 0.455 ms at six).
+template <typename IDX>
+__global__ __launch_bounds__(kBlk) __attribute__((amdgpu_waves_per_eu(6))) void k_stats(const StatArgs a) {
   __shared__ double lds[kWaves][kMom];
   const uint32_t s = blockIdx.y, bb = blockIdx.x;
   const rmh_transform P = a.pre[a.npre == 1 ? 0 : s];
-  const M3d R = quat_matrix(P.R);
-  // moments are taken relative to the pre-transform's translation (exact shift, keeps the
-  // cancellation in sum(m d^T)/n - mean_m mean_d^T small)
-  const double c[3] = {(double)P.t.x, (double)P.t.y, (double)P.t.z};
+  M3d R = quat_matrix(P.R);
+  // block-uniform: held in scalar registers, used as the scalar operand of the FP64 FMAs
+  for (int k = 0; k < 9; k++) R.m[k] = uniform_f64(R.m[k]);
+  // moments are taken relative to the pre-transform's translation c = t (keeps the
+  // cancellation in sum(m d^T)/n - mean_m mean_d^T small); relative to c the transformed
+  // dataset point is R d and the model point is m - c
+  const double c[3] = {uniform_f64((double)P.t.x), uniform_f64((double)P.t.y), uniform_f64((double)P.t.z)};
   Acc<kMom> m;
   zero(m);
   const uint64_t db = static_cast<uint64_t>(s) * a.d.segment_stride;
   const uint64_t mb = static_cast<uint64_t>(s) * a.m.segment_stride;
-  const uint64_t stride = static_cast<uint64_t>(a.bps) * kBlk;
-  const double maxd = a.prm.max_dist;
-  for (uint64_t i = static_cast<uint64_t>(bb) * kBlk + threadIdx.x; i < a.seg; i += stride) {
-    const uint64_t di = db + i, mi = mb + i;
+  const rmh_vec3* __restrict__ dpts = a.d.points + db;
+  const rmh_vec3* __restrict__ mpts = a.m.points + mb;
+  const rmh_vec3* __restrict__ mnrm = a.m.normals ? a.m.normals + mb : nullptr;
+  const uint8_t* __restrict__ dmask = a.d.mask ? a.d.mask + db : nullptr;
+  const uint8_t* __restrict__ mmask = a.m.mask ? a.m.mask + mb : nullptr;
+  const uint32_t* __restrict__ dids = a.d.ids ? a.d.ids + db : nullptr;
+  const uint32_t* __restrict__ mids = a.m.ids ? a.m.ids + mb : nullptr;
+  const IDX stride = static_cast<IDX>(a.bps) * kBlk;
+  const IDX seg = static_cast<IDX>(a.seg);
+  const double maxd = uniform_f64(a.prm.max_dist);
+  for (IDX i = static_cast<IDX>(bb) * kBlk + threadIdx.x; i < seg; i += stride) {
     // statistics.cu:230-235 / statistics.cpp:99-103: mask and id filters
-    if (a.d.mask && a.d.mask[di] == 0) continue;
-    if (a.m.mask && a.m.mask[mi] == 0) continue;
-    if (a.d.ids && a.d.ids[di] != a.prm.dataset_id) continue;
-    if (a.m.ids && a.m.ids[mi] != a.prm.model_id) continue;
-    const rmh_vec3 dp = a.d.points[di];
-    const rmh_vec3 mp = a.m.points[mi];
-    // Di = pre_transform * dataset point (FP64)
-    const double dx = R.m[0] * dp.x + R.m[1] * dp.y + R.m[2] * dp.z + (double)P.t.x;
-    const double dy = R.m[3] * dp.x + R.m[4] * dp.y + R.m[5] * dp.z + (double)P.t.y;
-    const double dz = R.m[6] * dp.x + R.m[7] * dp.y + R.m[8] * dp.z + (double)P.t.z;
-    double mx = mp.x, my = mp.y, mz = mp.z;
+    if (dmask && dmask[i] == 0) continue;
+    if (mmask && mmask[i] == 0) continue;
+    if (dids && dids[i] != a.prm.dataset_id) continue;
+    if (mids && mids[i] != a.prm.model_id) continue;
+    const rmh_vec3 dp = dpts[i];
+    const rmh_vec3 mp = mpts[i];
+    // Di - c = R * dataset point (the pre-transform's rotation, FP64)
+    const double dx = R.m[0] * dp.x + R.m[1] * dp.y + R.m[2] * dp.z;
+    const double dy = R.m[3] * dp.x + R.m[4] * dp.y + R.m[5] * dp.z;
+    const double dz = R.m[6] * dp.x + R.m[7] * dp.y + R.m[8] * dp.z;
+    double mx = (double)mp.x - c[0], my = (double)mp.y - c[1], mz = (double)mp.z - c[2];
     if (a.kind == 1) {
       // point to plane: project Di onto the model plane (statistics.cpp:105-119)
-      const rmh_vec3 nn = a.m.normals[mi];
+      const rmh_vec3 nn = mnrm[i];
       const double spd = (mx - dx) * nn.x + (my - dy) * nn.y + (mz - dz) * nn.z;
       if (!(fabs(spd) < maxd)) continue;
       mx = dx + nn.x * spd;
@@ -287,7 +310,7 @@ __global__ __launch_bounds__(kBlk) void k_stats(const StatArgs a) {
       const double ex = mx - dx, ey = my - dy, ez = mz - dz;
       if (!(sqrt(ex * ex + ey * ey + ez * ez) < maxd)) continue;
     }
-    acc_pair(m, dx - c[0], dy - c[1], dz - c[2], mx - c[0], my - c[1], mz - c[2]);
+    acc_pair(m, dx, dy, dz, mx, my, mz);
   }
   block_sum(m, lds);
   if (threadIdx.x != 0) return;
@@ -443,9 +466,11 @@ __global__ void k_umeyama(const rmh_cross_stats* st, uint32_t n, rmh_transform* 
 }
 
 uint32_t blocks_per_segment(uint64_t seg, uint32_t nseg) {
-  // enough blocks in total to fill the chip (>= 1024), at least 2048 elements per block
+  // enough blocks for several rounds of the chip's resident blocks (>= 8192 in total, so
+  // that the last, partly filled round is a small part of the launch), at least 2048
+  // elements per block
   const uint64_t by_size = (seg + 2047) / 2048;
-  const uint64_t want = (1024 + nseg - 1) / nseg;
+  const uint64_t want = (8192 + nseg - 1) / nseg;
   uint64_t b = want < by_size ? want : by_size;
   if (b < 1) b = 1;
   if (b > 256) b = 256;
@@ -578,7 +603,11 @@ int rmh_statistics(int kind, const rmh_transform* pre_transforms, uint32_t n_pre
       A.partial = static_cast<double*>(scratch.p);
     }
   }
-  k_stats<<<dim3(A.bps, n_segments), kBlk, 0, stream>>>(A);
+  // 32-bit offsets when i * 12 B (the largest element) and i + stride stay below 2^32
+  if (segment_size + static_cast<uint64_t>(A.bps) * kBlk < (1ull << 32) / sizeof(rmh_vec3))
+    k_stats<uint32_t><<<dim3(A.bps, n_segments), kBlk, 0, stream>>>(A);
+  else
+    k_stats<uint64_t><<<dim3(A.bps, n_segments), kBlk, 0, stream>>>(A);
   RMH_HIP_TRY(hipGetLastError());
   if (A.bps > 1) {
     k_stats_combine<<<grid1(n_segments), kBlk, 0, stream>>>(A);
