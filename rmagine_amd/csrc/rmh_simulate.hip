@@ -54,13 +54,6 @@ struct SimArgs {
   rmh_outputs out;
 };
 
-// Workgroups are dispatched round-robin over the 8 XCDs (workgroup b runs on XCD b % 8).
-// Renumbering so that XCD x receives one contiguous range of (pose, tile) indices keeps
-// the BVH nodes of neighbouring tiles in that XCD's L2.  A bijection on [0, n).
-__device__ __forceinline__ uint32_t xcd_contiguous(uint32_t b, uint32_t n) {
-  const uint32_t xcd = b & 7u, idx = b >> 3, q = n >> 3, r = n & 7u;
-  return xcd < r ? xcd * (q + 1) + idx : r * (q + 1) + (xcd - r) * q + idx;
-}
 
 // Launch bounds ask for 8 waves per SIMD (64 VGPRs); the few values that do not fit are
 // set-up constants spilled outside the node loop.
@@ -396,8 +389,7 @@ int rmh_simulate(const rmh_map* map, const rmh_sensor* sensor, const rmh_transfo
   hipStream_t stream = static_cast<hipStream_t>(stream_);
 
   a.tl = sensor_tiling(sensor);
-  const uint64_t max_blocks = 1ull << 31;
-  const uint32_t poses_per_launch = static_cast<uint32_t>(std::max<uint64_t>(1, max_blocks / a.tl.tiles));
+  const uint32_t poses_per_launch = static_cast<uint32_t>(std::max<uint64_t>(1, kMaxTileBlocks / a.tl.tiles));
   for (uint32_t p0 = 0; p0 < n_poses; p0 += poses_per_launch) {
     const uint32_t np = std::min<uint32_t>(poses_per_launch, n_poses - p0);
     a.pose_offset = p0;
@@ -426,7 +418,7 @@ int rmh_simulate_visit_stats(const rmh_map* map, const rmh_sensor* sensor, const
   a.Tsb = load_tf(Tsb ? *Tsb : I);
   a.Tbm = Tbm;
   a.tl = sensor_tiling(sensor);
-  if (static_cast<uint64_t>(n_poses) * a.tl.tiles >= (1ull << 31))
+  if (static_cast<uint64_t>(n_poses) * a.tl.tiles > kMaxTileBlocks)
     return fail(RMH_ERR_UNSUPPORTED, "rmh_simulate_visit_stats: too many rays for one launch");
   unsigned long long* dev = nullptr;
   RMH_HIP_TRY(hipMalloc(&dev, sizeof(unsigned long long) * 6));

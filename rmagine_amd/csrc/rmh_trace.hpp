@@ -558,6 +558,18 @@ inline Tiling sensor_tiling(const rmh_sensor* s, uint32_t rows = kDefaultTileRow
 }
 
 // (row, column) of lane `lane` in tile `tile`; false for lanes outside the image.
+// Workgroups are dispatched round-robin over the 8 XCDs (workgroup b runs on XCD b % 8).
+// Renumbering so that XCD x receives one contiguous range of (pose, tile) indices keeps
+// the BVH nodes of neighbouring tiles in that XCD's L2.  A bijection on [0, n).
+__device__ __forceinline__ uint32_t xcd_contiguous(uint32_t b, uint32_t n) {
+  const uint32_t xcd = b & 7u, idx = b >> 3, q = n >> 3, r = n & 7u;
+  return xcd < r ? xcd * (q + 1) + idx : r * (q + 1) + (xcd - r) * q + idx;
+}
+
+// Most one-wave workgroups per launch of the (pose, tile) kernels: a dispatch holds the grid
+// size in work-items as a 32-bit count, so blocks x 64 stays below 2^31.
+constexpr uint64_t kMaxTileBlocks = 1ull << 25;
+
 __device__ __forceinline__ bool tile_ray(const Tiling& t, uint32_t W, uint32_t H, uint32_t tile, uint32_t lane,
                                          uint32_t& vid, uint32_t& hid) {
   const uint32_t ty = tile / t.tiles_w, tx = tile - ty * t.tiles_w;

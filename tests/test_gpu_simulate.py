@@ -233,6 +233,35 @@ def test_empty_map_and_degenerate():
     compare(run_gpu(hm2, m, Tbm), oracle.simulate(om2, m, Tbm), "degenerate tri")
 
 
+def test_poses_split_over_launches(cube):
+    """More (pose, tile) workgroups than one dispatch may hold (2^25): the poses are cast by
+    several launches; poses on both sides of the split equal their single-pose results."""
+    V, F, hm, om = cube
+    rng = np.random.default_rng(21)
+    d = rng.normal(size=(64, 3))
+    model = T.O1DnModel(64, 1, T.Interval(0.01, 10.0), (0.0, 0.0, 0.0),
+                        (d / np.linalg.norm(d, axis=1, keepdims=True)).astype(np.float32))
+    n = (1 << 25) + 5  # one 64-ray tile per pose
+    idx = torch.arange(n, device="cuda", dtype=torch.int64)
+    Tbm = torch.zeros((n, 8), dtype=torch.float32, device="cuda")
+    Tbm[:, 3] = 1.0
+    Tbm[:, 4] = ((idx % 97) - 48).float() * 1e-3
+    Tbm[:, 5] = ((idx % 89) - 44).float() * 1e-3
+    sim = simulator_for(model, hm)
+    res = sim.simulate(Tbm, attrs=("hits", "ranges"))
+    torch.cuda.synchronize()
+    for p in (0, (1 << 25) - 1, 1 << 25, n - 1):
+        one = sim.simulate(Tbm[p:p + 1].contiguous(), attrs=("hits", "ranges"))
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(res.ranges[p * 64:(p + 1) * 64].cpu().numpy().view(np.uint32),
+                                      one.ranges.cpu().numpy().view(np.uint32), err_msg=f"pose {p}")
+        np.testing.assert_array_equal(res.hits[p * 64:(p + 1) * 64].cpu().numpy(), one.hits.cpu().numpy())
+        o = oracle.simulate(om, model, Tbm[p:p + 1].cpu().numpy(), attrs=("hits", "ranges"))
+        assert (o["hits"] == one.hits.cpu().numpy()).all() and one.hits.cpu().numpy().all()
+    del res
+    torch.cuda.empty_cache()
+
+
 def test_zero_poses_and_errors(cube):
     _, _, hm, _ = cube
     sim = SphereSimulatorHip(hm)
