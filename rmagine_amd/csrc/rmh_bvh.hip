@@ -147,7 +147,7 @@ __global__ __launch_bounds__(kBlock) void k_bottom_up(int n, const uint32_t* __r
                                                       const int* __restrict__ parent_int,
                                                       const int* __restrict__ parent_leaf,
                                                       float4* node_lo, float4* node_hi,
-                                                      uint32_t* arrive, uint8_t* collapse,
+                                                      uint32_t* arrive, uint8_t* collapse, uint32_t* subtree,
                                                       uint32_t max_leaf, float c_node, float c_tri) {
   const int k = blockIdx.x * kBlock + threadIdx.x;
   if (k >= n) return;
@@ -160,6 +160,7 @@ __global__ __launch_bounds__(kBlock) void k_bottom_up(int n, const uint32_t* __r
     if (old == 0u) return;  // first arrival: the sibling's thread finishes this node
     float4 lo[2], hi[2];
     float cost[2], cnt[2];
+    uint32_t below = 0;  // interior nodes of the final tree in the children's subtrees
     for (int c = 0; c < 2; c++) {
       const uint32_t ch = child[2 * p + c];
       if (ch & kLeafBit) {
@@ -174,6 +175,7 @@ __global__ __launch_bounds__(kBlock) void k_bottom_up(int n, const uint32_t* __r
         hi[c] = node_hi[ch];
         cost[c] = lo[c].w;
         cnt[c] = hi[c].w;
+        below += subtree[ch];
       }
     }
     float4 L = make_float4(fminf(lo[0].x, lo[1].x), fminf(lo[0].y, lo[1].y), fminf(lo[0].z, lo[1].z), 0.0f);
@@ -192,6 +194,7 @@ __global__ __launch_bounds__(kBlock) void k_bottom_up(int n, const uint32_t* __r
     node_lo[p] = L;
     node_hi[p] = H;
     collapse[p] = col;
+    subtree[p] = col ? 0u : 1u + below;  // a collapsed node becomes a leaf of the final tree
     p = parent_int[p];
   }
 }
@@ -260,31 +263,9 @@ __global__ __launch_bounds__(kBlock) void k_layout(int n, const uint32_t* __rest
 // one after another lie far apart.  The final tree is rewritten in depth-first preorder
 // (a node, its first child's subtree, its second child's subtree): a descent through first
 // children reads consecutive 64-B nodes, every subtree is one contiguous range, and the
-// array holds only reachable nodes.
-// 8a: interior nodes per reachable subtree, bottom-up; a node is finished by the arrival
-// of its last interior child (the acquire-release atomic publishes the children's sizes).
-__global__ __launch_bounds__(kBlock) void k_subtree_sizes(uint32_t ni, const uint8_t* __restrict__ reach,
-                                                          const int* __restrict__ parent, const Node2* __restrict__ nodes,
-                                                          uint32_t* size, uint32_t* arrive, uint32_t* err) {
-  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-  if (i >= ni || !reach[i]) return;
-  const int4 d0 = nodes[i].d;
-  if (d0.x >= 0 || d0.y >= 0) return;  // a node with interior children is finished by them
-  size[i] = 1u;
-  int j = static_cast<int>(i);
-  for (int steps = 0; j != 0; steps++) {
-    if (steps >= kMaxTreeWalk) { atomicOr(err, 1u); return; }
-    const int p = parent[j];
-    const int4 d = nodes[p].d;
-    const uint32_t need = (d.x >= 0 ? 1u : 0u) + (d.y >= 0 ? 1u : 0u);
-    const uint32_t old = __hip_atomic_fetch_add(&arrive[p], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    if (old + 1u != need) return;  // the other interior child finishes p
-    size[p] = 1u + (d.x >= 0 ? size[d.x] : 0u) + (d.y >= 0 ? size[d.y] : 0u);
-    j = p;
-  }
-}
-
-// 8b: preorder position = sum over the path from the root of 1 per step, plus the size
+// array holds only reachable nodes.  The interior-node count of every subtree of the final
+// tree (size) comes from the bottom-up pass of step 5.
+// 8a: preorder position = sum over the path from the root of 1 per step, plus the size
 // of the first child's subtree wherever the path takes the second child.
 __global__ __launch_bounds__(kBlock) void k_preorder_rank(uint32_t ni, const uint8_t* __restrict__ reach,
                                                           const int* __restrict__ parent, const Node2* __restrict__ nodes,
@@ -304,7 +285,7 @@ __global__ __launch_bounds__(kBlock) void k_preorder_rank(uint32_t ni, const uin
   rank[i] = r;
 }
 
-// 8c: move every reachable node to its preorder slot, with its child references, parent
+// 8b: move every reachable node to its preorder slot, with its child references, parent
 // link and reach flag renumbered.
 __global__ __launch_bounds__(kBlock) void k_preorder_scatter(uint32_t ni, uint32_t n_out, const uint8_t* __restrict__ reach,
                                                              const int* __restrict__ parent, const Node2* __restrict__ nodes,
@@ -791,28 +772,22 @@ hipError_t build_wide(rmh_map* m, hipStream_t stream) {
 }  // namespace
 
 // Rewrites m->nodes / m->parent / m->reach (ni radix-tree slots, n_reach of them reachable,
-// root at slot 0) into preorder, in buffers of exactly n_reach entries.
-static hipError_t preorder_nodes(rmh_map* m, uint32_t ni, uint32_t n_reach, hipStream_t stream) {
-  DevBuf size, rank, arrive, err;
+// root at slot 0, size = interior nodes of the final tree per subtree) into preorder, in
+// buffers of exactly n_reach entries.
+static hipError_t preorder_nodes(rmh_map* m, uint32_t ni, uint32_t n_reach, const uint32_t* size,
+                                 hipStream_t stream) {
+  DevBuf rank, err;
   Node2* nodes = nullptr;
   int* parent = nullptr;
   uint8_t* reach = nullptr;
-  hipError_t e = hipMalloc(&size.p, sizeof(uint32_t) * ni);
-  if (e == hipSuccess) e = hipMalloc(&rank.p, sizeof(uint32_t) * ni);
-  if (e == hipSuccess) e = hipMalloc(&arrive.p, sizeof(uint32_t) * ni);
+  hipError_t e = hipMalloc(&rank.p, sizeof(uint32_t) * ni);
   if (e == hipSuccess) e = hipMalloc(&err.p, sizeof(uint32_t));
   if (e == hipSuccess) e = hipMalloc(&nodes, sizeof(Node2) * n_reach);
   if (e == hipSuccess) e = hipMalloc(&parent, sizeof(int) * n_reach);
   if (e == hipSuccess) e = hipMalloc(&reach, n_reach);
-  if (e == hipSuccess) e = hipMemsetAsync(arrive.p, 0, sizeof(uint32_t) * ni, stream);
   if (e == hipSuccess) e = hipMemsetAsync(err.p, 0, sizeof(uint32_t), stream);
   if (e == hipSuccess) {
-    k_subtree_sizes<<<blocks_for(ni), kBlock, 0, stream>>>(ni, m->reach, m->parent, m->nodes, size.as<uint32_t>(),
-                                                           arrive.as<uint32_t>(), err.as<uint32_t>());
-    e = hipGetLastError();
-  }
-  if (e == hipSuccess) {
-    k_preorder_rank<<<blocks_for(ni), kBlock, 0, stream>>>(ni, m->reach, m->parent, m->nodes, size.as<uint32_t>(),
+    k_preorder_rank<<<blocks_for(ni), kBlock, 0, stream>>>(ni, m->reach, m->parent, m->nodes, size,
                                                            rank.as<uint32_t>(), err.as<uint32_t>());
     e = hipGetLastError();
   }
@@ -823,7 +798,7 @@ static hipError_t preorder_nodes(rmh_map* m, uint32_t ni, uint32_t n_reach, hipS
   }
   uint32_t herr = 0, root_size = 0;
   if (e == hipSuccess) e = hipMemcpyAsync(&herr, err.p, sizeof(uint32_t), hipMemcpyDeviceToHost, stream);
-  if (e == hipSuccess) e = hipMemcpyAsync(&root_size, size.p, sizeof(uint32_t), hipMemcpyDeviceToHost, stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(&root_size, size, sizeof(uint32_t), hipMemcpyDeviceToHost, stream);
   if (e == hipSuccess) e = hipStreamSynchronize(stream);
   // a walk that did not end at the root, or a count that differs from the reachable
   // nodes, means the links do not form one tree: the build fails
@@ -919,7 +894,7 @@ static int map_build(const float* vertices, uint32_t nv, const uint32_t* faces, 
   const int n = static_cast<int>(nf);
   const uint32_t ni = nf > 1 ? nf - 1 : 0;
   DevBuf flo, fhi, bounds, err, keys, keys2, vals, vals2, child, range, pint, pleaf, nlo, nhi, arrive, coll,
-      stats, moff, tmp;
+      stats, moff, tmp, subtree;
   BUILD_TRY(hipMalloc(&flo.p, sizeof(float4) * nf));
   BUILD_TRY(hipMalloc(&fhi.p, sizeof(float4) * nf));
   BUILD_TRY(hipMalloc(&bounds.p, sizeof(float) * 8));
@@ -936,6 +911,7 @@ static int map_build(const float* vertices, uint32_t nv, const uint32_t* faces, 
   BUILD_TRY(hipMalloc(&nhi.p, sizeof(float4) * (ni + 1)));
   BUILD_TRY(hipMalloc(&arrive.p, sizeof(uint32_t) * (ni + 1)));
   BUILD_TRY(hipMalloc(&coll.p, sizeof(uint8_t) * (ni + 1)));
+  BUILD_TRY(hipMalloc(&subtree.p, sizeof(uint32_t) * (ni + 1)));
   BUILD_TRY(hipMalloc(&stats.p, sizeof(uint32_t) * 4));
   BUILD_TRY(hipMalloc(&moff.p, sizeof(uint32_t) * (n_meshes + 1)));
   BUILD_TRY(hipMalloc(&m->tris, sizeof(float4) * 3 * static_cast<size_t>(nf)));
@@ -1027,7 +1003,8 @@ static int map_build(const float* vertices, uint32_t nv, const uint32_t* faces, 
     BUILD_TRY(hipMemsetAsync(arrive.p, 0, sizeof(uint32_t) * ni, stream));
     k_bottom_up<<<blocks_for(nf), kBlock, 0, stream>>>(n, svals, flo.as<float4>(), fhi.as<float4>(), child.as<uint32_t>(),
                                                        pint.as<int>(), pleaf.as<int>(), nlo.as<float4>(), nhi.as<float4>(),
-                                                       arrive.as<uint32_t>(), coll.as<uint8_t>(), prm.max_leaf_size,
+                                                       arrive.as<uint32_t>(), coll.as<uint8_t>(), subtree.as<uint32_t>(),
+                                                       prm.max_leaf_size,
                                                        prm.sah_cost_node, prm.sah_cost_tri);
     BUILD_TRY(hipGetLastError());
     k_layout<<<blocks_for(ni), kBlock, 0, stream>>>(n, svals, flo.as<float4>(), fhi.as<float4>(), child.as<uint32_t>(),
@@ -1073,7 +1050,7 @@ static int map_build(const float* vertices, uint32_t nv, const uint32_t* faces, 
   }
   if (m->root == 0 && hstats[0] > 0) {
     BUILD_TRY(hipEventRecord(ev0, stream));
-    BUILD_TRY(preorder_nodes(m, ni, hstats[0], stream));
+    BUILD_TRY(preorder_nodes(m, ni, hstats[0], subtree.as<uint32_t>(), stream));
     BUILD_TRY(hipEventRecord(ev1, stream));
     BUILD_TRY(hipStreamSynchronize(stream));
     float order_ms = 0.0f;
