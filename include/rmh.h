@@ -271,7 +271,8 @@ typedef struct {
                              1, 2, 4, 8, 16, 32 or 64, capped by the image height
                              (0 = keep / default 4)                                      */
   uint32_t lds_stack;     /* traversal stack entries per lane kept in LDS: 8, 16 or 24
-                             (0 = keep / default 16); deeper entries go to scratch       */
+                             (8 B each; 0 = keep / default 8); deeper entries go to
+                             scratch                                                     */
 } rmh_batch_params;
 
 typedef struct rmh_batch rmh_batch;

@@ -43,7 +43,7 @@ constexpr int kWave = 64;
 
 template <int KIND, int STACK, bool WIDE>
 __global__ __launch_bounds__(kWave, 8) void k_pose_grad(const GradArgs a) {
-  __shared__ int stk[kLdsStack][kWave];
+  __shared__ int2 stk[kLdsStack][kWave];
   const uint32_t pose = a.pose_offset + blockIdx.y;
   uint32_t vid, hid, lid;
   const bool active = tile_ray(a.tl, a.g.grid_w, a.g.grid_h, blockIdx.x, threadIdx.x, vid, hid);

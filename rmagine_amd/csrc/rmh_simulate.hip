@@ -59,7 +59,7 @@ struct SimArgs {
 // set-up constants spilled outside the node loop.
 template <int KIND, int STACK, bool WIDE>
 __global__ __launch_bounds__(kWave, 8) void k_simulate(const SimArgs a) {
-  __shared__ int stk[kLdsStack][kWave];
+  __shared__ int2 stk[kLdsStack][kWave];
   const uint32_t lb = xcd_contiguous(blockIdx.x, gridDim.x);
   const uint32_t pl = lb / a.tl.tiles;
   const uint32_t tile = lb - pl * a.tl.tiles;
@@ -129,7 +129,7 @@ struct VisitArgs {
 // of writing: per-lane counts are summed over the wave and added to the totals.
 template <int KIND, int STACK>
 __global__ __launch_bounds__(kWave, 8) void k_visit_stats(const VisitArgs a) {
-  __shared__ int stk[kLdsStack][kWave];
+  __shared__ int2 stk[kLdsStack][kWave];
   const uint32_t lb = xcd_contiguous(blockIdx.x, gridDim.x);
   const uint32_t pl = lb / a.tl.tiles;
   const uint32_t tile = lb - pl * a.tl.tiles;
@@ -204,7 +204,7 @@ __device__ __forceinline__ void batch_gen(const BatchEntryDev& e, uint32_t vid, 
 // through the traversal.
 template <int STACK, int BLOCK, bool WIDE, int LDS, int SPEC, int K0, int K1>
 __device__ __forceinline__ void batch_ray(const BatchArgs& a, const BatchEntryDev& e, bool k0, uint32_t pose,
-                                          uint32_t vid, uint32_t hid, int (*stk)[BLOCK]) {
+                                          uint32_t vid, uint32_t hid, int2 (*stk)[BLOCK]) {
   Hit h;
   {
     const Tf Tsm = tmul(load_tf(e.Tbm[pose]), e.Tsb);
@@ -235,7 +235,7 @@ template <int STACK, int BLOCK, bool WIDE, int LDS, int SPEC>
 __global__ __launch_bounds__(BLOCK, 8) void k_batch(const BatchArgs a) {
   constexpr int K0 = SPEC == kSpecOff ? RMH_MODEL_SPHERICAL : RMH_MODEL_O1DN;
   constexpr int K1 = SPEC == kSpecOff ? RMH_MODEL_PINHOLE : RMH_MODEL_ONDN;
-  __shared__ int stk[LDS][BLOCK];
+  __shared__ int2 stk[LDS][BLOCK];
   const int lane = threadIdx.x & 63;
   for (;;) {
     uint32_t c = 0;
@@ -273,7 +273,7 @@ hipError_t launch_batch(const BatchArgs& a, bool lists, uint32_t blocks, hipStre
 }
 
 // LDS stack depth of the batched kernel (a launch parameter the autotuner chooses): 8, 16
-// or 24 entries per lane (2, 4 or 6 KiB per wave); deeper entries go to scratch.
+// or 24 entries of 8 B per lane (4, 8 or 12 KiB per wave); deeper entries go to scratch.
 template <int STACK, bool WIDE>
 hipError_t launch_batch_lds(uint32_t lds, const BatchArgs& a, bool lists, uint32_t blocks, hipStream_t stream) {
   if (lds == 8) return launch_batch<STACK, WIDE, 8>(a, lists, blocks, stream);
