@@ -233,6 +233,56 @@ def test_empty_map_and_degenerate():
     compare(run_gpu(hm2, m, Tbm), oracle.simulate(om2, m, Tbm), "degenerate tri")
 
 
+def test_exact_ties_on_shared_edges_and_vertices():
+    """Rays that land exactly on shared vertices, shared edges and cell diagonals of a
+    triangle grid hit several triangles at the same distance.  Closed forms: every such
+    ray hits (no crack between triangles) at t = 5, and the reported face is the lowest
+    global face among the triangles whose closed region holds the point."""
+    n = 6
+    xs, ys = np.meshgrid(np.arange(n + 1), np.arange(n + 1), indexing="ij")
+    V = np.stack([xs.ravel(), ys.ravel(), np.zeros(xs.size)], axis=1).astype(np.float32)
+    vid = lambda i, j: i * (n + 1) + j
+    F = []
+    for i in range(n):
+        for j in range(n):
+            # alternate the diagonal so that vertices have 4 to 8 incident triangles
+            if (i + j) % 2 == 0:
+                F += [[vid(i, j), vid(i + 1, j), vid(i + 1, j + 1)], [vid(i, j), vid(i + 1, j + 1), vid(i, j + 1)]]
+            else:
+                F += [[vid(i, j), vid(i + 1, j), vid(i, j + 1)], [vid(i + 1, j), vid(i + 1, j + 1), vid(i, j + 1)]]
+    F = np.array(F, dtype=np.uint32)
+    rng = np.random.default_rng(4)
+    F = F[rng.permutation(len(F))]  # face order unrelated to the Morton order of the build
+    pts = []
+    for i in range(n + 1):
+        for j in range(n + 1):
+            for dx, dy in ((0.0, 0.0), (0.5, 0.0), (0.0, 0.5), (0.5, 0.5)):
+                if i + dx <= n and j + dy <= n:
+                    pts.append((i + dx, j + dy))
+    pts = np.array(pts, dtype=np.float64)
+
+    def in_closed(p, tri):
+        a, b, c = (V[k, :2].astype(np.float64) for k in tri)
+        cr = lambda u, v, w: (v[0] - u[0]) * (w[1] - u[1]) - (v[1] - u[1]) * (w[0] - u[0])
+        d = np.array([cr(a, b, p), cr(b, c, p), cr(c, a, p)])
+        return (d >= 0).all() or (d <= 0).all()
+
+    expect = np.array([min(k for k, tri in enumerate(F) if in_closed(p, tri)) for p in pts], dtype=np.uint32)
+    hm = HipMap(V, F)
+    om = oracle.OracleMap(V, F)
+    for d in ((0.0, 0.0, -1.0), (1.0, 1.0, -1.0)):
+        d = np.array(d, dtype=np.float32)
+        origs = np.concatenate([pts, np.zeros((len(pts), 1))], axis=1).astype(np.float32) - 5 * d
+        dirs = np.repeat(d[None], len(pts), axis=0)
+        m = T.OnDnModel(len(pts), 1, T.Interval(0.0, 100.0), origs, dirs)
+        g = run_gpu(hm, m, T.identity_transforms(1))
+        assert g["hits"].all(), f"dir {d}: {int((g['hits'] == 0).sum())} rays fell through"
+        np.testing.assert_array_equal(g["ranges"], np.float32(5.0), err_msg=f"dir {d}")
+        np.testing.assert_array_equal(g["face_ids"], expect, err_msg=f"dir {d}: lowest face among tied hits")
+        o = oracle.simulate(om, m, T.identity_transforms(1))
+        np.testing.assert_array_equal(g["face_ids"], o["face_ids"])
+
+
 def test_poses_split_over_launches(cube):
     """More (pose, tile) workgroups than one dispatch may hold (2^25): the poses are cast by
     several launches; poses on both sides of the split equal their single-pose results."""
