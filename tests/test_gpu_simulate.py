@@ -233,7 +233,8 @@ def test_empty_map_and_degenerate():
     compare(run_gpu(hm2, m, Tbm), oracle.simulate(om2, m, Tbm), "degenerate tri")
 
 
-def test_exact_ties_on_shared_edges_and_vertices():
+@pytest.mark.parametrize("wide", [False, True])
+def test_exact_ties_on_shared_edges_and_vertices(wide):
     """Rays that land exactly on shared vertices, shared edges and cell diagonals of a
     triangle grid hit several triangles at the same distance.  Closed forms: every such
     ray hits (no crack between triangles) at t = 5, and the reported face is the lowest
@@ -268,7 +269,7 @@ def test_exact_ties_on_shared_edges_and_vertices():
         return (d >= 0).all() or (d <= 0).all()
 
     expect = np.array([min(k for k, tri in enumerate(F) if in_closed(p, tri)) for p in pts], dtype=np.uint32)
-    hm = HipMap(V, F)
+    hm = HipMap(V, F, wide=wide)
     om = oracle.OracleMap(V, F)
     for d in ((0.0, 0.0, -1.0), (1.0, 1.0, -1.0)):
         d = np.array(d, dtype=np.float32)
