@@ -252,9 +252,9 @@ int rmh_simulate_visit_stats(const rmh_map* map, const rmh_sensor* sensor, const
  * No counterpart in the reference (it launches one sensor model per optixLaunch,
  * src/rmagine_optix/src/simulation/SphereSimulatorOptix.cpp:64-81; the closest piece is the
  * model union of src/rmagine_optix/include/rmagine/simulation/optix/sim_program_data.h:32-44).
- * A batch is a device table of sensors (any mix of models, poses and ray counts) cast by two
- * persistent launches (image sensors, then O1Dn/OnDn ray lists, which walk the tree
- * speculatively); every entry writes its own rmh_outputs exactly as rmh_simulate would. */
+ * A batch is a device table of sensors (any mix of models, poses and ray counts) cast in one
+ * persistent launch (rmh_batch_params.launches = 2 splits it into image sensors, then O1Dn/OnDn
+ * ray lists); every entry writes its own rmh_outputs exactly as rmh_simulate would. */
 typedef struct {
   const rmh_sensor*    sensor;
   rmh_transform        Tsb;        /* sensor -> base (host value)               */
@@ -273,6 +273,9 @@ typedef struct {
   uint32_t lds_stack;     /* traversal stack entries per lane kept in LDS: 8, 16 or 24
                              (8 B each; 0 = keep / default 8); deeper entries go to
                              scratch                                                     */
+  uint32_t launches;      /* persistent launches per rmh_batch_simulate: 1 = every entry in
+                             one launch (0 = keep / default), 2 = image sensors, then ray
+                             lists, each kernel with its own walk                       */
 } rmh_batch_params;
 
 typedef struct rmh_batch rmh_batch;
@@ -284,9 +287,9 @@ int rmh_batch_create(const rmh_map* map, const rmh_batch_entry* entries, uint32_
 int rmh_batch_set_params(rmh_batch* batch, const rmh_batch_params* params);
 int rmh_batch_get_info(const rmh_batch* batch, uint64_t* total_rays, uint32_t* total_chunks,
                        rmh_batch_params* params);
-/* Per group present (image sensors, ray lists): one counter-reset kernel + one persistent
- * kernel, in stream order on `stream`; capturable into a graph.  Launches of one batch must
- * not overlap (they share its work counters). */
+/* One counter-reset kernel + one persistent kernel on `stream` (per group when launches = 2,
+ * in stream order); capturable into a graph.  Launches of one batch must not overlap (they
+ * share its work counters). */
 int rmh_batch_simulate(rmh_batch* batch, void* stream);
 int rmh_batch_destroy(rmh_batch* batch);
 

@@ -12,9 +12,11 @@ first chunk x waves at the default tile, then the tile rows at the best pair, th
 depth of the traversal stack kept in LDS (`lds_stack`: 8, 16 or 24 rows of 8 B per ray; a
 shallower one leaves more LDS for resident waves, a deeper one spills less to private
 memory), then the ray order of the O1Dn/OnDn sensors (sorted by direction, or the list
-order the caller gave), then (when asked with layouts=(False, True)) the BVH node layout of
-the batch's map: binary or 4-wide with quantised boxes.  Stack depth, ray order and layout
-change speed, never results.  Results are cached as JSON keyed by device architecture, mesh
+order the caller gave), then (when asked with launches=(1, 2)) one persistent launch for
+every entry or two (image sensors, then ray lists, each kernel with its own walk), then
+(when asked with layouts=(False, True)) the BVH node layout of the batch's map: binary or
+4-wide with quantised boxes.  Stack depth, ray order, launch split and layout change speed,
+never results.  Results are cached as JSON keyed by device architecture, mesh
 size (power of two bucket of the face count) and sensor mix, in $RMH_TUNE_CACHE or
 ~/.cache/rmagine_amd/autotune.json.
 
@@ -84,7 +86,8 @@ def time_batch(batch, reps: int = 3, stream=None) -> float:
 
 def tune_batch(batch, key: str | None = None, chunks=DEFAULT_CHUNKS, waves=DEFAULT_WAVES,
                tile_rows=DEFAULT_TILE_ROWS, reps: int = 3, use_cache: bool = True, stream=None,
-               layouts=(False,), lds_stacks=DEFAULT_LDS_STACKS, ray_orders=DEFAULT_RAY_ORDERS) -> dict:
+               layouts=(False,), lds_stacks=DEFAULT_LDS_STACKS, ray_orders=DEFAULT_RAY_ORDERS,
+               launches=(1,)) -> dict:
     """Picks (chunk_rays, waves_per_cu, tile_rows, lds_stack, ray order) for `batch` and, if
     `layouts` holds both False and True, the node layout of its map (True = 4-wide
     quantised); applies the choice and returns the record.  Re-capture graphs of the batch
@@ -95,7 +98,8 @@ def tune_batch(batch, key: str | None = None, chunks=DEFAULT_CHUNKS, waves=DEFAU
         rec = _load(path).get(key)
         if rec:
             batch.set_params(chunk_rays=rec["chunk_rays"], waves_per_cu=rec["waves_per_cu"],
-                             tile_rows=rec.get("tile_rows", 4), lds_stack=rec.get("lds_stack", 0))
+                             tile_rows=rec.get("tile_rows", 4), lds_stack=rec.get("lds_stack", 0),
+                             launches=rec.get("launches", 0))
             if "rays_sorted" in rec:
                 batch.set_ray_order(rec["rays_sorted"])
             if "wide" in rec and hm.wide != rec["wide"]:
@@ -143,6 +147,17 @@ def tune_batch(batch, key: str | None = None, chunks=DEFAULT_CHUNKS, waves=DEFAU
     ms = order_ms[best_o]
     rec.update(rays_sorted=best_o, ms=ms, ray_order_ms={("sorted" if k else "list"): v for k, v in order_ms.items()})
     keys.append("rays_sorted")
+    launch0 = batch.params()["launches"]
+    launch_ms = {launch0: ms}
+    for n in launches:
+        if n not in launch_ms:
+            batch.set_params(launches=n)
+            launch_ms[n] = time_batch(batch, reps, stream)
+    best_l = min(launch_ms, key=launch_ms.get)
+    batch.set_params(launches=best_l)
+    ms = launch_ms[best_l]
+    rec.update(launches=best_l, ms=ms, launches_ms=launch_ms)
+    keys.append("launches")
     if len(set(layouts)) > 1:
         start = hm.wide
         timed = {start: ms}

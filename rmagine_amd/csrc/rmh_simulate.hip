@@ -21,12 +21,13 @@
 //
 // Two launch forms:
 //   rmh_simulate        one sensor model, n poses, one workgroup per (pose, tile);
-//   rmh_batch_simulate  many sensors (any mix of models, poses and ray counts) in two
-//                       persistent launches, image sensors then ray lists (their walks
-//                       differ): waves grab chunks of tiles of one sensor from an atomic
-//                       counter until the work is exhausted.  All pointers and sizes live
-//                       in a device table built once, so the launches (counter reset
-//                       kernel + persistent kernel, per group) can be captured into a graph.
+//   rmh_batch_simulate  many sensors (any mix of models, poses and ray counts) in one
+//                       persistent launch (or, as a tuning choice, two: image sensors,
+//                       then ray lists, each with its own walk): waves grab chunks of tiles
+//                       of one sensor from an atomic counter until the work is exhausted.
+//                       All pointers and sizes live in a device table built once, so the
+//                       launch (counter reset kernel + persistent kernel) can be captured
+//                       into a graph.
 // The reference has no batched form; its closest piece is the per-launch model union
 // (src/rmagine_optix/include/rmagine/simulation/optix/sim_program_data.h:32-44).
 #include "rmh_trace.hpp"
@@ -142,7 +143,7 @@ __global__ __launch_bounds__(kWave, 8) void k_visit_stats(const VisitArgs a) {
     const Tf Tsm = tmul(load_tf(a.Tbm[pl]), a.Tsb);
     Hit h;
     trace<STACK, kWave, false, spec_mode<KIND>(), true>(a.map, map_origin<KIND>(Tsm, os), qrot(Tsm.R, ds),
-                                                       a.g.range_max, stk, h, &vc);
+                                                       a.g.range_max, stk, h, false, &vc);
   }
   uint32_t v[6] = {vc.vec_nodes, vc.uni_nodes, vc.leaves, vc.tris, vc.wave_iters, vc.wave_uni_iters};
   for (int k = 0; k < 6; k++)
@@ -175,7 +176,7 @@ struct BatchEntryDev {
   uint64_t n_slots;  // n_poses * tiles * 64 ray slots, enumerated tile by tile
 };
 
-// One persistent launch covers the entries [entry_lo, entry_hi), whose chunks are
+// A persistent launch covers the entries [entry_lo, entry_hi), whose chunks are
 // [chunk_prefix[entry_lo], chunk_prefix[entry_hi]).
 struct BatchArgs {
   MapData map;
@@ -196,45 +197,58 @@ __device__ __forceinline__ void batch_gen(const BatchEntryDev& e, uint32_t vid, 
   om = map_origin<KIND>(Tsm, os);
 }
 
-// One batch slot of kind K0 or K1 (the two image models or the two ray-list models, which
-// share the walk of their launch).  Ray generation and attribute writing depend on the
-// kind, the traversal does not, so it is instantiated once per launch (one copy of the
+template <int K>
+struct KindTag {
+  static constexpr int value = K;
+};
+
+// Calls fn(KindTag<kind>) for the sensor kinds a batched kernel of mode SPEC serves: the
+// image models (kSpecOff), the ray-list models (kSpecOn) or all four (kSpecRuntime).
+template <int SPEC, class Fn>
+__device__ __forceinline__ void with_kind(int kind, Fn&& fn) {
+  if constexpr (SPEC != kSpecOn) {
+    if (kind == RMH_MODEL_SPHERICAL) { fn(KindTag<RMH_MODEL_SPHERICAL>{}); return; }
+    if (SPEC == kSpecOff || kind == RMH_MODEL_PINHOLE) { fn(KindTag<RMH_MODEL_PINHOLE>{}); return; }
+  }
+  if (kind == RMH_MODEL_O1DN) { fn(KindTag<RMH_MODEL_O1DN>{}); return; }
+  fn(KindTag<RMH_MODEL_ONDN>{});
+}
+
+// One batch slot.  Ray generation and attribute writing depend on the sensor kind, the
+// traversal does not, so it is instantiated once per kernel (one copy of the
 // octant-specialised walks keeps the kernel inside the instruction cache).  As in
 // k_simulate, the ray set-up is recomputed for the writers rather than held (spilled)
 // through the traversal.
-template <int STACK, int BLOCK, bool WIDE, int LDS, int SPEC, int K0, int K1>
-__device__ __forceinline__ void batch_ray(const BatchArgs& a, const BatchEntryDev& e, bool k0, uint32_t pose,
+template <int STACK, int BLOCK, bool WIDE, int LDS, int SPEC>
+__device__ __forceinline__ void batch_ray(const BatchArgs& a, const BatchEntryDev& e, int kind, uint32_t pose,
                                           uint32_t vid, uint32_t hid, int2 (*stk)[BLOCK]) {
   Hit h;
   {
     const Tf Tsm = tmul(load_tf(e.Tbm[pose]), e.Tsb);
     uint32_t lid;
     V3 ds, os, dm, om;
-    if (k0) batch_gen<K0>(e, vid, hid, lid, Tsm, ds, os, dm, om);
-    else batch_gen<K1>(e, vid, hid, lid, Tsm, ds, os, dm, om);
-    trace<STACK, BLOCK, WIDE, SPEC, false, LDS>(a.map, om, dm, e.g.range_max, stk, h);
+    with_kind<SPEC>(kind, [&](auto k) { batch_gen<decltype(k)::value>(e, vid, hid, lid, Tsm, ds, os, dm, om); });
+    trace<STACK, BLOCK, WIDE, SPEC, false, LDS>(a.map, om, dm, e.g.range_max, stk, h, speculative_walk(kind));
   }
   asm volatile("" : "+v"(vid), "+v"(hid));
   const Tf Tsm = tmul(load_tf(e.Tbm[pose]), e.Tsb);
   uint32_t lid;
   V3 ds, os;
-  if (k0) gen_ray<K0>(e.g, vid, hid, lid, ds, os);
-  else gen_ray<K1>(e.g, vid, hid, lid, ds, os);
+  with_kind<SPEC>(kind, [&](auto k) { gen_ray<decltype(k)::value>(e.g, vid, hid, lid, ds, os); });
   const uint64_t r = static_cast<uint64_t>(pose) * e.g.ray_count + lid;
-  if (k0) write_outputs<K0>(e.out, r, e.g, a.map, Tsm, ds, os, h);
-  else write_outputs<K1>(e.out, r, e.g, a.map, Tsm, ds, os, h);
+  with_kind<SPEC>(kind, [&](auto k) { write_outputs<decltype(k)::value>(e.out, r, e.g, a.map, Tsm, ds, os, h); });
 }
 
 __global__ void k_reset_counter(uint32_t* counter) {
   if (threadIdx.x == 0) *counter = 0u;  // a kernel store (not a memset node) keeps graph replays correct
 }
 
-// Persistent kernel over one group of entries: image sensors (spherical, pinhole) walk the
-// tree classically, ray lists (O1Dn, OnDn) speculatively, as they do in k_simulate.
+// Persistent kernel over a range of entries.  SPEC = kSpecRuntime: every sensor kind in one
+// launch, ray lists walking speculatively and image sensors with the leaf holding switched
+// off; kSpecOff / kSpecOn: image sensors only (classic walk) / ray lists only, for a batch
+// cast as two launches.
 template <int STACK, int BLOCK, bool WIDE, int LDS, int SPEC>
 __global__ __launch_bounds__(BLOCK, 8) void k_batch(const BatchArgs a) {
-  constexpr int K0 = SPEC == kSpecOff ? RMH_MODEL_SPHERICAL : RMH_MODEL_O1DN;
-  constexpr int K1 = SPEC == kSpecOff ? RMH_MODEL_PINHOLE : RMH_MODEL_ONDN;
   __shared__ int2 stk[LDS][BLOCK];
   const int lane = threadIdx.x & 63;
   for (;;) {
@@ -251,7 +265,7 @@ __global__ __launch_bounds__(BLOCK, 8) void k_batch(const BatchArgs a) {
     const BatchEntryDev& e = a.entries[lo];
     const uint64_t tile0 = static_cast<uint64_t>(c - a.chunk_prefix[lo]) * (a.chunk_rays / 64);
     const uint64_t n_tiles = static_cast<uint64_t>(e.n_poses) * e.tl.tiles;
-    const bool k0 = e.g.kind == K0;
+    const int kind = e.g.kind;
     for (uint32_t k = 0; k < a.chunk_rays / 64; k++) {
       const uint64_t tg = tile0 + k;  // wave-uniform
       if (tg >= n_tiles) break;
@@ -259,15 +273,16 @@ __global__ __launch_bounds__(BLOCK, 8) void k_batch(const BatchArgs a) {
       const uint32_t tile = static_cast<uint32_t>(tg - static_cast<uint64_t>(pose) * e.tl.tiles);
       uint32_t vid, hid;
       if (!tile_ray(e.tl, e.g.grid_w, e.g.grid_h, tile, lane, vid, hid)) continue;
-      batch_ray<STACK, BLOCK, WIDE, LDS, SPEC, K0, K1>(a, e, k0, pose, vid, hid, stk);
+      batch_ray<STACK, BLOCK, WIDE, LDS, SPEC>(a, e, kind, pose, vid, hid, stk);
     }
   }
 }
 
 template <int STACK, bool WIDE, int LDS>
-hipError_t launch_batch(const BatchArgs& a, bool lists, uint32_t blocks, hipStream_t stream) {
+hipError_t launch_batch(const BatchArgs& a, int spec, uint32_t blocks, hipStream_t stream) {
   k_reset_counter<<<1, 64, 0, stream>>>(a.counter);
-  if (lists) k_batch<STACK, kWave, WIDE, LDS, kSpecOn><<<blocks, kWave, 0, stream>>>(a);
+  if (spec == kSpecRuntime) k_batch<STACK, kWave, WIDE, LDS, kSpecRuntime><<<blocks, kWave, 0, stream>>>(a);
+  else if (spec == kSpecOn) k_batch<STACK, kWave, WIDE, LDS, kSpecOn><<<blocks, kWave, 0, stream>>>(a);
   else k_batch<STACK, kWave, WIDE, LDS, kSpecOff><<<blocks, kWave, 0, stream>>>(a);
   return hipGetLastError();
 }
@@ -275,22 +290,22 @@ hipError_t launch_batch(const BatchArgs& a, bool lists, uint32_t blocks, hipStre
 // LDS stack depth of the batched kernel (a launch parameter the autotuner chooses): 8, 16
 // or 24 entries of 8 B per lane (4, 8 or 12 KiB per wave); deeper entries go to scratch.
 template <int STACK, bool WIDE>
-hipError_t launch_batch_lds(uint32_t lds, const BatchArgs& a, bool lists, uint32_t blocks, hipStream_t stream) {
-  if (lds == 8) return launch_batch<STACK, WIDE, 8>(a, lists, blocks, stream);
-  if (lds == 24) return launch_batch<STACK, WIDE, 24>(a, lists, blocks, stream);
-  return launch_batch<STACK, WIDE, 16>(a, lists, blocks, stream);
+hipError_t launch_batch_lds(uint32_t lds, const BatchArgs& a, int spec, uint32_t blocks, hipStream_t stream) {
+  if (lds == 8) return launch_batch<STACK, WIDE, 8>(a, spec, blocks, stream);
+  if (lds == 24) return launch_batch<STACK, WIDE, 24>(a, spec, blocks, stream);
+  return launch_batch<STACK, WIDE, 16>(a, spec, blocks, stream);
 }
 
-hipError_t launch_batch_sel(const TravSel& ts, uint32_t lds, const BatchArgs& a, bool lists, uint32_t blocks,
+hipError_t launch_batch_sel(const TravSel& ts, uint32_t lds, const BatchArgs& a, int spec, uint32_t blocks,
                             hipStream_t stream) {
   if (ts.wide) {
-    if (ts.stack == 32) return launch_batch_lds<32, true>(lds, a, lists, blocks, stream);
-    if (ts.stack == 64) return launch_batch_lds<64, true>(lds, a, lists, blocks, stream);
-    return launch_batch_lds<128, true>(lds, a, lists, blocks, stream);
+    if (ts.stack == 32) return launch_batch_lds<32, true>(lds, a, spec, blocks, stream);
+    if (ts.stack == 64) return launch_batch_lds<64, true>(lds, a, spec, blocks, stream);
+    return launch_batch_lds<128, true>(lds, a, spec, blocks, stream);
   }
-  if (ts.stack == 32) return launch_batch_lds<32, false>(lds, a, lists, blocks, stream);
-  if (ts.stack == 64) return launch_batch_lds<64, false>(lds, a, lists, blocks, stream);
-  return launch_batch_lds<128, false>(lds, a, lists, blocks, stream);
+  if (ts.stack == 32) return launch_batch_lds<32, false>(lds, a, spec, blocks, stream);
+  if (ts.stack == 64) return launch_batch_lds<64, false>(lds, a, spec, blocks, stream);
+  return launch_batch_lds<128, false>(lds, a, spec, blocks, stream);
 }
 
 bool valid_lds_stack(uint32_t v) { return v == 8 || v == 16 || v == 24; }
@@ -454,11 +469,13 @@ int rmh_batch_create(const rmh_map* map, const rmh_batch_entry* entries, uint32_
   b->params.waves_per_cu = 32;
   b->params.tile_rows = kDefaultTileRows;
   b->params.lds_stack = kLdsStack;
+  b->params.launches = 1;
   if (params) {
     if (params->chunk_rays) b->params.chunk_rays = params->chunk_rays;
     if (params->waves_per_cu) b->params.waves_per_cu = params->waves_per_cu;
     if (params->tile_rows) b->params.tile_rows = params->tile_rows;
     if (params->lds_stack) b->params.lds_stack = params->lds_stack;
+    if (params->launches) b->params.launches = params->launches;
   }
   if (b->params.chunk_rays % 64 != 0 || b->params.chunk_rays > (1u << 20)) {
     batch_free(b);
@@ -471,6 +488,10 @@ int rmh_batch_create(const rmh_map* map, const rmh_batch_entry* entries, uint32_
   if (!valid_lds_stack(b->params.lds_stack)) {
     batch_free(b);
     return fail(RMH_ERR_INVALID, "rmh_batch_create: lds_stack must be 8, 16 or 24");
+  }
+  if (b->params.launches != 1 && b->params.launches != 2) {
+    batch_free(b);
+    return fail(RMH_ERR_INVALID, "rmh_batch_create: launches must be 1 or 2");
   }
   int dev = 0, ncu = 0;
   (void)hipGetDevice(&dev);
@@ -517,8 +538,11 @@ int rmh_batch_set_params(rmh_batch* b, const rmh_batch_params* params) {
   if (params->waves_per_cu) p.waves_per_cu = params->waves_per_cu;
   if (params->tile_rows) p.tile_rows = params->tile_rows;
   if (params->lds_stack) p.lds_stack = params->lds_stack;
+  if (params->launches) p.launches = params->launches;
   if (!valid_lds_stack(p.lds_stack))
     return fail(RMH_ERR_INVALID, "rmh_batch_set_params: lds_stack must be 8, 16 or 24");
+  if (p.launches != 1 && p.launches != 2)
+    return fail(RMH_ERR_INVALID, "rmh_batch_set_params: launches must be 1 or 2");
   if (p.chunk_rays % 64 != 0 || p.chunk_rays > (1u << 20))
     return fail(RMH_ERR_INVALID, "rmh_batch_set_params: chunk_rays must be a multiple of 64 (<= 2^20)");
   if (p.waves_per_cu > 64) return fail(RMH_ERR_INVALID, "rmh_batch_set_params: waves_per_cu must be <= 64");
@@ -553,21 +577,33 @@ int rmh_batch_simulate(rmh_batch* b, void* stream_) {
   if (ts.stack < 0) return fail(RMH_ERR_UNSUPPORTED, "rmh_batch_simulate: BVH deeper than 128 levels");
   hipStream_t stream = static_cast<hipStream_t>(stream_);
   const uint32_t waves = b->n_cu * b->params.waves_per_cu;
+  BatchArgs a;
+  a.map = make_mapdata(b->map);
+  a.entries = b->entries;
+  a.chunk_prefix = b->chunk_prefix;
+  a.chunk_rays = b->params.chunk_rays;
+  if (b->params.launches == 1) {
+    // one persistent launch over every entry (image entries first, then ray lists)
+    a.entry_lo = 0;
+    a.entry_hi = b->n_entries;
+    a.chunk_lo = 0;
+    a.chunk_hi = b->total_chunks;
+    a.counter = b->counter;
+    const uint32_t blocks = std::max<uint32_t>(1, std::min<uint32_t>(waves, b->total_chunks));
+    const hipError_t e = launch_batch_sel(ts, b->params.lds_stack, a, kSpecRuntime, blocks, stream);
+    if (e != hipSuccess) return hip_fail(e, "rmh_batch_simulate launch");
+    return RMH_OK;
+  }
   // two persistent launches in stream order: the image entries, then the ray lists
   for (int lists = 0; lists < 2; lists++) {
-    BatchArgs a;
-    a.map = make_mapdata(b->map);
-    a.entries = b->entries;
-    a.chunk_prefix = b->chunk_prefix;
     a.entry_lo = lists ? b->n_image_entries : 0u;
     a.entry_hi = lists ? b->n_entries : b->n_image_entries;
     a.chunk_lo = lists ? b->image_chunks : 0u;
     a.chunk_hi = lists ? b->total_chunks : b->image_chunks;
-    a.chunk_rays = b->params.chunk_rays;
     a.counter = b->counter + lists;
     if (a.chunk_hi == a.chunk_lo) continue;
     const uint32_t blocks = std::max<uint32_t>(1, std::min<uint32_t>(waves, a.chunk_hi - a.chunk_lo));
-    const hipError_t e = launch_batch_sel(ts, b->params.lds_stack, a, lists == 1, blocks, stream);
+    const hipError_t e = launch_batch_sel(ts, b->params.lds_stack, a, lists ? kSpecOn : kSpecOff, blocks, stream);
     if (e != hipSuccess) return hip_fail(e, "rmh_batch_simulate launch");
   }
   return RMH_OK;

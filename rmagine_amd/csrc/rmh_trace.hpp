@@ -147,7 +147,8 @@ __device__ __forceinline__ void test_leaf(const MapData& m, const TriRay& tr, in
 // lane (one column per lane); deeper entries go to a private per-lane array, which measured
 // rays rarely reach (a lane holds one entry per level only where both children were hit).
 // 8 rows = 4 KB of LDS per wave leave room for 8 resident waves per SIMD.
-//   CULL (classic walks): an entry is a subtree reference and the entry distance of its box
+//   CULL (classic walks, and the run-time mode of batched kernels): an entry is a subtree
+//     reference and the entry distance of its box
 //     (the slab test's lower bound, taken at the push), one row.  A pop skips entries whose
 //     box starts beyond the closest hit found since the push: every triangle below lies
 //     farther than that hit, so the subtree can neither win nor tie, and its node is never
@@ -228,10 +229,12 @@ constexpr int kDone = 0x7fffffff;
 
 // Walks interior nodes from `ref` until a leaf is reached (ref < 0) or the stack runs
 // dry (ref = kDone).
-// Speculation modes of trace(): never (classic while-while) or always.  It is a template
-// argument: holding both loop forms in one kernel costs registers (a batched kernel that did
-// so was 50 % slower), so the batched launch runs one kernel per mode.
-constexpr int kSpecOff = 0, kSpecOn = 1;
+// Speculation modes of trace(): never (classic while-while), always, or per a wave-uniform
+// run-time flag (one kernel for several sensor kinds: it takes the speculative loop form,
+// with the walk's leaf holding switched by the flag; a classic form in the same kernel as
+// well costs registers and made a batched kernel 50 % slower).  Stack entries carry entry
+// distances (pop skipping) in every mode but kSpecOn.
+constexpr int kSpecOff = 0, kSpecOn = 1, kSpecRuntime = 2;
 
 // Per-lane traversal counts of the diagnostic build of trace() (COUNT = true): node
 // visits through the vector and the scalar path, leaves, triangle tests, and (on the
@@ -276,7 +279,7 @@ __device__ __forceinline__ int pop_ref(int2 (*stk)[BLOCK], volatile int* spill, 
   return kDone;
 }
 
-// SPEC == kSpecOn: speculative walk (Aila and Laine, "Understanding the Efficiency of
+// SPEC == kSpecOn (or kSpecRuntime with spec): speculative walk (Aila and Laine, "Understanding the Efficiency of
 // Ray Traversal on GPUs", HPG 2009): a lane that reaches its first leaf holds it in `leaf`
 // and keeps walking; the wave leaves the loop once a quarter of its walking lanes hold a
 // leaf (on 2,000 x 32k random directions: 25.9 ms when waiting for all lanes, 24.1 for
@@ -286,7 +289,7 @@ __device__ __forceinline__ int pop_ref(int2 (*stk)[BLOCK], volatile int* spill, 
 // walk without it.
 template <int OCT, int STACK, int BLOCK, int SPEC, bool COUNT, int LDS>
 __device__ __forceinline__ void walk(const MapData& m, const SlabRay& r, float tbf, int2 (*stk)[BLOCK],
-                                     volatile int* spill, int& ref, int& sp, int& leaf, VisitCount* vc) {
+                                     volatile int* spill, int& ref, int& sp, int& leaf, bool spec, VisitCount* vc) {
   const int tid = threadIdx.x;
   while (ref >= 0 && ref != kDone) {
     float tmin0, tmax0, tmin1, tmax1;
@@ -324,19 +327,19 @@ __device__ __forceinline__ void walk(const MapData& m, const SlabRay& r, float t
       int near = nr.x, far = nr.y;
       float tfar = tmin1;
       if (tmin1 < tmin0) { near = nr.y; far = nr.x; tfar = tmin0; }
-      push_ref<STACK, BLOCK, LDS, SPEC == kSpecOff>(stk, spill, sp, far, tfar);
+      push_ref<STACK, BLOCK, LDS, SPEC != kSpecOn>(stk, spill, sp, far, tfar);
       ref = near;
     } else if (h0) {
       ref = nr.x;
     } else if (h1) {
       ref = nr.y;
     } else {
-      ref = pop_ref<STACK, BLOCK, LDS, SPEC == kSpecOff>(stk, spill, sp, tbf);
+      ref = pop_ref<STACK, BLOCK, LDS, SPEC != kSpecOn>(stk, spill, sp, tbf);
     }
-    if (SPEC == kSpecOn) {
+    if (SPEC == kSpecOn || (SPEC == kSpecRuntime && spec)) {
       if (ref < 0 && leaf == 0) {
         leaf = ref;
-        ref = pop_ref<STACK, BLOCK, LDS, SPEC == kSpecOff>(stk, spill, sp, tbf);
+        ref = pop_ref<STACK, BLOCK, LDS, SPEC != kSpecOn>(stk, spill, sp, tbf);
       }
       if (__popcll(__ballot(leaf != 0)) * 4 >= __popcll(__ballot(1))) break;
     }
@@ -455,14 +458,15 @@ __device__ __forceinline__ void walk4(const MapData& m, const SlabRay& r, float 
 // Closest hit along o + t d, 0 <= t <= range_max.  While-while traversal: a lane walks
 // interior nodes (near child first, far child onto its stack) until it reaches a leaf or
 // runs out of work, then the lanes holding leaves test their triangles together; with
-// SPEC == kSpecOn the walk is speculative (see walk()).  The stack never overflows:
+// SPEC == kSpecOn (or kSpecRuntime and spec) the walk is speculative (see walk()).  The
+// stack never overflows:
 // the host picks STACK >= the depth of the BVH.
 //
 // Waves whose 64 rays share one direction octant (almost all tiles) run the walk
 // specialised for it; mixed waves run the generic one.
 template <int STACK, int BLOCK, bool WIDE, int SPEC, bool COUNT = false, int LDS = kLdsStack>
 __device__ __forceinline__ void trace(const MapData& m, const V3& o, const V3& d, float range_max,
-                                      int2 (*stk)[BLOCK], Hit& h, VisitCount* vc = nullptr) {
+                                      int2 (*stk)[BLOCK], Hit& h, bool spec = false, VisitCount* vc = nullptr) {
   h.t = (double)range_max;
   h.slot = 0xFFFFFFFFu;
   h.face = 0xFFFFFFFFu;
@@ -493,27 +497,27 @@ __device__ __forceinline__ void trace(const MapData& m, const V3& o, const V3& d
   for (;;) {
     if constexpr (WIDE) {
       switch (woct) {
-        case 0: walk4<0, STACK, BLOCK, LDS, SPEC == kSpecOff>(m, r, tbf, stk, spill, ref, sp); break;
-        case 1: walk4<1, STACK, BLOCK, LDS, SPEC == kSpecOff>(m, r, tbf, stk, spill, ref, sp); break;
-        case 2: walk4<2, STACK, BLOCK, LDS, SPEC == kSpecOff>(m, r, tbf, stk, spill, ref, sp); break;
-        case 3: walk4<3, STACK, BLOCK, LDS, SPEC == kSpecOff>(m, r, tbf, stk, spill, ref, sp); break;
-        case 4: walk4<4, STACK, BLOCK, LDS, SPEC == kSpecOff>(m, r, tbf, stk, spill, ref, sp); break;
-        case 5: walk4<5, STACK, BLOCK, LDS, SPEC == kSpecOff>(m, r, tbf, stk, spill, ref, sp); break;
-        case 6: walk4<6, STACK, BLOCK, LDS, SPEC == kSpecOff>(m, r, tbf, stk, spill, ref, sp); break;
-        case 7: walk4<7, STACK, BLOCK, LDS, SPEC == kSpecOff>(m, r, tbf, stk, spill, ref, sp); break;
-        default: walk4<-1, STACK, BLOCK, LDS, SPEC == kSpecOff>(m, r, tbf, stk, spill, ref, sp); break;
+        case 0: walk4<0, STACK, BLOCK, LDS, SPEC != kSpecOn>(m, r, tbf, stk, spill, ref, sp); break;
+        case 1: walk4<1, STACK, BLOCK, LDS, SPEC != kSpecOn>(m, r, tbf, stk, spill, ref, sp); break;
+        case 2: walk4<2, STACK, BLOCK, LDS, SPEC != kSpecOn>(m, r, tbf, stk, spill, ref, sp); break;
+        case 3: walk4<3, STACK, BLOCK, LDS, SPEC != kSpecOn>(m, r, tbf, stk, spill, ref, sp); break;
+        case 4: walk4<4, STACK, BLOCK, LDS, SPEC != kSpecOn>(m, r, tbf, stk, spill, ref, sp); break;
+        case 5: walk4<5, STACK, BLOCK, LDS, SPEC != kSpecOn>(m, r, tbf, stk, spill, ref, sp); break;
+        case 6: walk4<6, STACK, BLOCK, LDS, SPEC != kSpecOn>(m, r, tbf, stk, spill, ref, sp); break;
+        case 7: walk4<7, STACK, BLOCK, LDS, SPEC != kSpecOn>(m, r, tbf, stk, spill, ref, sp); break;
+        default: walk4<-1, STACK, BLOCK, LDS, SPEC != kSpecOn>(m, r, tbf, stk, spill, ref, sp); break;
       }
     } else {
       switch (woct) {
-        case 0: walk<0, STACK, BLOCK, SPEC, COUNT, LDS>(m, r, tbf, stk, spill, ref, sp, leaf, vc); break;
-        case 1: walk<1, STACK, BLOCK, SPEC, COUNT, LDS>(m, r, tbf, stk, spill, ref, sp, leaf, vc); break;
-        case 2: walk<2, STACK, BLOCK, SPEC, COUNT, LDS>(m, r, tbf, stk, spill, ref, sp, leaf, vc); break;
-        case 3: walk<3, STACK, BLOCK, SPEC, COUNT, LDS>(m, r, tbf, stk, spill, ref, sp, leaf, vc); break;
-        case 4: walk<4, STACK, BLOCK, SPEC, COUNT, LDS>(m, r, tbf, stk, spill, ref, sp, leaf, vc); break;
-        case 5: walk<5, STACK, BLOCK, SPEC, COUNT, LDS>(m, r, tbf, stk, spill, ref, sp, leaf, vc); break;
-        case 6: walk<6, STACK, BLOCK, SPEC, COUNT, LDS>(m, r, tbf, stk, spill, ref, sp, leaf, vc); break;
-        case 7: walk<7, STACK, BLOCK, SPEC, COUNT, LDS>(m, r, tbf, stk, spill, ref, sp, leaf, vc); break;
-        default: walk<-1, STACK, BLOCK, SPEC, COUNT, LDS>(m, r, tbf, stk, spill, ref, sp, leaf, vc); break;
+        case 0: walk<0, STACK, BLOCK, SPEC, COUNT, LDS>(m, r, tbf, stk, spill, ref, sp, leaf, spec, vc); break;
+        case 1: walk<1, STACK, BLOCK, SPEC, COUNT, LDS>(m, r, tbf, stk, spill, ref, sp, leaf, spec, vc); break;
+        case 2: walk<2, STACK, BLOCK, SPEC, COUNT, LDS>(m, r, tbf, stk, spill, ref, sp, leaf, spec, vc); break;
+        case 3: walk<3, STACK, BLOCK, SPEC, COUNT, LDS>(m, r, tbf, stk, spill, ref, sp, leaf, spec, vc); break;
+        case 4: walk<4, STACK, BLOCK, SPEC, COUNT, LDS>(m, r, tbf, stk, spill, ref, sp, leaf, spec, vc); break;
+        case 5: walk<5, STACK, BLOCK, SPEC, COUNT, LDS>(m, r, tbf, stk, spill, ref, sp, leaf, spec, vc); break;
+        case 6: walk<6, STACK, BLOCK, SPEC, COUNT, LDS>(m, r, tbf, stk, spill, ref, sp, leaf, spec, vc); break;
+        case 7: walk<7, STACK, BLOCK, SPEC, COUNT, LDS>(m, r, tbf, stk, spill, ref, sp, leaf, spec, vc); break;
+        default: walk<-1, STACK, BLOCK, SPEC, COUNT, LDS>(m, r, tbf, stk, spill, ref, sp, leaf, spec, vc); break;
       }
     }
     if constexpr (SPEC == kSpecOff) {
@@ -524,7 +528,7 @@ __device__ __forceinline__ void trace(const MapData& m, const V3& o, const V3& d
       if constexpr (COUNT) { vc->leaves++; vc->tris += (~static_cast<uint32_t>(ref) & 7u) + 1u; }
       test_leaf(m, tr, ref, h);
       tbf = __double2float_ru(h.t);
-      ref = pop_ref<STACK, BLOCK, LDS, SPEC == kSpecOff>(stk, spill, sp, tbf);
+      ref = pop_ref<STACK, BLOCK, LDS, SPEC != kSpecOn>(stk, spill, sp, tbf);
       if (ref == kDone) break;
       continue;
     }
@@ -532,7 +536,7 @@ __device__ __forceinline__ void trace(const MapData& m, const V3& o, const V3& d
     // subtree, test the held leaf, then a leaf the walk stopped at after it
     if (ref < 0 && leaf == 0) {
       leaf = ref;
-      ref = pop_ref<STACK, BLOCK, LDS, SPEC == kSpecOff>(stk, spill, sp, tbf);
+      ref = pop_ref<STACK, BLOCK, LDS, SPEC != kSpecOn>(stk, spill, sp, tbf);
     }
     while (leaf < 0) {
       if constexpr (COUNT) { vc->leaves++; vc->tris += (~static_cast<uint32_t>(leaf) & 7u) + 1u; }
@@ -540,7 +544,7 @@ __device__ __forceinline__ void trace(const MapData& m, const V3& o, const V3& d
       tbf = __double2float_ru(h.t);
       if (ref < 0) {
         leaf = ref;
-        ref = pop_ref<STACK, BLOCK, LDS, SPEC == kSpecOff>(stk, spill, sp, tbf);
+        ref = pop_ref<STACK, BLOCK, LDS, SPEC != kSpecOn>(stk, spill, sp, tbf);
       } else {
         leaf = 0;
       }

@@ -392,8 +392,8 @@ def simulator_for(model, map: HipMap | None = None, device="cuda") -> SimulatorH
 
 
 class BatchSimulatorHip:
-    """Many sensors -- any mix of models, poses and ray counts -- cast by two persistent
-    launches (image sensors, then ray lists).
+    """Many sensors -- any mix of models, poses and ray counts -- cast in ONE persistent
+    launch (launches=2 splits it into image sensors, then ray lists).
 
     entries: list of dicts {"model": sensor model, "Tbm": (N,8) device tensor,
     "attrs": attribute names (default hits+ranges), "Tsb": optional (8,) pose}.
@@ -403,7 +403,7 @@ class BatchSimulatorHip:
     """
 
     def __init__(self, map: HipMap, entries, chunk_rays: int = 0, waves_per_cu: int = 0, tile_rows: int = 0,
-                 device="cuda", lds_stack: int = 0):
+                 device="cuda", lds_stack: int = 0, launches: int = 0):
         self._map = map
         self.device = torch.device(device)
         self._sensors, self._keep, self.bundles = [], [], []
@@ -427,7 +427,7 @@ class BatchSimulatorHip:
             self._keep.append(Tbm)
             self.bundles.append(b)
             self.total_rays += sh.W * sh.H * n
-        prm = T.rmh_batch_params(chunk_rays, waves_per_cu, tile_rows, lds_stack)
+        prm = T.rmh_batch_params(chunk_rays, waves_per_cu, tile_rows, lds_stack, launches)
         self._h = ctypes.c_void_p()
         _lib.call("rmh_batch_create", map.handle, table, len(entries), ctypes.byref(prm), ctypes.byref(self._h))
 
@@ -436,11 +436,12 @@ class BatchSimulatorHip:
         return self._map
 
     def set_params(self, chunk_rays: int = 0, waves_per_cu: int = 0, tile_rows: int = 0,
-                   lds_stack: int = 0) -> None:
+                   lds_stack: int = 0, launches: int = 0) -> None:
         """Launch shape: rays per work grab, persistent waves per CU, image rows per 64-ray
         tile, traversal-stack entries held in LDS (8, 16 or 24; deeper entries spill to
-        private memory) -- 0 keeps the current value."""
-        prm = T.rmh_batch_params(chunk_rays, waves_per_cu, tile_rows, lds_stack)
+        private memory), persistent launches (1: every entry; 2: image sensors, then ray
+        lists) -- 0 keeps the current value."""
+        prm = T.rmh_batch_params(chunk_rays, waves_per_cu, tile_rows, lds_stack, launches)
         _lib.call("rmh_batch_set_params", self._h, ctypes.byref(prm))
 
     def set_ray_order(self, sorted_: bool) -> None:
@@ -458,7 +459,8 @@ class BatchSimulatorHip:
         chunks = ctypes.c_uint32()
         _lib.call("rmh_batch_get_info", self._h, ctypes.byref(rays), ctypes.byref(chunks), ctypes.byref(prm))
         return {"chunk_rays": prm.chunk_rays, "waves_per_cu": prm.waves_per_cu, "tile_rows": prm.tile_rows,
-                "lds_stack": prm.lds_stack, "total_rays": rays.value, "total_chunks": chunks.value}
+                "lds_stack": prm.lds_stack, "launches": prm.launches, "total_rays": rays.value,
+                "total_chunks": chunks.value}
 
     def simulate(self, stream=None) -> list:
         _lib.call("rmh_batch_simulate", self._h, stream_handle(stream))

@@ -229,7 +229,7 @@ class rmh_batch_entry(ctypes.Structure):
 
 class rmh_batch_params(ctypes.Structure):
     _fields_ = [("chunk_rays", ctypes.c_uint32), ("waves_per_cu", ctypes.c_uint32), ("tile_rows", ctypes.c_uint32),
-                ("lds_stack", ctypes.c_uint32)]
+                ("lds_stack", ctypes.c_uint32), ("launches", ctypes.c_uint32)]
 
 
 assert ctypes.sizeof(rmh_cross_stats) == 128

@@ -9,7 +9,7 @@ import torch
 
 from rmagine_amd import autotune, scenes
 from rmagine_amd import types as T
-from rmagine_amd.simulation import BatchSimulatorHip, HipMap
+from rmagine_amd.simulation import BatchSimulatorHip, HipMap, simulator_for
 
 pytestmark = pytest.mark.gpu
 
@@ -33,7 +33,7 @@ def test_tune_batch_shape_and_layout(tmp_path, monkeypatch):
     batch = BatchSimulatorHip(hm, entries)
     before = _outputs(batch)
     rec = autotune.tune_batch(batch, key="room-test", chunks=(64, 256), waves=(8, 16), tile_rows=(2, 4),
-                              reps=1, layouts=(False, True))
+                              reps=1, layouts=(False, True), launches=(1, 2))
     assert not rec["cached"] and rec["wide"] in (False, True)
     assert set(rec["layout_ms"]) == {"binary", "wide"}
     assert rec["ms"] == min(rec["layout_ms"].values())
@@ -45,6 +45,7 @@ def test_tune_batch_shape_and_layout(tmp_path, monkeypatch):
     assert p["lds_stack"] == rec["lds_stack"]
     assert set(rec["ray_order_ms"]) == {"sorted", "list"}
     assert batch.ray_order_sorted() == rec["rays_sorted"]
+    assert rec["launches"] in (1, 2) and set(rec["launches_ms"]) == {1, 2} and p["launches"] == rec["launches"]
     after = _outputs(batch)
     for a, b in zip(before, after):
         for k in a:
@@ -52,14 +53,17 @@ def test_tune_batch_shape_and_layout(tmp_path, monkeypatch):
     stored = json.loads((tmp_path / "tune.json").read_text())["room-test"]
     assert stored["wide"] == rec["wide"] and stored["chunk_rays"] == rec["chunk_rays"]
     assert stored["lds_stack"] == rec["lds_stack"] and stored["rays_sorted"] == rec["rays_sorted"]
+    assert stored["launches"] == rec["launches"]
     # a fresh batch on a map in the other layout picks the cached choice up
     hm.set_layout(not rec["wide"])
     batch.set_params(chunk_rays=1024, waves_per_cu=32, tile_rows=8, lds_stack=24 if rec["lds_stack"] != 24 else 8)
     batch.set_ray_order(not rec["rays_sorted"])
+    batch.set_params(launches=3 - rec["launches"])
     again = autotune.tune_batch(batch, key="room-test", layouts=(False, True))
     assert again["cached"] and hm.wide == rec["wide"]
     assert batch.params()["chunk_rays"] == rec["chunk_rays"] and batch.params()["tile_rows"] == rec["tile_rows"]
     assert batch.params()["lds_stack"] == rec["lds_stack"] and batch.ray_order_sorted() == rec["rays_sorted"]
+    assert batch.params()["launches"] == rec["launches"]
     for a, b in zip(before, _outputs(batch)):
         for k in a:
             np.testing.assert_array_equal(a[k].view(np.uint8), b[k].view(np.uint8), err_msg=k)
@@ -84,3 +88,34 @@ def test_batch_lds_stack_depths_and_validation():
         with pytest.raises(Exception):
             batch.set_params(lds_stack=bad)
         assert batch.params()["lds_stack"] == 24
+
+
+def test_batch_one_or_two_launches():
+    """One persistent launch for every entry (the default) and the split into an image
+    kernel and a ray-list kernel write the same bytes; other launch counts are refused."""
+    V, F, P = scenes.room_workload(n_sensors=12)
+    hm = HipMap(V, F)
+    rng = np.random.default_rng(8)
+    d = rng.normal(size=(1500, 3))
+    o1 = T.O1DnModel(1500, 1, T.Interval(0.1, 60.0), (0.0, 0.0, 0.05),
+                     (d / np.linalg.norm(d, axis=1, keepdims=True)).astype(np.float32))
+    pin = T.PinholeModel(96, 64, T.Interval(0.05, 80.0), (50.0, 50.0), (47.5, 31.5))
+    models = (o1, scenes.vlp16_workload_model(), pin, o1)  # list entries before image entries
+    entries = [{"model": m, "Tbm": torch.as_tensor(P[3 * i:3 * i + 3], device="cuda"), "attrs": T.ATTRIBUTES}
+               for i, m in enumerate(models)]
+    batch = BatchSimulatorHip(hm, entries)
+    assert batch.params()["launches"] == 1
+    one = _outputs(batch)
+    batch.set_params(launches=2)
+    assert batch.params()["launches"] == 2
+    for a, b in zip(one, _outputs(batch)):
+        for k in a:
+            np.testing.assert_array_equal(a[k].view(np.uint8), b[k].view(np.uint8), err_msg=k)
+    for i, m in enumerate(models):  # and each entry equals its own simulate()
+        sim = simulator_for(m, hm)
+        ref = sim.simulate(torch.as_tensor(P[3 * i:3 * i + 3], device="cuda"), attrs=T.ATTRIBUTES).to_numpy()
+        for k in ref:
+            np.testing.assert_array_equal(one[i][k].view(np.uint8), ref[k].view(np.uint8), err_msg=f"entry {i} {k}")
+    with pytest.raises(Exception):
+        batch.set_params(launches=3)
+    assert batch.params()["launches"] == 2
