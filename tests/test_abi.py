@@ -1,6 +1,8 @@
 """The C-ABI library loads and exports every entry point include/rmh.h declares (CPU)."""
 import ctypes
 import re
+
+import pytest
 from pathlib import Path
 
 from rmagine_amd import _lib
@@ -43,3 +45,35 @@ def test_error_path_without_device():
     assert lib.rmh_simulate(None, None, None, None, 1, None, None) == 3  # no map
     assert b"no map" in lib.rmh_last_error()
     assert lib.rmh_sensor_create(7, ctypes.byref(T.rmh_spherical_model()), ctypes.byref(ctypes.c_void_p())) != 0
+
+
+def test_struct_layouts_match_header(tmp_path):
+    """Every ctypes mirror in rmagine_amd.types has the size and field offsets that a C
+    compiler gives the same struct from include/rmh.h."""
+    import shutil
+    import subprocess
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        pytest.skip("no C compiler")
+    structs = {name: cls for name, cls in vars(T).items()
+               if name.startswith("rmh_") and isinstance(cls, type) and issubclass(cls, ctypes.Structure)}
+    assert "rmh_batch_params" in structs and len(structs) >= 19
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "rmh.h"', 'int main(void) {']
+    for name, cls in sorted(structs.items()):
+        lines.append(f'  printf("{name} (sizeof) %zu\\n", sizeof({name}));')
+        for field, _ in cls._fields_:
+            lines.append(f'  printf("{name} {field} %zu\\n", offsetof({name}, {field}));')
+    lines += ['  return 0;', '}']
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    inc = Path(__file__).resolve().parents[1] / "include"
+    subprocess.run([cc, "-std=c99", f"-I{inc}", str(src), "-o", str(exe)], check=True, capture_output=True)
+    got = {}
+    for line in subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.splitlines():
+        parts = line.split()
+        got[tuple(parts[:-1])] = int(parts[-1])
+    for name, cls in structs.items():
+        assert got[(name, "(sizeof)")] == ctypes.sizeof(cls), name
+        for field, _ in cls._fields_:
+            assert got[(name, field)] == getattr(cls, field).offset, f"{name}.{field}"
