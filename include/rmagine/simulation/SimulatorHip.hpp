@@ -122,6 +122,11 @@ class SimulatorHip {
     if (m_map) RM_RMH_CHECK(rmh_stream_synchronize(m_map->stream()));
   }
 
+  // the librmh sensor of the current model (shared: a batch keeps it alive) and Tsb, for
+  // BatchSimulatorHip and poseGradHip
+  std::shared_ptr<rmh_sensor> sensorHandle() const { return m_sensor; }
+  const Transform& getTsb() const { return m_Tsb; }
+
  protected:
   void setSensor(int kind, const void* model) {
     rmh_sensor* s = nullptr;

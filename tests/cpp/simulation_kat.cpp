@@ -8,16 +8,20 @@
 //   pinhole     tests/embree/simulation_pinhole.cpp:28-79 (point (0.5, 0.5, 0.375))
 //   ondn        tests/embree/simulation_ondn.cpp:30-76 (range 0.5)
 //   single ray  tests/embree/simulation_single_ray.cpp:24-70 (range 0.5)
-// plus the error convention (no map -> std::runtime_error).  Throws on failure.
+// plus the error convention (no map -> std::runtime_error), and the batched, graph and
+// pose-gradient forms (BatchSimulatorHip, HipGraph, poseGradHip; no reference counterpart)
+// against the plain simulators.  Throws on failure.
 #include <rmagine/map/HipMap.hpp>
 #include <rmagine/noise/NoiseHip.hpp>
 #include <rmagine/types/ouster_sensors.h>
+#include <rmagine/simulation/BatchSimulatorHip.hpp>
 #include <rmagine/simulation/O1DnSimulatorHip.hpp>
 #include <rmagine/simulation/OnDnSimulatorHip.hpp>
 #include <rmagine/simulation/PinholeSimulatorHip.hpp>
 #include <rmagine/simulation/SphereSimulatorHip.hpp>
 
 #include <cmath>
+#include <cstring>
 #include <iostream>
 #include <sstream>
 #include <string>
@@ -197,6 +201,92 @@ static void test_noise(const HipMapPtr& map) {
   }
 }
 
+using HitsRanges = Bundle<Hits<VRAM_HIP>, Ranges<VRAM_HIP>>;
+
+static bool same_results(const HitsRanges& a, const HitsRanges& b) {
+  Memory<uint8_t, RAM> ha = a.hits, hb = b.hits;
+  Memory<float, RAM> ra = a.ranges, rb = b.ranges;
+  if (ha.size() != hb.size() || ra.size() != rb.size()) return false;
+  for (size_t i = 0; i < ha.size(); i++)
+    if (ha[i] != hb[i]) return false;
+  return std::memcmp(ra.raw(), rb.raw(), sizeof(float) * ra.size()) == 0;
+}
+
+static void scramble(HitsRanges& b) {
+  Memory<uint8_t, RAM> h(b.hits.size());
+  Memory<float, RAM> r(b.ranges.size());
+  for (size_t i = 0; i < h.size(); i++) { h[i] = 7; r[i] = -1.0f; }
+  b.hits = h;
+  b.ranges = r;
+}
+
+// Batched bundles equal each simulator's own simulate(), in one persistent launch and in
+// two; a captured graph replays the same bytes; the pose loss of a scan against itself is 0.
+static void test_batch(const HipMapPtr& map) {
+  SphereSimulatorHip s1(map);
+  s1.setModel(example_spherical());
+  PinholeSimulatorHip s2(map);
+  s2.setModel(example_pinhole());
+  O1DnSimulatorHip s3(map);
+  s3.setModel(example_o1dn());
+  Memory<Transform, RAM> Th(5);
+  for (size_t i = 0; i < Th.size(); i++) {
+    Th[i] = Transform::Identity();
+    Th[i].t = Vector{0.05f * i - 0.1f, 0.02f * i, -0.03f * i};
+  }
+  Memory<Transform, VRAM_HIP> T;
+  T = Th;
+  const SimulatorHip* sims[3] = {&s1, &s2, &s3};
+  HitsRanges ref[3], got[3];
+  for (int k = 0; k < 3; k++) {
+    ref[k] = (k == 0 ? s1.simulate<HitsRanges>(T) : k == 1 ? s2.simulate<HitsRanges>(T) : s3.simulate<HitsRanges>(T));
+    resize_memory_bundle<VRAM_HIP>(got[k], sims[k]->getWidth(), sims[k]->getHeight(), static_cast<unsigned>(T.size()));
+  }
+  s1.synchronize();
+  BatchSimulatorHip batch(map);
+  batch.add(s3, T, got[2]);  // a ray list before the image sensors
+  batch.add(s1, T, got[0]);
+  batch.add(s2, T, got[1]);
+  batch.commit();
+  require(batch.params().launches == 1, "batch: one persistent launch by default");
+  require(batch.totalRays() == T.size() * (s1.getWidth() * s1.getHeight() + s2.getWidth() * s2.getHeight() +
+                                           s3.getWidth() * s3.getHeight()),
+          "batch: total rays");
+  for (uint32_t launches : {1u, 2u}) {
+    BatchParamsHip p;
+    p.launches = launches;
+    batch.setParams(p);
+    for (auto& g : got) scramble(g);
+    batch.simulate();
+    s1.synchronize();
+    for (int k = 0; k < 3; k++) require(same_results(got[k], ref[k]), "batch entry equals its simulate()");
+  }
+  void* stream = nullptr;
+  RM_RMH_CHECK(rmh_stream_create(&stream));
+  {
+    HipGraph graph = HipGraph::capture(stream, [&] { batch.simulate(stream); });
+    for (int rep = 0; rep < 2; rep++) {
+      for (auto& g : got) scramble(g);
+      graph.launch(stream);
+      RM_RMH_CHECK(rmh_stream_synchronize(stream));
+      for (int k = 0; k < 3; k++) require(same_results(got[k], ref[k]), "graph replay equals simulate()");
+    }
+  }
+  RM_RMH_CHECK(rmh_stream_destroy(stream));
+  // pose loss of pose 0's own scan: only the FP32 rounding of the target ranges remains
+  // (|e| < 1e-7 m per ray), every hit ray is used
+  const size_t n = static_cast<size_t>(s1.getWidth()) * s1.getHeight();
+  Memory<double, VRAM_HIP> out(8);
+  poseGradHip(s1, T(0, 1), ref[0].ranges(0, n), ref[0].hits(0, n), false, 0.05f, out);
+  s1.synchronize();
+  Memory<double, RAM> o = out;
+  Memory<uint8_t, RAM> h = ref[0].hits(0, n);
+  size_t hits = 0;
+  for (size_t i = 0; i < n; i++) hits += h[i] != 0;
+  require(o[0] >= 0.0 && o[0] < 1e-10 && o[7] == static_cast<double>(hits) && hits > 0,
+          "pose loss of a scan against itself");
+}
+
 int main(int argc, char** argv) {
   const HipMapPtr map = make_cube_map();
   test_spherical(map);
@@ -205,6 +295,7 @@ int main(int argc, char** argv) {
   test_ondn(map);
   test_no_map();
   test_noise(map);
+  test_batch(map);
   // the same known answers through the 4-wide quantised node layout
   require(map->setLayout(true), "4-wide layout of the cube map");
   test_spherical(map);
