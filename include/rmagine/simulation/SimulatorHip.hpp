@@ -122,6 +122,13 @@ class SimulatorHip {
     if (m_map) RM_RMH_CHECK(rmh_stream_synchronize(m_map->stream()));
   }
 
+  // O1Dn/OnDn: cast the rays in direction-sorted order (the default) or in list order; the
+  // results are identical, only the speed differs (image models ignore it)
+  void setRayOrder(bool sorted) {
+    if (!m_sensor) throw std::runtime_error("[SimulatorHip] setRayOrder(): no sensor model set");
+    RM_RMH_CHECK(rmh_sensor_set_ray_order(m_sensor.get(), sorted ? 1 : 0));
+  }
+
   // the librmh sensor of the current model (shared: a batch keeps it alive) and Tsb, for
   // BatchSimulatorHip and poseGradHip
   std::shared_ptr<rmh_sensor> sensorHandle() const { return m_sensor; }
