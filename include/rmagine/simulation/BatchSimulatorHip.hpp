@@ -27,7 +27,7 @@
 namespace rmagine {
 
 // rmh_batch_params; 0 keeps the current value (defaults: 256 rays per grab, 32 waves per
-// compute unit, 4 image rows per tile, 8 LDS stack rows, one persistent launch)
+// compute unit, 4 image rows per tile, 10 LDS stack rows, one persistent launch)
 struct BatchParamsHip {
   uint32_t chunk_rays = 0;
   uint32_t waves_per_cu = 0;

@@ -270,8 +270,8 @@ typedef struct {
   uint32_t tile_rows;     /* image rows per 64-ray tile of spherical/pinhole sensors:
                              1, 2, 4, 8, 16, 32 or 64, capped by the image height
                              (0 = keep / default 4)                                      */
-  uint32_t lds_stack;     /* traversal stack entries per lane kept in LDS: 8, 16 or 24
-                             (8 B each; 0 = keep / default 8); deeper entries go to
+  uint32_t lds_stack;     /* traversal stack entries per lane kept in LDS: 8, 10 or 16
+                             (8 B each; 0 = keep / default 10); deeper entries go to
                              scratch                                                     */
   uint32_t launches;      /* persistent launches per rmh_batch_simulate: 1 = every entry in
                              one launch (0 = keep / default), 2 = image sensors, then ray

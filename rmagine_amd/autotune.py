@@ -9,7 +9,7 @@ registers run out; the tile shape decides how coherent a wave's rays are (on a s
 lidar wide tiles win because rays of one row end at similar ranges).  The tuner times
 every candidate on the caller's actual batch with HIP events and keeps the fastest:
 first chunk x waves at the default tile, then the tile rows at the best pair, then the
-depth of the traversal stack kept in LDS (`lds_stack`: 8, 16 or 24 rows of 8 B per ray; a
+depth of the traversal stack kept in LDS (`lds_stack`: 8, 10 or 16 rows of 8 B per ray; a
 shallower one leaves more LDS for resident waves, a deeper one spills less to private
 memory), then the ray order of the O1Dn/OnDn sensors (sorted by direction, or the list
 order the caller gave), then (when asked with launches=(1, 2)) one persistent launch for
@@ -34,7 +34,7 @@ import torch
 DEFAULT_CHUNKS = (64, 128, 256, 512, 1024)
 DEFAULT_WAVES = (8, 16, 24, 32)
 DEFAULT_TILE_ROWS = (1, 2, 4, 8, 16)
-DEFAULT_LDS_STACKS = (8, 16, 24)
+DEFAULT_LDS_STACKS = (8, 10, 16)
 DEFAULT_RAY_ORDERS = (True, False)  # True = direction-sorted
 
 

@@ -287,13 +287,14 @@ hipError_t launch_batch(const BatchArgs& a, int spec, uint32_t blocks, hipStream
   return hipGetLastError();
 }
 
-// LDS stack depth of the batched kernel (a launch parameter the autotuner chooses): 8, 16
-// or 24 entries of 8 B per lane (4, 8 or 12 KiB per wave); deeper entries go to scratch.
+// LDS stack depth of the batched kernel (a launch parameter the autotuner chooses): 8, 10
+// or 16 entries of 8 B per lane (4, 5 or 8 KiB per wave; 16 halves the resident waves);
+// deeper entries go to scratch.
 template <int STACK, bool WIDE>
 hipError_t launch_batch_lds(uint32_t lds, const BatchArgs& a, int spec, uint32_t blocks, hipStream_t stream) {
   if (lds == 8) return launch_batch<STACK, WIDE, 8>(a, spec, blocks, stream);
-  if (lds == 24) return launch_batch<STACK, WIDE, 24>(a, spec, blocks, stream);
-  return launch_batch<STACK, WIDE, 16>(a, spec, blocks, stream);
+  if (lds == 16) return launch_batch<STACK, WIDE, 16>(a, spec, blocks, stream);
+  return launch_batch<STACK, WIDE, 10>(a, spec, blocks, stream);
 }
 
 hipError_t launch_batch_sel(const TravSel& ts, uint32_t lds, const BatchArgs& a, int spec, uint32_t blocks,
@@ -308,7 +309,7 @@ hipError_t launch_batch_sel(const TravSel& ts, uint32_t lds, const BatchArgs& a,
   return launch_batch_lds<128, false>(lds, a, spec, blocks, stream);
 }
 
-bool valid_lds_stack(uint32_t v) { return v == 8 || v == 16 || v == 24; }
+bool valid_lds_stack(uint32_t v) { return v == 8 || v == 10 || v == 16; }
 
 }  // namespace
 
@@ -487,7 +488,7 @@ int rmh_batch_create(const rmh_map* map, const rmh_batch_entry* entries, uint32_
   }
   if (!valid_lds_stack(b->params.lds_stack)) {
     batch_free(b);
-    return fail(RMH_ERR_INVALID, "rmh_batch_create: lds_stack must be 8, 16 or 24");
+    return fail(RMH_ERR_INVALID, "rmh_batch_create: lds_stack must be 8, 10 or 16");
   }
   if (b->params.launches != 1 && b->params.launches != 2) {
     batch_free(b);
@@ -540,7 +541,7 @@ int rmh_batch_set_params(rmh_batch* b, const rmh_batch_params* params) {
   if (params->lds_stack) p.lds_stack = params->lds_stack;
   if (params->launches) p.launches = params->launches;
   if (!valid_lds_stack(p.lds_stack))
-    return fail(RMH_ERR_INVALID, "rmh_batch_set_params: lds_stack must be 8, 16 or 24");
+    return fail(RMH_ERR_INVALID, "rmh_batch_set_params: lds_stack must be 8, 10 or 16");
   if (p.launches != 1 && p.launches != 2)
     return fail(RMH_ERR_INVALID, "rmh_batch_set_params: launches must be 1 or 2");
   if (p.chunk_rays % 64 != 0 || p.chunk_rays > (1u << 20))

@@ -146,7 +146,8 @@ __device__ __forceinline__ void test_leaf(const MapData& m, const TriRay& tr, in
 // Traversal stack.  The top of every lane's stack lives in LDS, LDS rows of 8 bytes per
 // lane (one column per lane); deeper entries go to a private per-lane array, which measured
 // rays rarely reach (a lane holds one entry per level only where both children were hit).
-// 8 rows = 4 KB of LDS per wave leave room for 8 resident waves per SIMD.
+// 10 rows = 5 KB of LDS per wave: 32 resident waves (8 per SIMD, the most a CU holds) fill
+// the 160 KB of a CU exactly (terrain 9.44 -> 9.38 ms against 8 rows).
 //   CULL (classic walks, and the run-time mode of batched kernels): an entry is a subtree
 //     reference and the entry distance of its box
 //     (the slab test's lower bound, taken at the push), one row.  A pop skips entries whose
@@ -155,7 +156,7 @@ __device__ __forceinline__ void test_leaf(const MapData& m, const TriRay& tr, in
 //     loaded (terrain: 5.7 % of node visits; 10,000 particles in a room: 7.4 %).
 //   otherwise (speculative walks): an entry is a reference, two per row.  Their random
 //     rays keep deeper stacks and could skip 2 % of visits: the deeper LDS stack pays more.
-constexpr int kLdsStack = 8;
+constexpr int kLdsStack = 10;
 
 // pushes a subtree (and, when culling, its entry distance)
 template <int STACK, int BLOCK, int LDS, bool CULL>

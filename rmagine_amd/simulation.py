@@ -438,7 +438,7 @@ class BatchSimulatorHip:
     def set_params(self, chunk_rays: int = 0, waves_per_cu: int = 0, tile_rows: int = 0,
                    lds_stack: int = 0, launches: int = 0) -> None:
         """Launch shape: rays per work grab, persistent waves per CU, image rows per 64-ray
-        tile, traversal-stack entries held in LDS (8, 16 or 24; deeper entries spill to
+        tile, traversal-stack entries held in LDS (8, 10 or 16; deeper entries spill to
         private memory), persistent launches (1: every entry; 2: image sensors, then ray
         lists) -- 0 keeps the current value."""
         prm = T.rmh_batch_params(chunk_rays, waves_per_cu, tile_rows, lds_stack, launches)

@@ -41,7 +41,7 @@ def test_tune_batch_shape_and_layout(tmp_path, monkeypatch):
     assert (p["chunk_rays"], p["waves_per_cu"], p["tile_rows"]) == (rec["chunk_rays"], rec["waves_per_cu"], rec["tile_rows"])
     assert hm.wide == rec["wide"]
     assert len(rec["trials"]) == 2 * 2 + 1
-    assert rec["lds_stack"] in (8, 16, 24) and set(rec["lds_stack_ms"]) == {8, 16, 24}
+    assert rec["lds_stack"] in (8, 10, 16) and set(rec["lds_stack_ms"]) == {8, 10, 16}
     assert p["lds_stack"] == rec["lds_stack"]
     assert set(rec["ray_order_ms"]) == {"sorted", "list"}
     assert batch.ray_order_sorted() == rec["rays_sorted"]
@@ -56,7 +56,7 @@ def test_tune_batch_shape_and_layout(tmp_path, monkeypatch):
     assert stored["launches"] == rec["launches"]
     # a fresh batch on a map in the other layout picks the cached choice up
     hm.set_layout(not rec["wide"])
-    batch.set_params(chunk_rays=1024, waves_per_cu=32, tile_rows=8, lds_stack=24 if rec["lds_stack"] != 24 else 8)
+    batch.set_params(chunk_rays=1024, waves_per_cu=32, tile_rows=8, lds_stack=16 if rec["lds_stack"] != 16 else 8)
     batch.set_ray_order(not rec["rays_sorted"])
     batch.set_params(launches=3 - rec["launches"])
     again = autotune.tune_batch(batch, key="room-test", layouts=(False, True))
@@ -78,16 +78,16 @@ def test_batch_lds_stack_depths_and_validation():
     batch = BatchSimulatorHip(hm, entries, lds_stack=8)
     assert batch.params()["lds_stack"] == 8
     ref = _outputs(batch)
-    for d in (16, 24):
+    for d in (10, 16):
         batch.set_params(lds_stack=d)
         assert batch.params()["lds_stack"] == d
         for a, b in zip(ref, _outputs(batch)):
             for k in a:
                 np.testing.assert_array_equal(a[k].view(np.uint8), b[k].view(np.uint8), err_msg=f"{k} lds={d}")
-    for bad in (4, 12, 32):
+    for bad in (4, 12, 24, 32):
         with pytest.raises(Exception):
             batch.set_params(lds_stack=bad)
-        assert batch.params()["lds_stack"] == 24
+        assert batch.params()["lds_stack"] == 16
 
 
 def test_batch_one_or_two_launches():
