@@ -32,16 +32,17 @@ HBM_PEAK_GBS = 8000.0
 TRAFFIC_PROFILE = os.path.join("profiles", "k_simulate_terrain_1000x128x1024.md")
 
 
-def profiled_traffic(default_workload: bool):
-    """HBM-side bytes per launch measured with rocprofv3 for the default workload (the
-    profile summary records them on a `traffic_bytes_per_launch:` line), or None."""
+def profiled(default_workload: bool, key: str, cast):
+    """A figure measured with rocprofv3 for the default workload, recorded in the profile
+    summary on a `key: value` line (`traffic_bytes_per_launch`: HBM-side bytes per launch;
+    `l2_hit_rate`: TCC hits / (hits + misses)), or None."""
     if not default_workload:
         return None
     try:
         with open(os.path.join(ROOT, TRAFFIC_PROFILE)) as f:
             for line in f:
-                if line.startswith("traffic_bytes_per_launch:"):
-                    return int(line.split(":", 1)[1].strip())
+                if line.startswith(key + ":"):
+                    return cast(line.split(":", 1)[1].strip())
     except (OSError, ValueError):
         pass
     return None
@@ -497,7 +498,9 @@ def main():
     if not args.no_cpu:
         cpu, accuracy = cpu_baseline(V, F, model, Tbm, args.cpu_seconds, gpu_numpy_for_pose)
 
-    traffic = profiled_traffic(args.grid == 1024 and args.sensors == 1000)
+    default_workload = args.grid == 1024 and args.sensors == 1000
+    traffic = profiled(default_workload, "traffic_bytes_per_launch", int)
+    l2_hit = profiled(default_workload, "l2_hit_rate", float)
     hbm_measured = measure_hbm_gbs()
     log(f"hbm: plain device copy {hbm_measured:.0f} GB/s (read + write)")
     out = {
@@ -527,6 +530,11 @@ def main():
             "frac": round(achieved / HBM_PEAK_GBS, 5),
             "traffic": traffic,
             "traffic_source": TRAFFIC_PROFILE if traffic else None,
+            # the profiled HBM-side bytes over this run's kernel time (fetches include lines
+            # the Infinity Cache serves, so this bounds the HBM rate from above)
+            "traffic_gbs": round(traffic / avg_kernel_s / 1e9, 1) if traffic else None,
+            "traffic_frac": round(traffic / avg_kernel_s / 1e9 / HBM_PEAK_GBS, 4) if traffic else None,
+            "l2_hit_rate": l2_hit,
             "kernel": "k_simulate<spherical> (traversal + attribute writers)",
             "kernel_avg_ms": round(avg_kernel_s * 1e3, 4),
             "alg_bytes_per_launch": alg_bytes,
