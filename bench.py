@@ -86,7 +86,8 @@ def cpu_baseline(V, F, model, Tbm, seconds, gpu_numpy_for_pose):
     elapsed = 0.0
     poses = 0
     per_pose = []
-    worst = {"range_max_err": 0.0, "id_mismatch": 0, "hit_mismatch": 0, "ambiguous": 0, "rays": 0}
+    worst = {"range_max_err": 0.0, "point_max_err": 0.0, "normal_max_err": 0.0, "nan_mismatch": 0,
+             "id_mismatch": 0, "geom_object_id_mismatch": 0, "hit_mismatch": 0, "ambiguous": 0, "rays": 0}
     oracle.simulate(om, model, Tbm[:1], threads=threads, flags=False)  # warm-up, not timed
     # the reference's benchmark loop (velodyne_benchmark.hpp:92-114): run for at least
     # `seconds`, report the running mean; poses are cast in order and, once all are done,
@@ -104,6 +105,17 @@ def cpu_baseline(V, F, model, Tbm, seconds, gpu_numpy_for_pose):
             worst["id_mismatch"] += int((g["face_ids"] != o["face_ids"]).sum())
             worst["range_max_err"] = max(worst["range_max_err"],
                                          float(np.abs(g["ranges"].astype(np.float64) - o["ranges"]).max()))
+            for key in ("points", "normals"):
+                # misses hold NaN on both sides: finite values by their largest difference,
+                # the NaN pattern exactly
+                fg, fo = np.isfinite(g[key]), np.isfinite(o[key])
+                worst["nan_mismatch"] += int((fg != fo).sum())
+                both = fg & fo
+                if both.any():
+                    err = float(np.abs(g[key][both].astype(np.float64) - o[key][both]).max())
+                    worst[key[:-1] + "_max_err"] = max(worst[key[:-1] + "_max_err"], err)
+            worst["geom_object_id_mismatch"] += int((g["geom_ids"] != o["geom_ids"]).sum()
+                                                    + (g["object_ids"] != o["object_ids"]).sum())
             worst["rays"] += len(o["ranges"])
         rays += len(o["ranges"])
         poses += 1
@@ -491,7 +503,10 @@ def main():
     def gpu_numpy_for_pose(p):
         sl = slice(p * per_pose, (p + 1) * per_pose)
         return {"hits": res.hits[sl].cpu().numpy(), "ranges": res.ranges[sl].cpu().numpy(),
-                "face_ids": res.face_ids[sl].cpu().numpy().view(np.uint32)}
+                "points": res.points[sl].cpu().numpy(), "normals": res.normals[sl].cpu().numpy(),
+                "face_ids": res.face_ids[sl].cpu().numpy().view(np.uint32),
+                "geom_ids": res.geom_ids[sl].cpu().numpy().view(np.uint32),
+                "object_ids": res.object_ids[sl].cpu().numpy().view(np.uint32)}
 
     cpu = None
     accuracy = None
