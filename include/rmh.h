@@ -140,7 +140,9 @@ int rmh_device_synchronize(void);
  * offsets[g] <= f < offsets[g+1] reports geom_id = object_id = g and face_id =
  * f - offsets[g], which is what Embree reports for meshes attached to one scene
  * (EmbreeScene.cpp:54-71).  The map copies the mesh; the inputs may be freed after the
- * call returns.  The build runs on `stream` and is synchronised before returning. */
+ * call returns.  The build runs on `stream` and is synchronised before returning.
+ * At most 2^26 faces (the ray kernels address BVH nodes with 32-bit byte offsets):
+ * larger maps are refused with RMH_ERR_UNSUPPORTED before any device work. */
 int rmh_map_build(const float* vertices_dev, uint32_t n_vertices,
                   const uint32_t* faces_dev, uint32_t n_faces,
                   const uint32_t* mesh_face_offsets, uint32_t n_meshes,
@@ -353,9 +355,19 @@ int rmh_quat_normalize(rmh_transform* T_dev, uint32_t n, void* stream);
 /* T[i] = identity (setIdentity_kernel, memory_math.cu:1342-1352); a kernel, never a memset,
  * so it can be captured into a graph */
 int rmh_transform_set_identity(rmh_transform* T_dev, uint32_t n, void* stream);
+/* pack (src/rmagine_cuda/src/math/memory_math.cu:1258-1279, kernels :313-339): T[i].t = t[i]
+ * and T[i].R from R_dev[i], a rotation matrix of 9 floats in column-major order (Rmagine's
+ * Matrix3x3 layout) converted with the reference's FP32 matrix -> quaternion steps
+ * (MatrixOps.tcc:1492-1529, no normalisation), or from q_dev[i]; T[i].stamp is left as it
+ * was.  Device arrays of n entries. */
+int rmh_transform_pack_matrix(const float* R_dev, const rmh_vec3* t_dev, rmh_transform* T_dev, uint32_t n,
+                              void* stream);
+int rmh_transform_pack_quat(const rmh_quat* q_dev, const rmh_vec3* t_dev, rmh_transform* T_dev, uint32_t n,
+                            void* stream);
 
-/* Workspace (device bytes) that rmh_vec3_moments / rmh_statistics need for this shape when
- * a segment is split over several blocks; 0 when none is needed. */
+/* Workspace (device bytes) that rmh_vec3_moments / rmh_statistics / rmh_linear_system_p2l
+ * need for this shape when a segment is split over several blocks (one size serves all
+ * three); 0 when none is needed. */
 int rmh_reduction_workspace_size(uint64_t segment_size, uint32_t n_segments, size_t* bytes);
 
 /* Segmented FP64 moments of n_segments consecutive segments of segment_size vectors:
@@ -408,6 +420,50 @@ int rmh_statistics(int kind, const rmh_transform* pre_transforms_dev, uint32_t n
                    const rmh_cloud* dataset, const rmh_cloud* model, uint64_t segment_size,
                    uint32_t n_segments, const rmh_umeyama_constraints* params,
                    rmh_cross_stats* stats_dev, void* workspace, size_t workspace_bytes, void* stream);
+
+/* rmagine::AABB (src/rmagine_core/include/rmagine/math/types/AABB.hpp:50-53), 24 bytes */
+typedef struct { rmh_vec3 min, max; } rmh_aabb;
+
+/* Objectwise point-to-plane statistics (statistics_objectwise_p2l,
+ * src/rmagine_cuda/src/math/statistics.cu:278-371,422-470; declared statistics.cuh:139-158),
+ * FP64: one rmh_cross_stats per object o < n_objects over image-shaped scans of n_points
+ * elements with row stride `width` (height is accepted as in the reference and unused).
+ * bboxes_dev[o] is the object's pixel window (min.x/max.x columns, min.y/max.y rows, max
+ * exclusive; the floats are converted to unsigned like the reference's assignments, NaN and
+ * negatives giving 0); window element k (row-major, k < w*h) is scan element
+ * (min.y + k / w) * width + min.x + k % w, used while below n_points.  The reference's kernel
+ * runs k up to w*h rounded up to a multiple of 512, into the rows below the window; that
+ * element set is the default, RMH_OBJECTWISE_WINDOW_ONLY stops at w*h.  An inverted window
+ * is empty.  Each element passes the mask filters, the id filters and the plane-distance
+ * test of params_dev[o] under pre_transforms_dev[o], as in rmh_statistics (p2l).  One
+ * workgroup per object; no workspace. */
+#define RMH_OBJECTWISE_WINDOW_ONLY 1u
+int rmh_statistics_objectwise_p2l(const rmh_transform* pre_transforms_dev, const rmh_cloud* dataset,
+                                  const rmh_cloud* model, uint32_t width, uint32_t height, uint32_t n_points,
+                                  const rmh_umeyama_constraints* params_dev, const rmh_aabb* bboxes_dev,
+                                  uint32_t n_objects, uint32_t flags, rmh_cross_stats* stats_dev, void* stream);
+
+/* Point-to-plane Gauss-Newton system per segment: build_linear_system_p2l
+ * (src/rmagine_core/include/rmagine/math/optimization.h:170-200) with
+ * jacobian_and_residual_p2l (:113-137), in FP64.  With Pd = pre_transform * dataset point,
+ * Pm, Nm the model point and normal: r = (Pd - Pm) . Nm, J = [Pd x Nm, Nm] (the derivative
+ * of r for the left perturbation Pd -> Exp(omega) Pd + tau, unknowns (omega, tau)), weight
+ * w = GM_weight(s, r) = s^2 / (s + r)^2 with s = kernel_scale (the reference passes the signed
+ * residual and s = 5; kernel_scale <= 0 gives w = 1).  Sums J^T w J, J^T w r = the gradient
+ * of (1/2) sum w r^2 with w held fixed, and sum w r^2 over the correspondences that pass the
+ * mask/id filters and |r| < max_dist (rmh_statistics' filters).  Segments, pre-transforms and
+ * the workspace as in rmh_statistics. */
+typedef struct {
+  double   JTwJ[36];  /* row-major 6x6 (symmetric) */
+  double   JTwr[6];
+  double   wr2;       /* sum of w r^2 */
+  uint32_t n_meas;
+  uint32_t _pad;
+} rmh_p2l_system;
+int rmh_linear_system_p2l(const rmh_transform* pre_transforms_dev, uint32_t n_pre, const rmh_cloud* dataset,
+                          const rmh_cloud* model, uint64_t segment_size, uint32_t n_segments,
+                          const rmh_umeyama_constraints* params, double kernel_scale, rmh_p2l_system* out_dev,
+                          void* workspace, size_t workspace_bytes, void* stream);
 
 /* Umeyama per statistics entry (umeyama_transform, src/rmagine_core/src/math/optimization.cpp:12-55;
  * src/rmagine_cuda/src/math/linalg.cu:1150-1190) in FP64: out_dev receives FP32 transforms,

@@ -41,6 +41,8 @@ using namespace rmh;
 constexpr int kBlock = 256;
 constexpr uint32_t kLeafBit = kTreeLeafBit;
 constexpr int kTreeletRounds = 3;
+// largest map: node references must stay below 2^26 (32-bit node byte offsets in the ray kernels)
+constexpr uint32_t kMaxFaces = 1u << 26;
 
 __device__ inline void atomic_min_float(float* addr, float v) {
   if (v >= 0.0f) atomicMin(reinterpret_cast<int*>(addr), __float_as_int(v));
@@ -848,7 +850,11 @@ static int map_build(const float* vertices, uint32_t nv, const uint32_t* faces, 
   *out = nullptr;
   if (nf > 0 && (!vertices || !faces || nv == 0))
     return fail(RMH_ERR_INVALID, "rmh_map_build: faces given without vertices");
-  if (nf >= (1u << 28)) return fail(RMH_ERR_UNSUPPORTED, "rmh_map_build: at most 2^28 - 1 faces");
+  // The ray kernels address a node as a 32-bit byte offset (index << 6, 64-byte nodes), so
+  // every tree must have fewer than 2^26 interior nodes; a binary tree over n_faces leaves
+  // has at most n_faces - 1 of them (and the 4-wide tree fewer still).
+  if (nf > kMaxFaces)
+    return fail(RMH_ERR_UNSUPPORTED, "rmh_map_build: at most 2^26 faces (BVH node offsets are 32-bit)");
   rmh_bvh_params prm{4u, 1.0f, 1.5f, 0u};
   if (params) prm = *params;
   if (prm.max_leaf_size < 1 || prm.max_leaf_size > static_cast<uint32_t>(kMaxLeaf))

@@ -10,8 +10,16 @@
 //       math_batched.cu:30-546 sumBatched/covBatched), FP64 accumulation
 //   rmh_statistics
 //       point-to-point / point-to-plane correspondence statistics, segmented (one
-//       segment per pose hypothesis), FP64 (statistics.cu:58-160 p2p, :190-276 p2l,
-//       :278-371 objectwise; CPU statistics.cpp:20-126)
+//       segment per pose hypothesis), FP64 (statistics.cu:58-160 p2p, :190-276 p2l;
+//       CPU statistics.cpp:20-126)
+//   rmh_statistics_objectwise_p2l
+//       point-to-plane statistics per object over a pixel window of image-shaped scans,
+//       with per-object pre-transforms and constraints (statistics.cu:278-371,422-470)
+//   rmh_linear_system_p2l
+//       segmented FP64 point-to-plane Gauss-Newton system J^T w J, J^T w r
+//       (optimization.h:113-137,170-200)
+//   rmh_transform_pack_matrix / _quat
+//       (R, t) -> Transform (memory_math.cu:313-339,1258-1279)
 //   rmh_umeyama
 //       per-segment FP64 3x3 SVD (one-sided Jacobi) and the Umeyama rotation with the
 //       reflection fix (optimization.cpp:12-55, linalg.cu:1150-1190)
@@ -89,6 +97,66 @@ __global__ void k_quat_normalize(rmh_transform* T, uint32_t n) {
   const float l = sqrtf(q.x * q.x + q.y * q.y + q.z * q.z + q.w * q.w);  // Quaternion::l2norm
   q.x /= l; q.y /= l; q.z /= l; q.w /= l;
   T[i].R = q;
+}
+
+// Matrix3x3 -> Quaternion in FP32 with the reference's operation order
+// (src/rmagine_core/include/rmagine/math/types/MatrixOps.tcc:1492-1529).  M is column-major,
+// at(r, c) = M[c * 3 + r].  The reference writes `sqrtf(tr + 1.0)` and
+// `sqrtf(1.0 + a - b - c)` with a double literal: the sum is formed in double and rounded to
+// float for sqrtf, which is what the casts below do.  No normalisation.
+__device__ __forceinline__ rmh_quat mat_to_quat_f32(const float* M) {
+  const float a00 = M[0], a10 = M[1], a20 = M[2];
+  const float a01 = M[3], a11 = M[4], a21 = M[5];
+  const float a02 = M[6], a12 = M[7], a22 = M[8];
+  float tr = 0.0f;  // MatrixOps::trace: res += at(i, i)
+  tr += a00;
+  tr += a11;
+  tr += a22;
+  rmh_quat q;
+  if (tr > 0.0f) {
+    const float S = sqrtf(static_cast<float>(static_cast<double>(tr) + 1.0)) * 2.0f;
+    q.w = 0.25f * S;
+    q.x = (a21 - a12) / S;
+    q.y = (a02 - a20) / S;
+    q.z = (a10 - a01) / S;
+  } else if (a00 > a11 && a00 > a22) {
+    const float S = sqrtf(static_cast<float>(1.0 + static_cast<double>(a00) - a11 - a22)) * 2.0f;
+    q.w = (a21 - a12) / S;
+    q.x = 0.25f * S;
+    q.y = (a01 + a10) / S;
+    q.z = (a02 + a20) / S;
+  } else if (a11 > a22) {
+    const float S = sqrtf(static_cast<float>(1.0 + static_cast<double>(a11) - a00 - a22)) * 2.0f;
+    q.w = (a02 - a20) / S;
+    q.x = (a01 + a10) / S;
+    q.y = 0.25f * S;
+    q.z = (a12 + a21) / S;
+  } else {
+    const float S = sqrtf(static_cast<float>(1.0 + static_cast<double>(a22) - a00 - a11)) * 2.0f;
+    q.w = (a10 - a01) / S;
+    q.x = (a02 + a20) / S;
+    q.y = (a12 + a21) / S;
+    q.z = 0.25f * S;
+  }
+  return q;
+}
+
+// pack (src/rmagine_cuda/src/math/memory_math.cu:313-339): T.R from a rotation matrix or a
+// quaternion, T.t = t; the stamp of T is left as it is.
+__global__ void k_tf_pack_mat(const float* Rm, const rmh_vec3* t, rmh_transform* T, uint32_t n) {
+  const uint32_t i = blockIdx.x * kBlk + threadIdx.x;
+  if (i >= n) return;
+  float M[9];
+  for (int k = 0; k < 9; k++) M[k] = Rm[9 * static_cast<size_t>(i) + k];
+  T[i].R = mat_to_quat_f32(M);
+  T[i].t = t[i];
+}
+
+__global__ void k_tf_pack_quat(const rmh_quat* q, const rmh_vec3* t, rmh_transform* T, uint32_t n) {
+  const uint32_t i = blockIdx.x * kBlk + threadIdx.x;
+  if (i >= n) return;
+  T[i].R = q[i];
+  T[i].t = t[i];
 }
 
 // ---- reductions -------------------------------------------------------------------------
@@ -331,6 +399,185 @@ __global__ void k_stats_combine(const StatArgs a) {
   finalize_stats(&a.out[s], m, c);
 }
 
+// ---- objectwise point-to-plane statistics ---------------------------------------------------
+
+struct ObjArgs {
+  const rmh_transform* pre;             // one per object
+  rmh_cloud d, m;                       // image-shaped scans (segment_stride unused)
+  uint32_t width;                       // row stride of the scans
+  uint32_t n_points;                    // elements of the scans (the reference's bound N)
+  const rmh_umeyama_constraints* prm;   // one per object
+  const rmh_aabb* boxes;                // one per object: pixel window (x = column, y = row)
+  uint32_t window_only;
+  rmh_cross_stats* out;
+};
+
+// float -> unsigned as the GPU's saturating conversion gives it (the reference assigns the
+// float box corners to unsigned ints): NaN and values <= 0 give 0, values >= 2^32 give
+// 2^32 - 1, others are truncated toward zero.
+__device__ __forceinline__ uint32_t f2u_sat(float f) {
+  if (!(f > 0.0f)) return 0u;
+  if (f >= 4294967296.0f) return 0xFFFFFFFFu;
+  return static_cast<uint32_t>(f);
+}
+
+// statistics_objectwise_p2l_kernel (src/rmagine_cuda/src/math/statistics.cu:278-371): one
+// workgroup per object.  The element set is the reference kernel's: window element k
+// (row-major inside the box, k < bw * bh) is scan element start + (k / bw) * width + k % bw,
+// kept while below n_points.  The reference hands ceil(bw * bh / 512) consecutive window
+// elements to each of its 512 threads without bounding k by bw * bh, so its k runs up to
+// bw * bh rounded up to a multiple of 512, into the rows below the window; that is the
+// default here, window_only = 1 stops at bw * bh.  Each element then passes the mask and id
+// filters of the object's constraints and the plane-distance test exactly as in k_stats
+// (point-to-plane), in FP64.
+__global__ __launch_bounds__(kBlk) void k_stats_objectwise(const ObjArgs a) {
+  __shared__ double lds[kWaves][kMom];
+  const uint32_t o = blockIdx.x;
+  const rmh_aabb bb = a.boxes[o];
+  const uint64_t c0 = f2u_sat(bb.min.x), r0 = f2u_sat(bb.min.y);
+  const uint64_t c1 = f2u_sat(bb.max.x), r1 = f2u_sat(bb.max.y);
+  // an inverted box is empty (the reference's unsigned differences would wrap around)
+  const uint64_t bw = c1 > c0 ? c1 - c0 : 0, bh = r1 > r0 ? r1 - r0 : 0;
+  const uint64_t n_win = bw * bh;
+  uint64_t n_it = a.window_only ? n_win : (n_win + 511) / 512 * 512;
+  const uint64_t start = r0 * a.width + c0;
+  // rows from row_lim on start at or beyond n_points and contribute nothing
+  const uint64_t row_lim = start >= a.n_points ? 0 : (a.n_points - start + a.width - 1) / a.width;
+  if (n_it > row_lim * bw) n_it = row_lim * bw;
+
+  const rmh_transform P = a.pre[o];
+  M3d R = quat_matrix(P.R);
+  for (int k = 0; k < 9; k++) R.m[k] = uniform_f64(R.m[k]);
+  const double c[3] = {uniform_f64((double)P.t.x), uniform_f64((double)P.t.y), uniform_f64((double)P.t.z)};
+  const rmh_umeyama_constraints prm = a.prm[o];
+  const double maxd = uniform_f64(prm.max_dist);
+  Acc<kMom> m;
+  zero(m);
+  for (uint64_t k = threadIdx.x; k < n_it; k += kBlk) {
+    const uint64_t row = k / bw, col = k - row * bw;
+    const uint64_t i = start + row * a.width + col;
+    if (i >= a.n_points) continue;
+    if (a.d.mask && a.d.mask[i] == 0) continue;
+    if (a.m.mask && a.m.mask[i] == 0) continue;
+    if (a.d.ids && a.d.ids[i] != prm.dataset_id) continue;
+    if (a.m.ids && a.m.ids[i] != prm.model_id) continue;
+    const rmh_vec3 dp = a.d.points[i];
+    const rmh_vec3 mp = a.m.points[i];
+    const rmh_vec3 nn = a.m.normals[i];
+    const double dx = R.m[0] * dp.x + R.m[1] * dp.y + R.m[2] * dp.z;
+    const double dy = R.m[3] * dp.x + R.m[4] * dp.y + R.m[5] * dp.z;
+    const double dz = R.m[6] * dp.x + R.m[7] * dp.y + R.m[8] * dp.z;
+    const double mx = (double)mp.x - c[0], my = (double)mp.y - c[1], mz = (double)mp.z - c[2];
+    const double spd = (mx - dx) * nn.x + (my - dy) * nn.y + (mz - dz) * nn.z;
+    if (!(fabs(spd) < maxd)) continue;
+    acc_pair(m, dx, dy, dz, dx + nn.x * spd, dy + nn.y * spd, dz + nn.z * spd);
+  }
+  block_sum(m, lds);
+  if (threadIdx.x == 0) finalize_stats(&a.out[o], m, c);
+}
+
+// ---- point-to-plane Gauss-Newton system -------------------------------------------------------
+
+// accumulators: [0] count, [1..6] J^T w r, [7..27] upper triangle of J^T w J (row-major),
+// [28] sum w r^2
+constexpr int kLin = 29;
+
+struct LinArgs {
+  const rmh_transform* pre;
+  uint32_t npre;
+  rmh_cloud d, m;
+  uint64_t seg;
+  uint32_t nseg, bps;
+  rmh_umeyama_constraints prm;
+  double kscale;                  // GM kernel scale; <= 0: unit weights
+  double* partial;
+  rmh_p2l_system* out;
+};
+
+__device__ void finalize_system(rmh_p2l_system* out, const Acc<kLin>& m) {
+  rmh_p2l_system r;
+  r.n_meas = static_cast<uint32_t>(m.v[0]);
+  r._pad = 0;
+  for (int k = 0; k < 6; k++) r.JTwr[k] = m.v[1 + k];
+  int u = 7;
+  for (int i = 0; i < 6; i++)
+    for (int j = i; j < 6; j++, u++) {
+      r.JTwJ[i * 6 + j] = m.v[u];
+      r.JTwJ[j * 6 + i] = m.v[u];
+    }
+  r.wr2 = m.v[28];
+  *out = r;
+}
+
+// build_linear_system_p2l (src/rmagine_core/include/rmagine/math/optimization.h:170-200) with
+// jacobian_and_residual_p2l (:113-137) per correspondence, segmented and in FP64: Pd = pre *
+// dataset point, r = (Pd - Pm) . Nm, J = [Pd x Nm, Nm] (d r / d(omega, tau) for the left
+// perturbation Pd -> Exp(omega) Pd + tau), w = GM_weight(s, r) = s^2 / (s + r)^2 (the
+// reference passes the signed residual to GM_weight), and J^T w J, J^T w r, sum w r^2.  The
+// mask, id and |r| < max_dist filters of the statistics apply (with no mask and an infinite
+// max_dist every correspondence counts, as in the reference's loop).
+__global__ __launch_bounds__(kBlk) void k_p2l_system(const LinArgs a) {
+  __shared__ double lds[kWaves][kLin];
+  const uint32_t s = blockIdx.y, bb = blockIdx.x;
+  const rmh_transform P = a.pre[a.npre == 1 ? 0 : s];
+  M3d R = quat_matrix(P.R);
+  for (int k = 0; k < 9; k++) R.m[k] = uniform_f64(R.m[k]);
+  const double t[3] = {uniform_f64((double)P.t.x), uniform_f64((double)P.t.y), uniform_f64((double)P.t.z)};
+  Acc<kLin> m;
+#pragma unroll
+  for (int k = 0; k < kLin; k++) m.v[k] = 0.0;
+  const uint64_t db = static_cast<uint64_t>(s) * a.d.segment_stride;
+  const uint64_t mb = static_cast<uint64_t>(s) * a.m.segment_stride;
+  const uint64_t stride = static_cast<uint64_t>(a.bps) * kBlk;
+  const double maxd = a.prm.max_dist, ks = a.kscale;
+  for (uint64_t i = static_cast<uint64_t>(bb) * kBlk + threadIdx.x; i < a.seg; i += stride) {
+    const uint64_t di = db + i, mi = mb + i;
+    if (a.d.mask && a.d.mask[di] == 0) continue;
+    if (a.m.mask && a.m.mask[mi] == 0) continue;
+    if (a.d.ids && a.d.ids[di] != a.prm.dataset_id) continue;
+    if (a.m.ids && a.m.ids[mi] != a.prm.model_id) continue;
+    const rmh_vec3 dp = a.d.points[di];
+    const rmh_vec3 mp = a.m.points[mi];
+    const rmh_vec3 nn = a.m.normals[mi];
+    const double px = R.m[0] * dp.x + R.m[1] * dp.y + R.m[2] * dp.z + t[0];
+    const double py = R.m[3] * dp.x + R.m[4] * dp.y + R.m[5] * dp.z + t[1];
+    const double pz = R.m[6] * dp.x + R.m[7] * dp.y + R.m[8] * dp.z + t[2];
+    const double nx = nn.x, ny = nn.y, nz = nn.z;
+    const double r = (px - mp.x) * nx + (py - mp.y) * ny + (pz - mp.z) * nz;
+    if (!(fabs(r) < maxd)) continue;
+    const double J[6] = {py * nz - pz * ny, pz * nx - px * nz, px * ny - py * nx, nx, ny, nz};
+    double w = 1.0;
+    if (ks > 0.0) {
+      const double den = ks + r;
+      w = (ks * ks) / (den * den);
+    }
+    m.v[0] += 1.0;
+#pragma unroll
+    for (int k = 0; k < 6; k++) m.v[1 + k] += w * J[k] * r;
+    int u = 7;
+#pragma unroll
+    for (int p = 0; p < 6; p++)
+#pragma unroll
+      for (int q = p; q < 6; q++, u++) m.v[u] += w * J[p] * J[q];
+    m.v[28] += w * r * r;
+  }
+  block_sum(m, lds);
+  if (threadIdx.x != 0) return;
+  if (a.bps == 1) finalize_system(&a.out[s], m);
+  else
+    for (int k = 0; k < kLin; k++) a.partial[(static_cast<uint64_t>(s) * a.bps + bb) * kLin + k] = m.v[k];
+}
+
+__global__ void k_p2l_system_combine(const LinArgs a) {
+  const uint32_t s = blockIdx.x * kBlk + threadIdx.x;
+  if (s >= a.nseg) return;
+  Acc<kLin> m;
+  for (int k = 0; k < kLin; k++) m.v[k] = 0.0;
+  for (uint32_t b = 0; b < a.bps; b++)
+    for (int k = 0; k < kLin; k++) m.v[k] += a.partial[(static_cast<uint64_t>(s) * a.bps + b) * kLin + k];
+  finalize_system(&a.out[s], m);
+}
+
 // ---- Umeyama ------------------------------------------------------------------------------
 
 __device__ __forceinline__ double det3(const double* M) {  // row-major
@@ -477,9 +724,10 @@ uint32_t blocks_per_segment(uint64_t seg, uint32_t nseg) {
   return static_cast<uint32_t>(b);
 }
 
-struct Scratch {
+// device memory the call allocates itself when the caller passes no workspace
+struct TempAlloc {
   void* p = nullptr;
-  ~Scratch() { if (p) (void)hipFree(p); }
+  ~TempAlloc() { if (p) (void)hipFree(p); }
 };
 
 }  // namespace
@@ -529,10 +777,27 @@ int rmh_quat_normalize(rmh_transform* T, uint32_t n, void* stream) {
   return RMH_OK;
 }
 
+int rmh_transform_pack_matrix(const float* R, const rmh_vec3* t, rmh_transform* T, uint32_t n, void* stream) {
+  if (n == 0) return RMH_OK;
+  if (!R || !t || !T) return fail(RMH_ERR_INVALID, "rmh_transform_pack_matrix: NULL pointer");
+  k_tf_pack_mat<<<grid1(n), kBlk, 0, static_cast<hipStream_t>(stream)>>>(R, t, T, n);
+  RMH_HIP_TRY(hipGetLastError());
+  return RMH_OK;
+}
+
+int rmh_transform_pack_quat(const rmh_quat* q, const rmh_vec3* t, rmh_transform* T, uint32_t n, void* stream) {
+  if (n == 0) return RMH_OK;
+  if (!q || !t || !T) return fail(RMH_ERR_INVALID, "rmh_transform_pack_quat: NULL pointer");
+  k_tf_pack_quat<<<grid1(n), kBlk, 0, static_cast<hipStream_t>(stream)>>>(q, t, T, n);
+  RMH_HIP_TRY(hipGetLastError());
+  return RMH_OK;
+}
+
+// one workspace serves every segmented reduction: sized for the widest accumulator set
 int rmh_reduction_workspace_size(uint64_t segment_size, uint32_t n_segments, size_t* bytes) {
   if (!bytes) return fail(RMH_ERR_INVALID, "rmh_reduction_workspace_size: NULL");
   const uint32_t b = blocks_per_segment(segment_size, n_segments ? n_segments : 1);
-  *bytes = b > 1 ? sizeof(double) * kMom * static_cast<size_t>(b) * n_segments : 0;
+  *bytes = b > 1 ? sizeof(double) * kLin * static_cast<size_t>(b) * n_segments : 0;
   return RMH_OK;
 }
 
@@ -545,15 +810,15 @@ int rmh_vec3_moments(const rmh_vec3* a, const rmh_vec3* b, const uint8_t* mask, 
   hipStream_t stream = static_cast<hipStream_t>(stream_);
   MomArgs A{a, b, mask, segment_size, n_segments, blocks_per_segment(segment_size, n_segments), nullptr,
             sum_a, mean_a, cov, count};
-  Scratch scratch;
+  TempAlloc tmp;
   if (A.bps > 1) {
     const size_t need = sizeof(double) * kMom * static_cast<size_t>(A.bps) * n_segments;
     if (workspace) {
       if (workspace_bytes < need) return fail(RMH_ERR_INVALID, "rmh_vec3_moments: workspace too small");
       A.partial = static_cast<double*>(workspace);
     } else {
-      RMH_HIP_TRY(hipMalloc(&scratch.p, need));
-      A.partial = static_cast<double*>(scratch.p);
+      RMH_HIP_TRY(hipMalloc(&tmp.p, need));
+      A.partial = static_cast<double*>(tmp.p);
     }
   }
   k_moments<<<dim3(A.bps, n_segments), kBlk, 0, stream>>>(A);
@@ -561,7 +826,7 @@ int rmh_vec3_moments(const rmh_vec3* a, const rmh_vec3* b, const uint8_t* mask, 
   if (A.bps > 1) {
     k_moments_combine<<<grid1(n_segments), kBlk, 0, stream>>>(A);
     RMH_HIP_TRY(hipGetLastError());
-    if (scratch.p) RMH_HIP_TRY(hipStreamSynchronize(stream));
+    if (tmp.p) RMH_HIP_TRY(hipStreamSynchronize(stream));
   }
   return RMH_OK;
 }
@@ -592,15 +857,15 @@ int rmh_statistics(int kind, const rmh_transform* pre_transforms, uint32_t n_pre
   A.prm = *params;
   A.partial = nullptr;
   A.out = stats;
-  Scratch scratch;
+  TempAlloc tmp;
   if (A.bps > 1) {
     const size_t need = sizeof(double) * kMom * static_cast<size_t>(A.bps) * n_segments;
     if (workspace) {
       if (workspace_bytes < need) return fail(RMH_ERR_INVALID, "rmh_statistics: workspace too small");
       A.partial = static_cast<double*>(workspace);
     } else {
-      RMH_HIP_TRY(hipMalloc(&scratch.p, need));
-      A.partial = static_cast<double*>(scratch.p);
+      RMH_HIP_TRY(hipMalloc(&tmp.p, need));
+      A.partial = static_cast<double*>(tmp.p);
     }
   }
   // 32-bit offsets when i * 12 B (the largest element) and i + stride stay below 2^32
@@ -612,7 +877,80 @@ int rmh_statistics(int kind, const rmh_transform* pre_transforms, uint32_t n_pre
   if (A.bps > 1) {
     k_stats_combine<<<grid1(n_segments), kBlk, 0, stream>>>(A);
     RMH_HIP_TRY(hipGetLastError());
-    if (scratch.p) RMH_HIP_TRY(hipStreamSynchronize(stream));
+    if (tmp.p) RMH_HIP_TRY(hipStreamSynchronize(stream));
+  }
+  return RMH_OK;
+}
+
+int rmh_statistics_objectwise_p2l(const rmh_transform* pre_transforms, const rmh_cloud* dataset, const rmh_cloud* model,
+                                  uint32_t width, uint32_t height, uint32_t n_points,
+                                  const rmh_umeyama_constraints* params, const rmh_aabb* bboxes, uint32_t n_objects,
+                                  uint32_t flags, rmh_cross_stats* stats, void* stream) {
+  (void)height;  // the reference's kernel takes it but addresses rows with width only
+  if (n_objects == 0) return RMH_OK;
+  if (!pre_transforms || !dataset || !model || !params || !bboxes || !stats)
+    return fail(RMH_ERR_INVALID, "rmh_statistics_objectwise_p2l: NULL argument");
+  if (flags & ~RMH_OBJECTWISE_WINDOW_ONLY) return fail(RMH_ERR_INVALID, "rmh_statistics_objectwise_p2l: unknown flags");
+  if (width == 0) return fail(RMH_ERR_INVALID, "rmh_statistics_objectwise_p2l: width must be > 0");
+  if (n_points > 0 && (!dataset->points || !model->points || !model->normals))
+    return fail(RMH_ERR_INVALID, "rmh_statistics_objectwise_p2l: dataset points, model points and normals are required");
+  if (n_objects > 0x7FFFFFFFu) return fail(RMH_ERR_UNSUPPORTED, "rmh_statistics_objectwise_p2l: too many objects");
+  ObjArgs A;
+  A.pre = pre_transforms;
+  A.d = *dataset;
+  A.m = *model;
+  A.width = width;
+  A.n_points = n_points;
+  A.prm = params;
+  A.boxes = bboxes;
+  A.window_only = (flags & RMH_OBJECTWISE_WINDOW_ONLY) ? 1u : 0u;
+  A.out = stats;
+  k_stats_objectwise<<<n_objects, kBlk, 0, static_cast<hipStream_t>(stream)>>>(A);
+  RMH_HIP_TRY(hipGetLastError());
+  return RMH_OK;
+}
+
+int rmh_linear_system_p2l(const rmh_transform* pre_transforms, uint32_t n_pre, const rmh_cloud* dataset,
+                          const rmh_cloud* model, uint64_t segment_size, uint32_t n_segments,
+                          const rmh_umeyama_constraints* params, double kernel_scale, rmh_p2l_system* out,
+                          void* workspace, size_t workspace_bytes, void* stream_) {
+  if (n_segments == 0) return RMH_OK;
+  if (!pre_transforms || !dataset || !model || !params || !out)
+    return fail(RMH_ERR_INVALID, "rmh_linear_system_p2l: NULL argument");
+  if (n_pre != 1 && n_pre != n_segments) return fail(RMH_ERR_INVALID, "rmh_linear_system_p2l: n_pre must be 1 or n_segments");
+  if (segment_size > 0 && (!dataset->points || !model->points || !model->normals))
+    return fail(RMH_ERR_INVALID, "rmh_linear_system_p2l: dataset points, model points and normals are required");
+  if (n_segments > 65535u) return fail(RMH_ERR_UNSUPPORTED, "rmh_linear_system_p2l: at most 65535 segments");
+  hipStream_t stream = static_cast<hipStream_t>(stream_);
+  LinArgs A;
+  A.pre = pre_transforms;
+  A.npre = n_pre;
+  A.d = *dataset;
+  A.m = *model;
+  A.seg = segment_size;
+  A.nseg = n_segments;
+  A.bps = blocks_per_segment(segment_size, n_segments);
+  A.prm = *params;
+  A.kscale = kernel_scale;
+  A.partial = nullptr;
+  A.out = out;
+  TempAlloc tmp;
+  if (A.bps > 1) {
+    const size_t need = sizeof(double) * kLin * static_cast<size_t>(A.bps) * n_segments;
+    if (workspace) {
+      if (workspace_bytes < need) return fail(RMH_ERR_INVALID, "rmh_linear_system_p2l: workspace too small");
+      A.partial = static_cast<double*>(workspace);
+    } else {
+      RMH_HIP_TRY(hipMalloc(&tmp.p, need));
+      A.partial = static_cast<double*>(tmp.p);
+    }
+  }
+  k_p2l_system<<<dim3(A.bps, n_segments), kBlk, 0, stream>>>(A);
+  RMH_HIP_TRY(hipGetLastError());
+  if (A.bps > 1) {
+    k_p2l_system_combine<<<grid1(n_segments), kBlk, 0, stream>>>(A);
+    RMH_HIP_TRY(hipGetLastError());
+    if (tmp.p) RMH_HIP_TRY(hipStreamSynchronize(stream));
   }
   return RMH_OK;
 }

@@ -208,3 +208,114 @@ def stats_to_numpy(stats: torch.Tensor) -> dict:
     n = s.view(np.uint32)[:, 30].copy()
     return {"dataset_mean": s[:, 0:3], "model_mean": s[:, 3:6],
             "covariance": s[:, 6:15].reshape(-1, 3, 3).transpose(0, 2, 1), "n_meas": n}
+
+
+def pack(R: torch.Tensor, t: torch.Tensor, out: torch.Tensor | None = None, stream=None) -> torch.Tensor:
+    """pack (src/rmagine_cuda/include/rmagine/math/memory_math.cuh:459-470): transforms from
+    rotations and translations.  R is (N, 3, 3) float32 rotation matrices (R[i, r, c] = at(r, c),
+    converted with the reference's FP32 matrix -> quaternion steps) or (N, 4) quaternions
+    (x, y, z, w); t is (N, 3).  `out` (N, 8) keeps its stamps; a new one has stamp 0."""
+    _check_dev(t, torch.float32, "t", 3)
+    n = t.shape[0]
+    if out is None:
+        out = torch.zeros((n, 8), dtype=torch.float32, device=t.device)
+    _check_dev(out, torch.float32, "out", 8)
+    if out.shape[0] != n or R.shape[0] != n:
+        raise ValueError("R, t and out must hold the same number of entries")
+    if R.dim() == 3 and tuple(R.shape[1:]) == (3, 3):
+        if R.dtype != torch.float32 or R.device.type != "cuda":
+            raise ValueError("R must be a float32 device tensor")
+        Rc = R.transpose(1, 2).contiguous()  # Rmagine's Matrix3x3 is column-major
+        _lib.call("rmh_transform_pack_matrix", _ptr(Rc), _ptr(t), _ptr(out), n, stream_handle(stream))
+    else:
+        _check_dev(R, torch.float32, "R", 4)
+        _lib.call("rmh_transform_pack_quat", _ptr(R), _ptr(t), _ptr(out), n, stream_handle(stream))
+    return out
+
+
+def _constraints_array(params, n: int, device) -> torch.Tensor:
+    if isinstance(params, UmeyamaReductionConstraints):
+        params = [params] * n
+    if len(params) != n:
+        raise ValueError(f"need {n} constraints, got {len(params)}")
+    arr = (T.rmh_umeyama_constraints * n)()
+    for i, p in enumerate(params):
+        arr[i] = T.rmh_umeyama_constraints(p.max_dist, p.dataset_id, p.model_id)
+    host = torch.frombuffer(bytearray(bytes(arr)), dtype=torch.uint8)
+    return host.to(device)
+
+
+def statistics_objectwise_p2l(pre_transforms: torch.Tensor, dataset: PointCloudView, model: PointCloudView,
+                              width: int, height: int, params, bboxes: torch.Tensor, window_only: bool = False,
+                              out: torch.Tensor | None = None, stream=None) -> torch.Tensor:
+    """statistics_objectwise_p2l (src/rmagine_cuda/include/rmagine/math/statistics.cuh:139-158):
+    one FP64 statistics row per object over its pixel window (bboxes (n, 6) float32: min xyz,
+    max xyz; x columns, y rows) of image-shaped scans with row stride `width`, with per-object
+    pre-transforms (n, 8) and constraints (one UmeyamaReductionConstraints or a list of n).
+    Returns (n, 16) float64."""
+    _check_dev(pre_transforms, torch.float32, "pre_transforms", 8)
+    _check_dev(bboxes, torch.float32, "bboxes", 6)
+    n = pre_transforms.shape[0]
+    if bboxes.shape[0] != n:
+        raise ValueError("one bounding box per pre-transform")
+    if model.normals is None:
+        raise ValueError("point-to-plane statistics need model normals")
+    n_points = dataset.points.shape[0]
+    for name, cloud in (("dataset", dataset), ("model", model)):
+        for f in ("points", "mask", "normals", "ids"):
+            tt = getattr(cloud, f)
+            if tt is not None and tt.shape[0] < n_points:
+                raise ValueError(f"{name}.{f} holds fewer elements than dataset.points")
+    dev = pre_transforms.device
+    prm = _constraints_array(params, n, dev)
+    if out is None:
+        out = torch.empty((n, 16), dtype=torch.float64, device=dev)
+    _check_dev(out, torch.float64, "out", 16)
+    dc, mc = dataset.to_c(0), model.to_c(0)
+    _lib.call("rmh_statistics_objectwise_p2l", _ptr(pre_transforms), ctypes.byref(dc), ctypes.byref(mc), int(width),
+              int(height), int(n_points), _ptr(prm), _ptr(bboxes), n, 1 if window_only else 0, _ptr(out),
+              stream_handle(stream))
+    return out
+
+
+P2L_SYSTEM_COLS = 44  # rmh_p2l_system: JTwJ (36), JTwr (6), wr2, n_meas (uint32) + pad
+
+
+def linear_system_p2l(pre_transforms: torch.Tensor, dataset: PointCloudView, model: PointCloudView,
+                      params: UmeyamaReductionConstraints, kernel_scale: float = 5.0, segment_size: int | None = None,
+                      n_segments: int = 1, out: torch.Tensor | None = None, workspace: torch.Tensor | None = None,
+                      stream=None) -> torch.Tensor:
+    """Segmented FP64 point-to-plane Gauss-Newton system (build_linear_system_p2l,
+    src/rmagine_core/include/rmagine/math/optimization.h:170-200).  Returns (n_segments, 44)
+    float64 rows laid out like rmh_p2l_system (JTwJ row-major 6x6, JTwr, sum w r^2, n_meas)."""
+    if segment_size is None:
+        segment_size = dataset.points.shape[0] // n_segments
+    _check_dev(pre_transforms, torch.float32, "pre_transforms", 8)
+    n_pre = pre_transforms.shape[0]
+    if n_pre not in (1, n_segments):
+        raise ValueError("pre_transforms must hold 1 or n_segments transforms")
+    if model.normals is None:
+        raise ValueError("point-to-plane system needs model normals")
+    for name, cloud in (("dataset", dataset), ("model", model)):
+        need = cloud.max_index(segment_size, n_segments)
+        for f in ("points", "mask", "normals", "ids"):
+            tt = getattr(cloud, f)
+            if tt is not None and tt.shape[0] < need:
+                raise ValueError(f"{name}.{f} holds {tt.shape[0]} elements, the segments need {need}")
+    dev = dataset.points.device
+    if out is None:
+        out = torch.empty((n_segments, P2L_SYSTEM_COLS), dtype=torch.float64, device=dev)
+    _check_dev(out, torch.float64, "out", P2L_SYSTEM_COLS)
+    ws = workspace if workspace is not None else _workspace(segment_size, n_segments, dev)
+    dc, mc = dataset.to_c(segment_size), model.to_c(segment_size)
+    prm = T.rmh_umeyama_constraints(params.max_dist, params.dataset_id, params.model_id)
+    _lib.call("rmh_linear_system_p2l", _ptr(pre_transforms), n_pre, ctypes.byref(dc), ctypes.byref(mc), segment_size,
+              n_segments, ctypes.byref(prm), float(kernel_scale), _ptr(out), _ptr(ws),
+              0 if ws is None else ws.numel(), stream_handle(stream))
+    return out
+
+
+def p2l_system_to_numpy(sysm: torch.Tensor) -> dict:
+    s = sysm.detach().cpu().numpy()
+    return {"JTwJ": s[:, 0:36].reshape(-1, 6, 6), "JTwr": s[:, 36:42], "wr2": s[:, 42],
+            "n_meas": s.view(np.uint32)[:, 86].copy()}

@@ -222,6 +222,15 @@ class rmh_cross_stats(ctypes.Structure):
                 ("covariance", ctypes.c_double * 9), ("n_meas", ctypes.c_uint32), ("_pad", ctypes.c_uint32)]
 
 
+class rmh_aabb(ctypes.Structure):
+    _fields_ = [("min", rmh_vec3), ("max", rmh_vec3)]
+
+
+class rmh_p2l_system(ctypes.Structure):
+    _fields_ = [("JTwJ", ctypes.c_double * 36), ("JTwr", ctypes.c_double * 6), ("wr2", ctypes.c_double),
+                ("n_meas", ctypes.c_uint32), ("_pad", ctypes.c_uint32)]
+
+
 class rmh_batch_entry(ctypes.Structure):
     _fields_ = [("sensor", ctypes.c_void_p), ("Tsb", rmh_transform), ("Tbm_dev", ctypes.c_void_p),
                 ("n_poses", ctypes.c_uint32), ("_pad", ctypes.c_uint32), ("out", rmh_outputs)]

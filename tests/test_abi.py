@@ -47,6 +47,21 @@ def test_error_path_without_device():
     assert lib.rmh_sensor_create(7, ctypes.byref(T.rmh_spherical_model()), ctypes.byref(ctypes.c_void_p())) != 0
 
 
+def test_map_size_limit_without_device():
+    """Maps whose BVH could hold 2^26 or more interior nodes (more than 2^26 faces) are
+    refused before any device work: the ray kernels address nodes with 32-bit byte offsets
+    (index << 6).  The pointers are never dereferenced: the size check comes first."""
+    lib = _lib.lib()
+    out = ctypes.c_void_p()
+    fake = ctypes.c_void_p(0x1000)
+    rc = lib.rmh_map_build(fake, 3, fake, (1 << 26) + 1, None, 0, None, None, ctypes.byref(out))
+    assert rc == 5  # RMH_ERR_UNSUPPORTED
+    assert b"2^26" in lib.rmh_last_error()
+    assert out.value is None
+    rc = lib.rmh_map_build(fake, 3, fake, 0xFFFFFFFF, None, 0, None, None, ctypes.byref(out))
+    assert rc == 5
+
+
 def test_struct_layouts_match_header(tmp_path):
     """Every ctypes mirror in rmagine_amd.types has the size and field offsets that a C
     compiler gives the same struct from include/rmh.h."""

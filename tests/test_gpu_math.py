@@ -166,3 +166,102 @@ def test_umeyama_against_oracle():
     np.testing.assert_array_equal(g[1, :4], [0, 0, 0, 1])
     np.testing.assert_allclose(Tf.cpu().numpy()[:, :7], g.astype(np.float32), atol=1e-6)
     print("umeyama max |dq|", np.abs(g[:, :4] * sign - ref[:, :4]).max(), "max |dt|", np.abs(g[:, 4:] - ref[:, 4:]).max())
+
+
+def test_pack_bit_exact():
+    """pack: matrix input bit-identical to the FP32 restatement of the reference's matrix ->
+    quaternion steps (all four branches), quaternion input copied, stamps kept."""
+    from tests.test_stats_oracle import _branch_rotations
+    rng = np.random.default_rng(21)
+    rots = _branch_rotations(rng)
+    Rm = np.stack([r.as_matrix() for r in rots]).astype(np.float32)
+    n = len(Rm)
+    t = rng.normal(size=(n, 3)).astype(np.float32)
+    out = torch.zeros((n, 8), dtype=torch.float32, device=dev)
+    out.view(torch.int32)[:, 7] = torch.arange(n, dtype=torch.int32, device=dev) + 7
+    M.pack(torch.as_tensor(Rm, device=dev), torch.as_tensor(t, device=dev), out=out)
+    g = out.cpu().numpy()
+    np.testing.assert_array_equal(g[:, :4], S.pack_matrix_f32(Rm))
+    np.testing.assert_array_equal(g[:, 4:7], t)
+    np.testing.assert_array_equal(g.view(np.int32)[:, 7], np.arange(n) + 7)
+    q = rng.normal(size=(n, 4)).astype(np.float32)
+    g2 = M.pack(torch.as_tensor(q, device=dev), torch.as_tensor(t, device=dev)).cpu().numpy()
+    np.testing.assert_array_equal(g2[:, :4], q)
+    np.testing.assert_array_equal(g2[:, 4:7], t)
+    assert not g2[:, 7].any()
+
+
+def _image_clouds(rng, width, height):
+    n = width * height
+    d = rng.normal(size=(n, 3)).astype(np.float32) * 3
+    m = (d + rng.normal(scale=0.05, size=(n, 3))).astype(np.float32)
+    nr = rng.normal(size=(n, 3))
+    nr = (nr / np.linalg.norm(nr, axis=1, keepdims=True)).astype(np.float32)
+    mask = (rng.uniform(size=n) > 0.15).astype(np.uint8)
+    ids = rng.integers(0, 4, n).astype(np.uint32)
+    return d, m, nr, mask, ids
+
+
+@pytest.mark.parametrize("window_only", [False, True])
+def test_statistics_objectwise_against_oracle(window_only):
+    """statistics_objectwise_p2l against the oracle's element set (the reference kernel's,
+    including its 512-rounded tail, or the window alone) and FP64 p2l statistics; corner
+    cases: fractional, negative, NaN, inverted and empty windows, windows past the scan."""
+    rng = np.random.default_rng(31)
+    width, height = 320, 120
+    n = width * height
+    d, m, nr, mask, ids = _image_clouds(rng, width, height)
+    boxes = np.float32([(10, 5, 0, 40, 25, 0), (0, 0, 0, 320, 120, 0), (300, 100, 0, 320, 120, 0),
+                        (5.7, 3.2, 0, 17.9, 9.99, 0), (-5, -3, 0, 8, 4, 0), (np.nan, 2, 0, 9, 6, 0),
+                        (50, 50, 0, 40, 60, 0), (7, 7, 0, 7, 50, 0), (100, 110, 0, 200, 400, 0),
+                        (0, 119, 0, 320, 130, 0), (1, 2, 0, 2, 3, 0)])
+    k = len(boxes)
+    pre = rand_transforms(k, rng, scale=0.05)
+    params = [(float(rng.uniform(0.05, 0.5)), int(rng.integers(0, 4)), int(rng.integers(0, 4))) for _ in range(k)]
+    params[1] = (1e9, 0, 0)
+    Td = lambda x: torch.as_tensor(x, device=dev)
+    use_ids = [True, False]
+    for with_ids in use_ids:
+        ds = {"points": d, "mask": mask, "ids": ids if with_ids else None}
+        ms = {"points": m, "normals": nr, "ids": ids if with_ids else None}
+        o = S.statistics_objectwise_p2l(pre, ds, ms, width, n, params, boxes, window_only)
+        g = M.stats_to_numpy(M.statistics_objectwise_p2l(
+            Td(pre), M.PointCloudView(Td(d), mask=Td(mask), ids=Td(ids.astype(np.int32)) if with_ids else None),
+            M.PointCloudView(Td(m), normals=Td(nr), ids=Td(ids.astype(np.int32)) if with_ids else None),
+            width, height, [M.UmeyamaReductionConstraints(*p) for p in params], Td(boxes), window_only=window_only))
+        np.testing.assert_array_equal(g["n_meas"], [x["n_meas"] for x in o])
+        assert g["n_meas"][1] > (0.2 if with_ids else 0.8) * n
+        for key in ("dataset_mean", "model_mean", "covariance"):
+            ref = np.stack([x[key] for x in o])
+            np.testing.assert_allclose(g[key], ref, rtol=1e-11, atol=1e-12)
+    # the default element set really differs from the window on a window of 30 x 20 = 600
+    e_ref = S.objectwise_elements(boxes[0], width, n)
+    e_win = S.objectwise_elements(boxes[0], width, n, window_only=True)
+    assert len(e_ref) == 1024 and len(e_win) == 600
+
+
+@pytest.mark.parametrize("ks", [0.0, 5.0])
+def test_linear_system_p2l_against_oracle(ks):
+    rng = np.random.default_rng(41)
+    seg, nseg = 20000, 12
+    pre = rand_transforms(nseg, rng, scale=0.2)
+    d = (rng.normal(size=(seg, 3)) * 4).astype(np.float32)
+    mp = (rng.normal(size=(seg * nseg, 3)) * 4).astype(np.float32)
+    nr = rng.normal(size=(seg * nseg, 3))
+    nr = (nr / np.linalg.norm(nr, axis=1, keepdims=True)).astype(np.float32)
+    dmask = (rng.uniform(size=seg) > 0.1).astype(np.uint8)
+    mmask = (rng.uniform(size=seg * nseg) > 0.1).astype(np.uint8)
+    md = 3.0
+    o = S.linear_system_p2l(pre, {"points": d, "mask": dmask, "stride": 0}, {"points": mp, "mask": mmask, "normals": nr},
+                            seg, nseg, md, ks)
+    Td = lambda x: torch.as_tensor(x, device=dev)
+    g = M.p2l_system_to_numpy(M.linear_system_p2l(
+        Td(pre), M.PointCloudView(Td(d), mask=Td(dmask), segment_stride=0),
+        M.PointCloudView(Td(mp), mask=Td(mmask), normals=Td(nr)), M.UmeyamaReductionConstraints(md),
+        kernel_scale=ks, segment_size=seg, n_segments=nseg))
+    np.testing.assert_array_equal(g["n_meas"], [x["n_meas"] for x in o])
+    for key in ("JTwJ", "JTwr"):
+        ref = np.stack([x[key] for x in o])
+        scale = np.abs(ref).max()
+        np.testing.assert_allclose(g[key], ref, rtol=1e-10, atol=1e-12 * scale)
+    np.testing.assert_allclose(g["wr2"], [x["wr2"] for x in o], rtol=1e-10)
