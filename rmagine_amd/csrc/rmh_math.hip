@@ -831,21 +831,23 @@ int rmh_vec3_moments(const rmh_vec3* a, const rmh_vec3* b, const uint8_t* mask, 
   return RMH_OK;
 }
 
-int rmh_statistics(int kind, const rmh_transform* pre_transforms, uint32_t n_pre, const rmh_cloud* dataset,
-                   const rmh_cloud* model, uint64_t segment_size, uint32_t n_segments,
-                   const rmh_umeyama_constraints* params, rmh_cross_stats* stats, void* workspace,
-                   size_t workspace_bytes, void* stream_) {
-  if (n_segments == 0) return RMH_OK;
-  if (!pre_transforms || !dataset || !model || !params || !stats)
-    return fail(RMH_ERR_INVALID, "rmh_statistics: NULL argument");
-  if (kind != RMH_STATS_P2P && kind != RMH_STATS_P2L) return fail(RMH_ERR_INVALID, "rmh_statistics: bad kind");
-  if (n_pre != 1 && n_pre != n_segments) return fail(RMH_ERR_INVALID, "rmh_statistics: n_pre must be 1 or n_segments");
-  if (segment_size > 0 && (!dataset->points || !model->points)) return fail(RMH_ERR_INVALID, "rmh_statistics: points are required");
+}  // extern "C"
+
+namespace {
+
+// Argument checks and set-up shared by rmh_statistics and rmh_statistics_noalloc; leaves
+// A.partial unset.  Returns RMH_OK, or an error whose message starts with `who`.
+int stats_prepare(const char* who, int kind, const rmh_transform* pre_transforms, uint32_t n_pre,
+                  const rmh_cloud* dataset, const rmh_cloud* model, uint64_t segment_size, uint32_t n_segments,
+                  const rmh_umeyama_constraints* params, rmh_cross_stats* stats, StatArgs& A) {
+  const std::string w(who);
+  if (!pre_transforms || !dataset || !model || !params || !stats) return fail(RMH_ERR_INVALID, w + ": NULL argument");
+  if (kind != RMH_STATS_P2P && kind != RMH_STATS_P2L) return fail(RMH_ERR_INVALID, w + ": bad kind");
+  if (n_pre != 1 && n_pre != n_segments) return fail(RMH_ERR_INVALID, w + ": n_pre must be 1 or n_segments");
+  if (segment_size > 0 && (!dataset->points || !model->points)) return fail(RMH_ERR_INVALID, w + ": points are required");
   if (kind == RMH_STATS_P2L && segment_size > 0 && !model->normals)
-    return fail(RMH_ERR_INVALID, "rmh_statistics: point-to-plane needs model normals");
-  if (n_segments > 65535u) return fail(RMH_ERR_UNSUPPORTED, "rmh_statistics: at most 65535 segments");
-  hipStream_t stream = static_cast<hipStream_t>(stream_);
-  StatArgs A;
+    return fail(RMH_ERR_INVALID, w + ": point-to-plane needs model normals");
+  if (n_segments > 65535u) return fail(RMH_ERR_UNSUPPORTED, w + ": at most 65535 segments");
   A.kind = kind;
   A.pre = pre_transforms;
   A.npre = n_pre;
@@ -857,9 +859,45 @@ int rmh_statistics(int kind, const rmh_transform* pre_transforms, uint32_t n_pre
   A.prm = *params;
   A.partial = nullptr;
   A.out = stats;
+  return RMH_OK;
+}
+
+size_t stats_partial_bytes(const StatArgs& A) {
+  return A.bps > 1 ? sizeof(double) * kMom * static_cast<size_t>(A.bps) * A.nseg : 0;
+}
+
+// k_stats, then (segments split over several blocks) k_stats_combine over A.partial
+int stats_launch(const StatArgs& A, hipStream_t stream) {
+  // 32-bit offsets when i * 12 B (the largest element) and i + stride stay below 2^32
+  if (A.seg + static_cast<uint64_t>(A.bps) * kBlk < (1ull << 32) / sizeof(rmh_vec3))
+    k_stats<uint32_t><<<dim3(A.bps, A.nseg), kBlk, 0, stream>>>(A);
+  else
+    k_stats<uint64_t><<<dim3(A.bps, A.nseg), kBlk, 0, stream>>>(A);
+  RMH_HIP_TRY(hipGetLastError());
+  if (A.bps > 1) {
+    k_stats_combine<<<grid1(A.nseg), kBlk, 0, stream>>>(A);
+    RMH_HIP_TRY(hipGetLastError());
+  }
+  return RMH_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int rmh_statistics(int kind, const rmh_transform* pre_transforms, uint32_t n_pre, const rmh_cloud* dataset,
+                   const rmh_cloud* model, uint64_t segment_size, uint32_t n_segments,
+                   const rmh_umeyama_constraints* params, rmh_cross_stats* stats, void* workspace,
+                   size_t workspace_bytes, void* stream_) {
+  if (n_segments == 0) return RMH_OK;
+  StatArgs A;
+  const int rc = stats_prepare("rmh_statistics", kind, pre_transforms, n_pre, dataset, model, segment_size, n_segments,
+                               params, stats, A);
+  if (rc != RMH_OK) return rc;
+  hipStream_t stream = static_cast<hipStream_t>(stream_);
   TempAlloc tmp;
   if (A.bps > 1) {
-    const size_t need = sizeof(double) * kMom * static_cast<size_t>(A.bps) * n_segments;
+    const size_t need = stats_partial_bytes(A);
     if (workspace) {
       if (workspace_bytes < need) return fail(RMH_ERR_INVALID, "rmh_statistics: workspace too small");
       A.partial = static_cast<double*>(workspace);
@@ -868,17 +906,9 @@ int rmh_statistics(int kind, const rmh_transform* pre_transforms, uint32_t n_pre
       A.partial = static_cast<double*>(tmp.p);
     }
   }
-  // 32-bit offsets when i * 12 B (the largest element) and i + stride stay below 2^32
-  if (segment_size + static_cast<uint64_t>(A.bps) * kBlk < (1ull << 32) / sizeof(rmh_vec3))
-    k_stats<uint32_t><<<dim3(A.bps, n_segments), kBlk, 0, stream>>>(A);
-  else
-    k_stats<uint64_t><<<dim3(A.bps, n_segments), kBlk, 0, stream>>>(A);
-  RMH_HIP_TRY(hipGetLastError());
-  if (A.bps > 1) {
-    k_stats_combine<<<grid1(n_segments), kBlk, 0, stream>>>(A);
-    RMH_HIP_TRY(hipGetLastError());
-    if (tmp.p) RMH_HIP_TRY(hipStreamSynchronize(stream));
-  }
+  const int lc = stats_launch(A, stream);
+  if (lc != RMH_OK) return lc;
+  if (tmp.p) RMH_HIP_TRY(hipStreamSynchronize(stream));
   return RMH_OK;
 }
 
