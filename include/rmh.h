@@ -421,6 +421,18 @@ int rmh_statistics(int kind, const rmh_transform* pre_transforms_dev, uint32_t n
                    uint32_t n_segments, const rmh_umeyama_constraints* params,
                    rmh_cross_stats* stats_dev, void* workspace, size_t workspace_bytes, void* stream);
 
+/* rmh_statistics without any allocation: no counterpart in the reference.  For callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving loops, hosts
+ * that forbid hipMalloc) and pass no workspace.  Same arguments, kernels and results as
+ * rmh_statistics; the per-block partial sums go to a fixed 2 MiB device array of the library
+ * (one per GPU, held until the process ends), which covers every shape rmh_statistics
+ * accepts (larger partial sets would be refused with RMH_ERR_UNSUPPORTED).  Calls that
+ * overlap on one GPU share that array and race: serialise them (one stream). */
+int rmh_statistics_noalloc(int kind, const rmh_transform* pre_transforms_dev, uint32_t n_pre,
+                           const rmh_cloud* dataset, const rmh_cloud* model, uint64_t segment_size,
+                           uint32_t n_segments, const rmh_umeyama_constraints* params,
+                           rmh_cross_stats* stats_dev, void* stream);
+
 /* rmagine::AABB (src/rmagine_core/include/rmagine/math/types/AABB.hpp:50-53), 24 bytes */
 typedef struct { rmh_vec3 min, max; } rmh_aabb;
 

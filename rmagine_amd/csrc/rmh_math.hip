@@ -30,6 +30,7 @@
 #include "rmh_internal.hpp"
 
 #include <cmath>
+#include <mutex>
 #include <vector>
 
 namespace {
@@ -910,6 +911,72 @@ int rmh_statistics(int kind, const rmh_transform* pre_transforms, uint32_t n_pre
   if (lc != RMH_OK) return lc;
   if (tmp.p) RMH_HIP_TRY(hipStreamSynchronize(stream));
   return RMH_OK;
+}
+
+}  // extern "C"
+
+// ---- allocation-free statistics -----------------------------------------------------------
+//
+// rmh_statistics_noalloc keeps the per-block partial sums of rmh_statistics in a fixed
+// device array instead of a caller workspace or a hipMalloc.  Each GPU the library runs on
+// holds its own copy of the array (2 MiB) until the process ends.  Every call on one GPU
+// uses the same array: calls that overlap on one GPU (two streams, two host threads) race.
+// Nothing in it needs clearing: k_stats writes every partial slot that k_stats_combine reads.
+
+namespace {
+
+constexpr size_t kScratchBytes = size_t(2) << 20;
+// the largest partial set rmh_statistics' partition makes is bps * n_segments < 16384 slots
+// of kMom doubles (bps > 1 only for n_segments <= 8191, and bps * n_segments < 8192 +
+// n_segments), 2 MiB
+static_assert(sizeof(double) * kMom * 16384 <= kScratchBytes, "scratch holds the largest partial set");
+
+static __device__ __align__(256) double scratch[kScratchBytes / sizeof(double)];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_dev[kMaxDevices] = {};
+
+// Device address of this device's `scratch` (the array's name is only a host-side stand-in),
+// looked up once per device with that device current.
+int scratch_address(void** out) {
+  int dev = 0;
+  RMH_HIP_TRY(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return fail(RMH_ERR_UNSUPPORTED, "rmh_statistics_noalloc: device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) return hip_fail(e != hipSuccess ? e : hipErrorInvalidSymbol, "hipGetSymbolAddress(scratch)");
+    g_scratch_dev[dev] = p;
+  }
+  *out = g_scratch_dev[dev];
+  return RMH_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int rmh_statistics_noalloc(int kind, const rmh_transform* pre_transforms, uint32_t n_pre, const rmh_cloud* dataset,
+                           const rmh_cloud* model, uint64_t segment_size, uint32_t n_segments,
+                           const rmh_umeyama_constraints* params, rmh_cross_stats* stats, void* stream) {
+  if (n_segments == 0) return RMH_OK;
+  StatArgs A;
+  const int rc = stats_prepare("rmh_statistics_noalloc", kind, pre_transforms, n_pre, dataset, model, segment_size,
+                               n_segments, params, stats, A);
+  if (rc != RMH_OK) return rc;
+  const size_t need = stats_partial_bytes(A);
+  if (need > kScratchBytes)
+    return fail(RMH_ERR_UNSUPPORTED, "rmh_statistics_noalloc: partial sums need " + std::to_string(need) +
+                                         " bytes, more than the 2 MiB scratch array");
+  if (need > 0) {
+    void* p = nullptr;
+    const int sc = scratch_address(&p);
+    if (sc != RMH_OK) return sc;
+    A.partial = static_cast<double*>(p);
+  }
+  return stats_launch(A, static_cast<hipStream_t>(stream));
 }
 
 int rmh_statistics_objectwise_p2l(const rmh_transform* pre_transforms, const rmh_cloud* dataset, const rmh_cloud* model,
