@@ -32,20 +32,43 @@ HBM_PEAK_GBS = 8000.0
 TRAFFIC_PROFILE = os.path.join("profiles", "k_simulate_terrain_1000x128x1024.md")
 
 
-def profiled(default_workload: bool, key: str, cast):
-    """A figure measured with rocprofv3 for the default workload, recorded in the profile
+def profiled(default_workload: bool, key: str, cast, path: str = TRAFFIC_PROFILE):
+    """A figure measured with rocprofv3 for a default workload, recorded in its profile
     summary on a `key: value` line (`traffic_bytes_per_launch`: HBM-side bytes per launch;
     `l2_hit_rate`: TCC hits / (hits + misses)), or None."""
     if not default_workload:
         return None
     try:
-        with open(os.path.join(ROOT, TRAFFIC_PROFILE)) as f:
+        with open(os.path.join(ROOT, path)) as f:
             for line in f:
                 if line.startswith(key + ":"):
                     return cast(line.split(":", 1)[1].strip())
     except (OSError, ValueError):
         pass
     return None
+
+
+def roofline(kernel: str, alg_bytes: float, kernel_ms: float, profile: str | None = None, note: str = "") -> dict:
+    """HBM roofline object of one kernel: algorithmic bytes per launch over its measured
+    launch time against 8 TB/s, and the HBM-side traffic per launch of its profile summary."""
+    gbs = alg_bytes / (kernel_ms * 1e-3) / 1e9
+    traffic = profiled(profile is not None, "traffic_bytes_per_launch", int, profile) if profile else None
+    out = {"bound": "hbm", "achieved": round(gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": round(gbs / HBM_PEAK_GBS, 5), "traffic": traffic, "kernel": kernel,
+           "kernel_avg_ms": round(kernel_ms, 4), "alg_bytes_per_launch": int(alg_bytes)}
+    if traffic:
+        out["traffic_source"] = profile
+        out["traffic_gbs"] = round(traffic / (kernel_ms * 1e-3) / 1e9, 1)
+        out["l2_hit_rate"] = profiled(True, "l2_hit_rate", float, profile)
+    if note:
+        out["note"] = note
+    return out
+
+
+BATCH_PROFILE = os.path.join("profiles", "k_batch_objects_2000_mixed.md")
+ICP_PROFILE = os.path.join("profiles", "k_simulate_o1dn_objects_2000x32768.md")
+
+
 BYTES_PER_RAY_ALL = 1 + 4 + 12 + 12 + 4 + 4 + 4  # hits, ranges, points, normals, 3 ids
 
 
@@ -59,6 +82,8 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the batched / pose-gradient / ICP workloads")
+    ap.add_argument("--extra-only", choices=["batched", "pose_gradient", "icp"],
+                    help="only run one secondary workload with default launch parameters (for profiling)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy 5 times (scale factor for HBM counters)")
     ap.add_argument("--visit-stats", type=int, metavar="POSES", default=0,
@@ -74,9 +99,11 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(V, F, model, Tbm, seconds, gpu_numpy_for_pose):
-    """Times the oracle on the host cores, one sensor pose at a time, for about `seconds`."""
+def cpu_baseline(V, F, model, Tbm, seconds, gpu_numpy_for_pose, flag_poses: int = 100):
+    """Times the oracle on the host cores, one sensor pose at a time, for about `seconds`,
+    and compares the GPU's results of the poses it casts (first pass) with the oracle's."""
     import oracle
+    from oracle import diff as D
 
     t0 = time.time()
     om = oracle.OracleMap(V, F)
@@ -86,8 +113,8 @@ def cpu_baseline(V, F, model, Tbm, seconds, gpu_numpy_for_pose):
     elapsed = 0.0
     poses = 0
     per_pose = []
-    worst = {"range_max_err": 0.0, "point_max_err": 0.0, "normal_max_err": 0.0, "nan_mismatch": 0,
-             "id_mismatch": 0, "geom_object_id_mismatch": 0, "hit_mismatch": 0, "ambiguous": 0, "rays": 0}
+    acc = None
+    reflagged = 0
     oracle.simulate(om, model, Tbm[:1], threads=threads, flags=False)  # warm-up, not timed
     # the reference's benchmark loop (velodyne_benchmark.hpp:92-114): run for at least
     # `seconds`, report the running mean; poses are cast in order and, once all are done,
@@ -101,33 +128,39 @@ def cpu_baseline(V, F, model, Tbm, seconds, gpu_numpy_for_pose):
         per_pose.append(dt)
         if poses < len(Tbm):
             g = gpu_numpy_for_pose(p)
-            worst["hit_mismatch"] += int((g["hits"] != o["hits"]).sum())
-            worst["id_mismatch"] += int((g["face_ids"] != o["face_ids"]).sum())
-            worst["range_max_err"] = max(worst["range_max_err"],
-                                         float(np.abs(g["ranges"].astype(np.float64) - o["ranges"]).max()))
-            for key in ("points", "normals"):
-                # misses hold NaN on both sides: finite values by their largest difference,
-                # the NaN pattern exactly
-                fg, fo = np.isfinite(g[key]), np.isfinite(o[key])
-                worst["nan_mismatch"] += int((fg != fo).sum())
-                both = fg & fo
-                if both.any():
-                    err = float(np.abs(g[key][both].astype(np.float64) - o[key][both]).max())
-                    worst[key[:-1] + "_max_err"] = max(worst[key[:-1] + "_max_err"], err)
-            worst["geom_object_id_mismatch"] += int((g["geom_ids"] != o["geom_ids"]).sum()
-                                                    + (g["object_ids"] != o["object_ids"]).sum())
-            worst["rays"] += len(o["ranges"])
+            r = D.compare(g, {**o, "flags": np.zeros(len(o["ranges"]), np.uint8)}, model.range.max)
+            if r["mismatching_rays"]:
+                # classify the differences: the same pose again with the ambiguity flags (untimed)
+                r = D.compare(g, oracle.simulate(om, model, Tbm[p:p + 1], threads=threads), model.range.max)
+                reflagged += 1
+            acc = D.merge(acc, {**r, "flagged": {}})
         rays += len(o["ranges"])
         poses += 1
     log(f"cpu: {poses} pose casts, {rays} rays in {elapsed:.2f} s on {threads} threads")
+    # the oracle's ambiguity flags over the first poses (untimed): rays within 1e-6 of an edge,
+    # near misses, near ties and range limits, where an FP32 intersector may decide otherwise
+    k = min(flag_poses, len(Tbm))
+    fo = oracle.simulate(om, model, Tbm[:k], threads=threads, attrs=("hits",))["flags"]
+    flagged = {"poses": k, "rays": int(len(fo)), "any": int((fo != 0).sum()),
+               "edge": int((fo & oracle.FLAG_EDGE != 0).sum()), "near_miss": int((fo & oracle.FLAG_NEAR_MISS != 0).sum()),
+               "tie": int((fo & oracle.FLAG_TIE != 0).sum()), "range_limit": int((fo & oracle.FLAG_RANGE != 0).sum())}
     med = float(np.median(per_pose))
     base = {"value": rays / elapsed / 1e6, "unit": "Mrays/s", "cores": threads, "kind": "port",
             "median": model.size() / med / 1e6, "cpu": cpu_model(),
             "sample": f"{poses} casts of the {len(Tbm)} sensor poses in order ({rays} rays, {elapsed:.1f} s), all "
-                      f"attributes, oracle (exact FP64 predicate, binned-SAH BVH) on {threads} host threads after one "
-                      f"untimed pose, BVH build excluded; value = rays / total time, median = per-pose median; "
-                      f"accuracy over the first {min(poses, len(Tbm))} poses"}
-    return base, worst
+                      f"attributes, oracle (scalar C++ port of the Embree simulate loop: exact FP64 ray/triangle "
+                      f"predicate, FP64 box tests, binned-SAH binary BVH; no SIMD packets) on {threads} host threads "
+                      f"after one untimed pose, BVH build excluded; value = rays / total time, median = per-pose "
+                      f"median; accuracy over the first {min(poses, len(Tbm))} poses"}
+    accuracy = {"rays": acc["rays"], "poses": min(poses, len(Tbm)), "mismatch": acc["mismatch"],
+                "mismatching_rays": acc["mismatching_rays"], "mismatching_unflagged": acc["mismatching_unflagged"],
+                "poses_reflagged": reflagged,
+                "range_max_err": acc["diff"]["ranges"]["max_diff"], "point_max_err": acc["diff"]["points"]["max_diff"],
+                "normal_max_err": acc["diff"]["normals"]["max_diff"], "diff": acc["diff"],
+                "ambiguous": flagged["any"], "flagged": flagged,
+                "note": "diff: apps/rmagine_simulation_diff_calc/Main.cpp:83-147 per attribute (a = GPU, b = oracle); "
+                        "ambiguous: rays the oracle flags over the first poses"}
+    return base, accuracy
 
 
 def measure_hbm_gbs(nbytes: int = 2 << 30, reps: int = 5) -> float:
@@ -168,7 +201,7 @@ def _events():
     return torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
 
 
-def extra_batched(cpu_seconds: float, do_cpu: bool) -> dict:
+def extra_batched(cpu_seconds: float, do_cpu: bool, tune: bool = True) -> dict:
     """Heterogeneous batch: 2,000 sensors (40 % VLP-16, 30 % pinhole 640x480, 30 % O1Dn with
     1k-50k rays) on a 500k-triangle scene, hits + ranges, one persistent launch, tuned; the
     split into two launches (image sensors, then ray lists) is timed beside it."""
@@ -186,8 +219,12 @@ def extra_batched(cpu_seconds: float, do_cpu: bool) -> dict:
     for m, _ in sensors:
         mix[{0: "vlp16", 1: "pinhole", 2: "o1dn"}[m.kind]] += 1
     key = autotune.cache_key(len(F), mix)
-    tuned = autotune.tune_batch(batch, key=key, chunks=(64, 256, 1024), waves=(16, 32), tile_rows=(2, 4, 8),
-                                reps=2, use_cache=False, layouts=(False, True))
+    if tune:
+        tuned = autotune.tune_batch(batch, key=key, chunks=(64, 256, 1024), waves=(16, 32), tile_rows=(2, 4, 8),
+                                    reps=2, use_cache=False, layouts=(False, True))
+    else:
+        tuned = {"chunk_rays": 256, "waves_per_cu": 32, "tile_rows": 4, "lds_stack": 10, "lds_stack_ms": {},
+                 "rays_sorted": True, "ray_order_ms": None, "wide": False, "layout_ms": None}
     ms = autotune.time_batch(batch, reps=5)
     batch.set_params(launches=2)
     split_ms = autotune.time_batch(batch, reps=5)
@@ -216,8 +253,12 @@ def extra_batched(cpu_seconds: float, do_cpu: bool) -> dict:
     mism = 0
     for bb, pb in zip(batch.bundles, bundles):
         mism += int((bb.ranges != pb.ranges).sum().item()) + int((bb.hits != pb.hits).sum().item())
+    o1dn_rays = sum(m.size() for m, _ in sensors if m.kind == 2)
+    alg = 5 * rays + 12 * o1dn_rays + 32 * len(sensors)
     out = {"workload": f"{len(F)} tris (random boxes + icospheres, 200 m cube), {len(sensors)} sensors "
                        f"(800 VLP-16, 600 pinhole 640x480 fov 90, 600 O1Dn 1k-50k rays), hits+ranges",
+           "roofline": roofline("k_batch (one persistent launch, tuned)", alg, ms, BATCH_PROFILE,
+                                "algorithmic bytes: hits + ranges (5 B/ray), O1Dn directions (12 B/ray), poses"),
            "rays": rays, "launch": "one persistent launch", "batched_ms": round(ms, 3),
            "batched_mrays_s": round(rays / ms / 1e3, 1), "two_launches_ms": round(split_ms, 3),
            "per_sensor_launches_ms": round(plain_ms, 3), "per_sensor_mrays_s": round(rays / plain_ms / 1e3, 1),
@@ -293,8 +334,12 @@ def extra_pose_grad() -> dict:
     b.synchronize()
     plain = (time.perf_counter() - t) * 1e3
     same = bool(torch.equal(torch.cat(outs), out))
+    alg = 5 * 14400 + 32 * n + 64 * n
     return {"workload": f"{len(F)} tris room, {n} VLP-16 particles (sigma 0.2 m, 3 deg), Huber(0.05) on "
                         "true-pose ranges + N(0, 0.01 m)",
+            "roofline": roofline("k_pose_grad + k_pose_grad_reduce (one evaluation)", alg, batched, None,
+                                 "algorithmic bytes: the shared target scan (4 B range + 1 B mask per ray, read once), "
+                                 "poses in, 8 doubles per pose out; the kernel is bound by traversal latency"),
             "rays_per_eval": n * 14400, "batched_ms_per_eval": round(batched, 3),
             "batched_evals_s": round(1e3 / batched, 2), "plain_ms_per_eval": round(plain, 3),
             "plain_evals_s": round(1e3 / plain, 3), "batched_over_plain": round(plain / batched, 1),
@@ -328,16 +373,35 @@ def extra_icp() -> dict:
     # sparse scene unless they are gated: 0.5 m point-to-plane gate
     rcc = icp.BatchedRCC(hm, model, data.points, data.hits, torch.as_tensor(hyp, device="cuda"), max_dist=0.5)
     s = torch.cuda.Stream()
-    rcc.capture(10, s)
+    graph = rcc.capture(10, s)
     torch.cuda.synchronize()
     a, b = _events()
     a.record(s)
-    rcc.graph.launch(s)
+    graph.launch(s)
     b.record(s)
     b.synchronize()
     ms = a.elapsed_time(b)
     err0 = np.linalg.norm(hyp[:, 4:7], axis=1)
     err1 = np.linalg.norm(rcc.Tbm.cpu().numpy()[:, 4:7], axis=1)
+    grad_norm = float(np.median(np.linalg.norm(rcc.gradient().cpu().numpy(), axis=1)))
+    # the iteration's kernels timed one by one (eager, same buffers): the simulate and one
+    # point-to-plane statistics pass
+    sim_ms, st_ms = [], []
+    for _ in range(3):
+        a, b = _events()
+        a.record()
+        rcc.sim.simulate(rcc.Tbm, rcc.model_scan)
+        b.record()
+        b.synchronize()
+        sim_ms.append(a.elapsed_time(b))
+        a, b = _events()
+        a.record()
+        rcc._statistics(None)
+        b.record()
+        b.synchronize()
+        st_ms.append(a.elapsed_time(b))
+    sim_ms, st_ms = min(sim_ms), min(st_ms)
+    rays_it = n * 32768
     # FP64 Umeyama of the last statistics against the FP64 CPU oracle (LAPACK SVD)
     from oracle import stats as S
     from rmagine_amd import math as M
@@ -353,6 +417,15 @@ def extra_icp() -> dict:
     return {"workload": f"{len(F)} tris, {n} hypotheses x 32768 O1Dn rays, 10 x (simulate + 5 x (p2l FP64 "
                         "stats + FP64 Umeyama + update)), one HIP graph",
             "graph_ms_10_iterations": round(ms, 3), "ms_per_iteration": round(ms / 10, 3),
+            "roofline": roofline("k_simulate<O1DN> (hits, points, normals)", 25 * rays_it + 12 * 32768 + 32 * n, sim_ms,
+                                 ICP_PROFILE, "algorithmic bytes: hits + points + normals written (25 B/ray), the "
+                                              "shared direction list, poses"),
+            "roofline_statistics": roofline("k_stats<uint32_t> (p2l, FP64)", 25 * rays_it + 13 * 32768 + 160 * n, st_ms,
+                                            None, "algorithmic bytes: model points + normals + mask (25 B per "
+                                                  "correspondence), the shared dataset scan, pre-transforms and "
+                                                  "statistics"),
+            "simulate_ms": round(sim_ms, 3), "statistics_ms": round(st_ms, 4),
+            "gauss_newton_gradient_norm_median": grad_norm,
             "simulated_mrays_s": round(10 * n * 32768 / ms / 1e3, 1),
             "median_translation_error_m": {"before": round(float(np.median(err0)), 4),
                                            "after": round(float(np.median(err1)), 6)},
@@ -457,6 +530,12 @@ def main():
         st["triangle_bytes_per_ray"] = 48 * st["triangle_tests_per_ray"]
         print(json.dumps({"visit_stats": st, "workload": f"terrain {args.grid}^2, first {args.visit_stats} of "
                                                         f"{len(Tbm)} Ouster-style poses"}))
+        return
+
+    if args.extra_only:
+        fn = {"batched": lambda: extra_batched(0.0, False, tune=False), "pose_gradient": extra_pose_grad,
+              "icp": extra_icp}[args.extra_only]
+        print(json.dumps({args.extra_only: fn()}))
         return
 
     from rmagine_amd import scenes

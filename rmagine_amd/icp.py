@@ -10,8 +10,13 @@ The reference runs one hypothesis with a host round trip per inner iteration
 Here every step runs on the device for all hypotheses at once: one simulate over all
 hypothesis poses, segmented FP64 p2l statistics (the observed dataset scan shared by every
 hypothesis, the model scan per hypothesis), a per-hypothesis FP64 Umeyama solve and the
-pose updates, with no host synchronisation.  `run(..., graph=True)` records the outer
-iterations once into a HIP graph and replays it.
+pose updates, with no host synchronisation.  After the inner iterations every outer
+iteration also evaluates, per hypothesis, the point-to-plane Gauss-Newton system at the
+corrected pose (build_linear_system_p2l, src/rmagine_core/include/rmagine/math/
+optimization.h:170-200): `grad` holds J^T w r (the gradient of (1/2) sum w r^2 over the pose
+update, rotation first), J^T w J and sum w r^2 of the last outer iteration.
+`run(..., graph=True)` records the outer iterations into a HIP graph (one per iteration
+count) and replays it.
 """
 from __future__ import annotations
 
@@ -23,7 +28,8 @@ from .simulation import Bundle, HipGraph, HipMap, resize_memory_bundle, simulato
 
 class BatchedRCC:
     def __init__(self, map: HipMap, model, dataset_points: torch.Tensor, dataset_mask: torch.Tensor,
-                 Tbm_init: torch.Tensor, n_inner: int = 5, max_dist: float = 100.0, device="cuda"):
+                 Tbm_init: torch.Tensor, n_inner: int = 5, max_dist: float = 100.0, kernel_scale: float = 5.0,
+                 device="cuda"):
         self.device = torch.device(device)
         self.sim = simulator_for(model, map, device=device)
         self.W, self.H = model.getWidth(), model.getHeight()
@@ -39,9 +45,12 @@ class BatchedRCC:
         self.Tpre = torch.empty((self.n, 8), dtype=torch.float32, device=self.device)
         self.Tnext = torch.empty_like(self.Tpre)
         self.stats = torch.empty((self.n, 16), dtype=torch.float64, device=self.device)
+        self.kernel_scale = float(kernel_scale)
+        self.grad = torch.zeros((self.n, M.P2L_SYSTEM_COLS), dtype=torch.float64, device=self.device)
         ws = M._workspace(self.W * self.H, self.n, self.device)
         self.workspace = ws
-        self.graph = None
+        self.graphs: dict[int, HipGraph] = {}
+        self._stream: torch.cuda.Stream | None = None
 
     def _statistics(self, stream):
         from . import _lib, types as T
@@ -63,18 +72,33 @@ class BatchedRCC:
             self._statistics(stream)
             M.umeyama_transform(self.stats, out=self.Tnext, stream=stream)
             M.transform_mul(self.Tpre, self.Tnext, out=self.Tpre, stream=stream)
+        # the gradient at the corrected pose, before the pose update consumes Tpre
+        M.linear_system_p2l(self.Tpre, self.dataset, self.model_view, self.params, kernel_scale=self.kernel_scale,
+                            segment_size=self.W * self.H, n_segments=self.n, out=self.grad,
+                            workspace=self.workspace, stream=stream)
         M.transform_mul(self.Tbm, self.Tpre, out=self.Tbm, stream=stream)
 
     def capture(self, n_outer: int, stream: torch.cuda.Stream) -> HipGraph:
-        self.graph = HipGraph.capture(lambda s: [self.iteration(s) for _ in range(n_outer)], stream)
-        return self.graph
+        g = HipGraph.capture(lambda s: [self.iteration(s) for _ in range(n_outer)], stream)
+        self.graphs[n_outer] = g
+        return g
+
+    def gradient(self) -> torch.Tensor:
+        """(n, 6) J^T w r of the last outer iteration (rotation xyz, then translation xyz)."""
+        return self.grad[:, 36:42]
 
     def run(self, n_outer: int = 10, graph: bool = False, stream: torch.cuda.Stream | None = None) -> torch.Tensor:
         if graph:
-            stream = stream or torch.cuda.Stream()
-            if self.graph is None:
-                self.capture(n_outer, stream)
-            self.graph.launch(stream)
+            if stream is None:
+                if self._stream is None:
+                    self._stream = torch.cuda.Stream()
+                stream = self._stream
+            g = self.graphs.get(n_outer)
+            if g is None:
+                g = self.capture(n_outer, stream)
+            # the replay reads Tbm and the dataset: wait for work queued on the caller's stream
+            stream.wait_stream(torch.cuda.current_stream())
+            g.launch(stream)
             stream.synchronize()
         else:
             for _ in range(n_outer):

@@ -350,9 +350,21 @@ def test_batched_rcc_graph_equals_eager_and_oracle(room):
     graphed = icp.BatchedRCC(hm, model, data.points, data.hits, torch.as_tensor(hyp, device=dev))
     s = torch.cuda.Stream()
     graphed.capture(2, s)
-    graphed.graph.launch(s)
+    graphed.graphs[2].launch(s)
     s.synchronize()
     np.testing.assert_array_equal(graphed.Tbm.cpu().numpy(), e1)
+    np.testing.assert_array_equal(graphed.grad.cpu().numpy(), eager.grad.cpu().numpy())
+    # one graph per iteration count: 1 then 2 replayed iterations equal 3 eager ones, and a
+    # replay waits for an update of Tbm queued on the caller's stream
+    again = icp.BatchedRCC(hm, model, data.points, data.hits, torch.as_tensor(hyp, device=dev))
+    again.run(1, graph=True)
+    again.run(2, graph=True)
+    assert sorted(again.graphs) == [1, 2]
+    e3 = icp.BatchedRCC(hm, model, data.points, data.hits, torch.as_tensor(hyp, device=dev))
+    np.testing.assert_array_equal(again.Tbm.cpu().numpy(), e3.run(3).cpu().numpy())
+    again.Tbm.copy_(torch.as_tensor(hyp, device=dev))
+    again.run(2, graph=True)
+    np.testing.assert_array_equal(again.Tbm.cpu().numpy(), e1)
     # CPU oracle loop (oracle simulate, FP64 statistics and Umeyama), first two hypotheses
     from tests import npref
     dpts = data.points.cpu().numpy()

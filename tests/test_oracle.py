@@ -180,3 +180,29 @@ def test_multi_mesh_ids():
     r = oracle.simulate(om, m, T0)
     assert r["geom_ids"][0] == 1 and r["object_ids"][0] == 1 and r["face_ids"][0] in (0, 1)
     assert r["ranges"][0] == 1.0
+
+
+def test_diff_report_counts():
+    """oracle/diff.py restates apps/rmagine_simulation_diff_calc/Main.cpp:83-147 (both hit,
+    only one, neither, high differences, max and average over both-hit rays) and adds exact
+    mismatch counts and the unflagged-mismatch count of the parity bar."""
+    from oracle import diff as D
+    nan = np.nan
+    g = {"hits": np.array([1, 0, 1, 1, 0], np.uint8), "ranges": np.array([1, 101, 2, 3, 101], np.float32),
+         "points": np.array([[0, 0, 0], [nan] * 3, [1, 1, 1], [2, 2, 2], [nan] * 3], np.float32),
+         "face_ids": np.array([4, T.UINT_MAX, 2, 3, T.UINT_MAX], np.uint32)}
+    o = {"hits": np.array([1, 1, 0, 1, 0], np.uint8), "ranges": np.array([1, 5, 101, 3.001, 101], np.float32),
+         "points": np.array([[0, 0, 0], [1, 1, 1], [nan] * 3, [2, 2, 2.5], [nan] * 3], np.float32),
+         "face_ids": np.array([4, 9, T.UINT_MAX, 3, T.UINT_MAX], np.uint32),
+         "flags": np.array([0, oracle.FLAG_EDGE, oracle.FLAG_TIE | oracle.FLAG_RANGE, 0, 0], np.uint8)}
+    r = D.compare(g, o, 100.0)
+    assert r["diff"]["ranges"] == {"both_hit": 2, "only_a": 1, "only_b": 1, "neither": 1, "both_hit_high_diff": 1,
+                                   "max_diff": pytest.approx(0.001, rel=1e-3),
+                                   "avg_diff": pytest.approx(0.0005, rel=1e-3)}
+    assert r["diff"]["points"]["both_hit_high_diff"] == 1 and r["diff"]["points"]["max_diff"] == pytest.approx(0.5)
+    assert r["mismatch"] == {"hits": 2, "ranges": 3, "points": 3, "face_ids": 2}
+    assert r["flagged"] == {"any": 2, "edge": 1, "near_miss": 0, "tie": 1, "range_limit": 1}
+    assert r["mismatching_rays"] == 3 and r["mismatching_unflagged"] == 1
+    m = D.merge(D.merge(None, r), r)
+    assert m["rays"] == 10 and m["mismatching_unflagged"] == 2
+    assert m["diff"]["ranges"]["both_hit"] == 4 and m["diff"]["ranges"]["avg_diff"] == pytest.approx(0.0005, rel=1e-3)
