@@ -127,7 +127,7 @@ using HipMapPtr = std::shared_ptr<HipMap>;
 // (src/rmagine_embree/include/rmagine/map/EmbreeMap.hpp:94-117,
 //  src/rmagine_optix/include/rmagine/map/OptixMap.hpp:88-109).  flags:
 // RMH_SCENE_POLYGONS_FAN triangulates polygons (default: their first three indices).
-static HipMapPtr import_hip_map(const std::string& meshfile, uint32_t flags = 0,
+inline HipMapPtr import_hip_map(const std::string& meshfile, uint32_t flags = 0,
                                 const rmh_bvh_params* params = nullptr) {
   rmh_scene* sc = nullptr;
   RM_RMH_CHECK(rmh_scene_load(meshfile.c_str(), flags, &sc));

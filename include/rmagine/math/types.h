@@ -4,6 +4,8 @@
 //   Quaternion_ Quaternion.hpp:61-64, Quaternion.tcc:19-75
 //   Transform_ Transform.hpp:63-68 (32 bytes with the stamp), Transform.tcc:24-62
 //   EulerAngles_ -> Quaternion  EulerAngles.tcc:40-61
+//   Quaternion -> EulerAngles_  Quaternion.tcc:206-250
+//   AABB_ (min, max)            AABB.hpp:50-53
 //   Matrix3x3 (column-major) Matrix.hpp:184
 //   CrossStatistics_  CrossStatistics.hpp:50-60, CrossStatistics.tcc:9-58
 #pragma once
@@ -12,6 +14,9 @@
 #include <cstdint>
 
 namespace rmagine {
+
+template <typename DataT>
+struct EulerAngles_;
 
 template <typename DataT>
 struct Vector3_ {
@@ -61,6 +66,7 @@ struct Quaternion_ {
   Quaternion_ operator*(const Quaternion_& q) const { return mult(q); }
   Vector3_<DataT> operator*(const Vector3_<DataT>& v) const { return mult(v); }
   Quaternion_ operator~() const { return inv(); }
+  operator EulerAngles_<DataT>() const;
 };
 
 template <typename DataT>
@@ -74,6 +80,21 @@ struct EulerAngles_ {
             cr * cp * cy + sr * sp * sy};
   }
 };
+
+// Quaternion -> roll, pitch, yaw (Quaternion.tcc:206-250); pitch is +-pi/2 where |sin| >= 1
+template <typename DataT>
+Quaternion_<DataT>::operator EulerAngles_<DataT>() const {
+  const DataT sinr_cosp = 2.0 * (w * x + y * z);
+  const DataT cosr_cosp = 1.0 - 2.0 * (x * x + y * y);
+  const DataT sinp = 2.0 * (w * y - z * x);
+  const DataT siny_cosp = 2.0 * (w * z + x * y);
+  const DataT cosy_cosp = 1.0 - 2.0 * (y * y + z * z);
+  EulerAngles_<DataT> e;
+  e.roll = std::atan2(sinr_cosp, cosr_cosp);
+  e.pitch = std::fabs(sinp) >= DataT(1) ? std::copysign(DataT(M_PI_2), sinp) : std::asin(sinp);
+  e.yaw = std::atan2(siny_cosp, cosy_cosp);
+  return e;
+}
 
 template <typename DataT>
 struct Transform_ {
@@ -133,6 +154,12 @@ struct CrossStatistics_ {
   unsigned int n_meas;
 };
 
+template <typename DataT>
+struct AABB_ {
+  Vector3_<DataT> min;
+  Vector3_<DataT> max;
+};
+
 using Vector3 = Vector3_<float>;
 using Vector = Vector3;
 using Point = Vector3;
@@ -142,8 +169,12 @@ using Transform = Transform_<float>;
 using Matrix3x3 = Matrix3x3_<float>;
 using CrossStatistics = CrossStatistics_<float>;
 using CrossStatisticsD = CrossStatistics_<double>;
+using AABB = AABB_<float>;
 
 static_assert(sizeof(Transform) == 32, "Transform must be 32 bytes");
+static_assert(sizeof(CrossStatistics) == 64, "CrossStatistics_<float> is 64 bytes");
+static_assert(sizeof(CrossStatisticsD) == 128, "CrossStatistics_<double> has the layout of rmh_cross_stats");
+static_assert(sizeof(AABB) == 24, "AABB is 24 bytes");
 
 struct Face {
   uint32_t v0, v1, v2;

@@ -94,6 +94,22 @@ class SimulatorHip {
     return res;
   }
 
+  // poses in host memory (a Memory<Transform, RAM> or a make_view of one pose): uploaded
+  // first, as the reference's simulate(const Memory<Transform, VRAM_CUDA>&) converts them
+  template <typename BundleT>
+  void simulate(const MemoryView<Transform, RAM>& Tbm, BundleT& res) const {
+    Memory<Transform, VRAM_HIP> T(Tbm);
+    simulate(T, res);
+  }
+
+  template <typename BundleT>
+  BundleT simulate(const MemoryView<Transform, RAM>& Tbm) const {
+    BundleT res;
+    resize_memory_bundle<VRAM_HIP>(res, m_width, m_height, static_cast<unsigned int>(Tbm.size()));
+    simulate(Tbm, res);
+    return res;
+  }
+
   // single pose from the host (SimulatorOptix.tcc:6-24 uploads it)
   template <typename BundleT>
   void simulate(const Transform& Tbm, BundleT& res) const {

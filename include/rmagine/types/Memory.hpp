@@ -37,6 +37,8 @@ class MemoryView {
 
   MemoryView() = delete;
   MemoryView(DataT* mem, size_t N) : m_mem(mem), m_size(N) {}
+  // copying a view copies the (pointer, size) pair; assignment copies the elements
+  MemoryView(const MemoryView&) = default;
 
   static MemoryView<DataT, MemT> Empty() { return MemoryView<DataT, MemT>(nullptr, 0); }
 
@@ -112,12 +114,13 @@ class Memory : public MemoryView<DataT, MemT> {
     if (m_mem) MemT::template free<DataT>(m_mem, m_size);
   }
 
-  // like VRAM_CUDA::realloc (MemoryCuda.tcc:15-22) the contents are not preserved
-  // across a size change of device memory
+  // Memory::resize (Memory.tcc:126-137): the memory space's realloc keeps the first
+  // min(old, new) elements in RAM and drops the contents of device memory, as
+  // VRAM_CUDA::realloc does (MemoryCuda.tcc:15-22)
   void resize(size_t N) {
     if (N == m_size) return;
-    if (m_mem) MemT::template free<DataT>(m_mem, m_size);
-    m_mem = MemT::template alloc<DataT>(N);
+    if (m_mem) m_mem = MemT::template realloc<DataT>(m_mem, m_size, N);
+    else m_mem = MemT::template alloc<DataT>(N);
     m_size = N;
   }
 

@@ -482,6 +482,19 @@ int rmh_linear_system_p2l(const rmh_transform* pre_transforms_dev, uint32_t n_pr
  * out64_dev (nullable) 7 doubles per entry (qx, qy, qz, qw, tx, ty, tz). */
 int rmh_umeyama(const rmh_cross_stats* stats_dev, uint32_t n, rmh_transform* out_dev, double* out64_dev,
                 void* stream);
+/* The same FP64 Umeyama on the host (host arrays): replaces the CPU umeyama_transform
+ * (src/rmagine_core/src/math/optimization.cpp:12-55) with the device's algorithm. */
+int rmh_umeyama_host(const rmh_cross_stats* stats, uint32_t n, rmh_transform* out, double* out64);
+
+/* FP64 statistics -> Rmagine's CrossStatistics_<float> (CrossStatistics.hpp:50-60: dataset
+ * mean, model mean, column-major covariance as 15 floats, then n_meas; 64 bytes each), device
+ * arrays; and FP32 (dataset means, model means, column-major covariances, counts or NULL for
+ * n_meas = 1) -> FP64 statistics, for the batched umeyama_transform(ds, ms, Cs, n_meas) of
+ * src/rmagine_cuda/include/rmagine/math/memory_math.cuh:590-618. */
+int rmh_cross_stats_to_f32(const rmh_cross_stats* in_dev, void* out_dev, uint32_t n, void* stream);
+int rmh_cross_stats_pack(const rmh_vec3* dataset_means_dev, const rmh_vec3* model_means_dev,
+                         const float* covariances_dev, const uint32_t* n_meas_dev, rmh_cross_stats* out_dev,
+                         uint32_t n, void* stream);
 
 #ifdef __cplusplus
 }

@@ -99,6 +99,9 @@ _SIGNATURES = {
     "rmh_statistics_noalloc": (c_int, [c_int, c_void_p, c_u32, ctypes.POINTER(T.rmh_cloud),
                                        ctypes.POINTER(T.rmh_cloud), ctypes.c_uint64, c_u32,
                                        ctypes.POINTER(T.rmh_umeyama_constraints), c_void_p, c_void_p]),
+    "rmh_umeyama_host": (c_int, [c_void_p, c_u32, c_void_p, c_void_p]),
+    "rmh_cross_stats_to_f32": (c_int, [c_void_p, c_void_p, c_u32, c_void_p]),
+    "rmh_cross_stats_pack": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_u32, c_void_p]),
     "rmh_transform_pack_matrix": (c_int, [c_void_p, c_void_p, c_void_p, c_u32, c_void_p]),
     "rmh_transform_pack_quat": (c_int, [c_void_p, c_void_p, c_void_p, c_u32, c_void_p]),
     "rmh_statistics_objectwise_p2l": (c_int, [c_void_p, ctypes.POINTER(T.rmh_cloud), ctypes.POINTER(T.rmh_cloud), c_u32,

@@ -581,13 +581,13 @@ __global__ void k_p2l_system_combine(const LinArgs a) {
 
 // ---- Umeyama ------------------------------------------------------------------------------
 
-__device__ __forceinline__ double det3(const double* M) {  // row-major
+__host__ __device__ __forceinline__ double det3(const double* M) {  // row-major
   return M[0] * (M[4] * M[8] - M[5] * M[7]) - M[1] * (M[3] * M[8] - M[5] * M[6]) + M[2] * (M[3] * M[7] - M[4] * M[6]);
 }
 
 // One-sided Jacobi SVD of a 3x3 matrix A (row-major): A = U diag(s) V^T, s descending,
 // U and V orthonormal (U completed by cross products for rank-deficient A).
-__device__ void svd3(const double* Ain, double* U, double* s, double* V) {
+__host__ __device__ void svd3(const double* Ain, double* U, double* s, double* V) {
   double A[9];
   for (int k = 0; k < 9; k++) A[k] = Ain[k];
   for (int k = 0; k < 9; k++) V[k] = (k % 4 == 0) ? 1.0 : 0.0;
@@ -653,64 +653,110 @@ __device__ void svd3(const double* Ain, double* U, double* s, double* V) {
 // converted to a quaternion with the reference's matrix -> quaternion formula
 // (src/rmagine_core/include/rmagine/math/types/MatrixOps.tcc:1492-1529), normalised;
 // a non-finite rotation falls back to identity; t = m - R d; n_meas == 0 -> identity.
-__global__ void k_umeyama(const rmh_cross_stats* st, uint32_t n, rmh_transform* out, double* out64) {
-  const uint32_t i = blockIdx.x * kBlk + threadIdx.x;
-  if (i >= n) return;
-  const rmh_cross_stats S = st[i];
-  double q[4] = {0.0, 0.0, 0.0, 1.0};
-  double t[3] = {0.0, 0.0, 0.0};
-  if (S.n_meas > 0) {
-    double C[9];
+// The same code runs on the device (k_umeyama) and on the host (rmh_umeyama_host).
+// v - v is 0 for finite v and NaN for infinities and NaN (no fast-math here)
+__host__ __device__ inline bool finite_hd(double v) { return v - v == 0.0; }
+
+__host__ __device__ void umeyama_one(const rmh_cross_stats& S, double q[4], double t[3]) {
+  q[0] = 0.0; q[1] = 0.0; q[2] = 0.0; q[3] = 1.0;
+  t[0] = 0.0; t[1] = 0.0; t[2] = 0.0;
+  if (S.n_meas == 0) return;
+  double C[9];
+  for (int r = 0; r < 3; r++)
+    for (int c = 0; c < 3; c++) C[r * 3 + c] = S.covariance[c * 3 + r];
+  double U[9], sv[3], V[9];
+  svd3(C, U, sv, V);
+  bool ok = sv[0] > 0.0;
+  if (ok) {
+    const double sd = (det3(U) * det3(V) < 0.0) ? -1.0 : 1.0;
+    double R[9];
     for (int r = 0; r < 3; r++)
-      for (int c = 0; c < 3; c++) C[r * 3 + c] = S.covariance[c * 3 + r];
-    double U[9], sv[3], V[9];
-    svd3(C, U, sv, V);
-    bool ok = sv[0] > 0.0;
-    if (ok) {
-      const double sd = (det3(U) * det3(V) < 0.0) ? -1.0 : 1.0;
-      double R[9];
-      for (int r = 0; r < 3; r++)
-        for (int c = 0; c < 3; c++)
-          R[r * 3 + c] = U[r * 3 + 0] * V[c * 3 + 0] + U[r * 3 + 1] * V[c * 3 + 1] + sd * U[r * 3 + 2] * V[c * 3 + 2];
-      const double tr = R[0] + R[4] + R[8];
-      double x, y, z, w;
-      if (tr > 0) {
-        const double s4 = sqrt(tr + 1.0) * 2.0;
-        w = 0.25 * s4; x = (R[7] - R[5]) / s4; y = (R[2] - R[6]) / s4; z = (R[3] - R[1]) / s4;
-      } else if (R[0] > R[4] && R[0] > R[8]) {
-        const double s4 = sqrt(1.0 + R[0] - R[4] - R[8]) * 2.0;
-        w = (R[7] - R[5]) / s4; x = 0.25 * s4; y = (R[1] + R[3]) / s4; z = (R[2] + R[6]) / s4;
-      } else if (R[4] > R[8]) {
-        const double s4 = sqrt(1.0 + R[4] - R[0] - R[8]) * 2.0;
-        w = (R[2] - R[6]) / s4; x = (R[1] + R[3]) / s4; y = 0.25 * s4; z = (R[5] + R[7]) / s4;
-      } else {
-        const double s4 = sqrt(1.0 + R[8] - R[0] - R[4]) * 2.0;
-        w = (R[3] - R[1]) / s4; x = (R[2] + R[6]) / s4; y = (R[5] + R[7]) / s4; z = 0.25 * s4;
-      }
-      const double l = sqrt(x * x + y * y + z * z + w * w);
-      x /= l; y /= l; z /= l; w /= l;
-      ok = isfinite(x) && isfinite(y) && isfinite(z) && isfinite(w);
-      if (ok) { q[0] = x; q[1] = y; q[2] = z; q[3] = w; }
+      for (int c = 0; c < 3; c++)
+        R[r * 3 + c] = U[r * 3 + 0] * V[c * 3 + 0] + U[r * 3 + 1] * V[c * 3 + 1] + sd * U[r * 3 + 2] * V[c * 3 + 2];
+    const double tr = R[0] + R[4] + R[8];
+    double x, y, z, w;
+    if (tr > 0) {
+      const double s4 = sqrt(tr + 1.0) * 2.0;
+      w = 0.25 * s4; x = (R[7] - R[5]) / s4; y = (R[2] - R[6]) / s4; z = (R[3] - R[1]) / s4;
+    } else if (R[0] > R[4] && R[0] > R[8]) {
+      const double s4 = sqrt(1.0 + R[0] - R[4] - R[8]) * 2.0;
+      w = (R[7] - R[5]) / s4; x = 0.25 * s4; y = (R[1] + R[3]) / s4; z = (R[2] + R[6]) / s4;
+    } else if (R[4] > R[8]) {
+      const double s4 = sqrt(1.0 + R[4] - R[0] - R[8]) * 2.0;
+      w = (R[2] - R[6]) / s4; x = (R[1] + R[3]) / s4; y = 0.25 * s4; z = (R[5] + R[7]) / s4;
+    } else {
+      const double s4 = sqrt(1.0 + R[8] - R[0] - R[4]) * 2.0;
+      w = (R[3] - R[1]) / s4; x = (R[2] + R[6]) / s4; y = (R[5] + R[7]) / s4; z = 0.25 * s4;
     }
-    // t = m - R * d, R applied as the quaternion sandwich
-    const double x = q[0], y = q[1], z = q[2], w = q[3];
-    const double d0 = S.dataset_mean[0], d1 = S.dataset_mean[1], d2 = S.dataset_mean[2];
-    const double r0 = (w * w + x * x - y * y - z * z) * d0 + 2.0 * (x * y - w * z) * d1 + 2.0 * (x * z + w * y) * d2;
-    const double r1 = 2.0 * (x * y + w * z) * d0 + (w * w - x * x + y * y - z * z) * d1 + 2.0 * (y * z - w * x) * d2;
-    const double r2 = 2.0 * (x * z - w * y) * d0 + 2.0 * (y * z + w * x) * d1 + (w * w - x * x - y * y + z * z) * d2;
-    t[0] = S.model_mean[0] - r0;
-    t[1] = S.model_mean[1] - r1;
-    t[2] = S.model_mean[2] - r2;
+    const double l = sqrt(x * x + y * y + z * z + w * w);
+    x /= l; y /= l; z /= l; w /= l;
+    ok = finite_hd(x) && finite_hd(y) && finite_hd(z) && finite_hd(w);
+    if (ok) { q[0] = x; q[1] = y; q[2] = z; q[3] = w; }
   }
+  // t = m - R * d, R applied as the quaternion sandwich
+  const double x = q[0], y = q[1], z = q[2], w = q[3];
+  const double d0 = S.dataset_mean[0], d1 = S.dataset_mean[1], d2 = S.dataset_mean[2];
+  const double r0 = (w * w + x * x - y * y - z * z) * d0 + 2.0 * (x * y - w * z) * d1 + 2.0 * (x * z + w * y) * d2;
+  const double r1 = 2.0 * (x * y + w * z) * d0 + (w * w - x * x + y * y - z * z) * d1 + 2.0 * (y * z - w * x) * d2;
+  const double r2 = 2.0 * (x * z - w * y) * d0 + 2.0 * (y * z + w * x) * d1 + (w * w - x * x - y * y + z * z) * d2;
+  t[0] = S.model_mean[0] - r0;
+  t[1] = S.model_mean[1] - r1;
+  t[2] = S.model_mean[2] - r2;
+}
+
+__host__ __device__ inline void umeyama_store(const double q[4], const double t[3], uint32_t i, rmh_transform* out,
+                                              double* out64) {
   rmh_transform T;
   T.R.x = (float)q[0]; T.R.y = (float)q[1]; T.R.z = (float)q[2]; T.R.w = (float)q[3];
   T.t.x = (float)t[0]; T.t.y = (float)t[1]; T.t.z = (float)t[2];
   T.stamp = 0;
   if (out) out[i] = T;
   if (out64) {
-    for (int k = 0; k < 4; k++) out64[7 * i + k] = q[k];
-    for (int k = 0; k < 3; k++) out64[7 * i + 4 + k] = t[k];
+    for (int k = 0; k < 4; k++) out64[7 * static_cast<size_t>(i) + k] = q[k];
+    for (int k = 0; k < 3; k++) out64[7 * static_cast<size_t>(i) + 4 + k] = t[k];
   }
+}
+
+__global__ void k_umeyama(const rmh_cross_stats* st, uint32_t n, rmh_transform* out, double* out64) {
+  const uint32_t i = blockIdx.x * kBlk + threadIdx.x;
+  if (i >= n) return;
+  double q[4], t[3];
+  umeyama_one(st[i], q, t);
+  umeyama_store(q, t, i, out, out64);
+}
+
+// CrossStatistics_<float> (CrossStatistics.hpp:50-60, 64 bytes): dataset mean, model mean,
+// covariance (column-major), n_meas
+struct CrossStatsF32 {
+  float dataset_mean[3], model_mean[3], covariance[9];
+  uint32_t n_meas;
+};
+
+__global__ void k_stats_to_f32(const rmh_cross_stats* in, CrossStatsF32* out, uint32_t n) {
+  const uint32_t i = blockIdx.x * kBlk + threadIdx.x;
+  if (i >= n) return;
+  const rmh_cross_stats S = in[i];
+  CrossStatsF32 r;
+  for (int k = 0; k < 3; k++) { r.dataset_mean[k] = (float)S.dataset_mean[k]; r.model_mean[k] = (float)S.model_mean[k]; }
+  for (int k = 0; k < 9; k++) r.covariance[k] = (float)S.covariance[k];
+  r.n_meas = S.n_meas;
+  out[i] = r;
+}
+
+// (dataset means, model means, covariances, counts) -> FP64 statistics; n_meas NULL -> 1
+// (the default of umeyama_transform, optimization.h:61-65)
+__global__ void k_stats_pack(const rmh_vec3* ds, const rmh_vec3* ms, const float* Cs, const uint32_t* n_meas,
+                             rmh_cross_stats* out, uint32_t n) {
+  const uint32_t i = blockIdx.x * kBlk + threadIdx.x;
+  if (i >= n) return;
+  rmh_cross_stats r;
+  const rmh_vec3 d = ds[i], m = ms[i];
+  r.dataset_mean[0] = d.x; r.dataset_mean[1] = d.y; r.dataset_mean[2] = d.z;
+  r.model_mean[0] = m.x; r.model_mean[1] = m.y; r.model_mean[2] = m.z;
+  for (int k = 0; k < 9; k++) r.covariance[k] = Cs[9 * static_cast<size_t>(i) + k];
+  r.n_meas = n_meas ? n_meas[i] : 1u;
+  r._pad = 0;
+  out[i] = r;
 }
 
 uint32_t blocks_per_segment(uint64_t seg, uint32_t nseg) {
@@ -929,6 +975,7 @@ constexpr size_t kScratchBytes = size_t(2) << 20;
 // the largest partial set rmh_statistics' partition makes is bps * n_segments < 16384 slots
 // of kMom doubles (bps > 1 only for n_segments <= 8191, and bps * n_segments < 8192 +
 // n_segments), 2 MiB
+static_assert(sizeof(CrossStatsF32) == 64, "CrossStatistics_<float> is 64 bytes");
 static_assert(sizeof(double) * kMom * 16384 <= kScratchBytes, "scratch holds the largest partial set");
 
 static __device__ __align__(256) double scratch[kScratchBytes / sizeof(double)];
@@ -1056,6 +1103,36 @@ int rmh_umeyama(const rmh_cross_stats* stats, uint32_t n, rmh_transform* out, do
   if (n == 0) return RMH_OK;
   if (!stats || (!out && !out64)) return fail(RMH_ERR_INVALID, "rmh_umeyama: NULL argument");
   k_umeyama<<<grid1(n), kBlk, 0, static_cast<hipStream_t>(stream)>>>(stats, n, out, out64);
+  RMH_HIP_TRY(hipGetLastError());
+  return RMH_OK;
+}
+
+int rmh_umeyama_host(const rmh_cross_stats* stats, uint32_t n, rmh_transform* out, double* out64) {
+  if (n == 0) return RMH_OK;
+  if (!stats || (!out && !out64)) return fail(RMH_ERR_INVALID, "rmh_umeyama_host: NULL argument");
+  for (uint32_t i = 0; i < n; i++) {
+    double q[4], t[3];
+    umeyama_one(stats[i], q, t);
+    umeyama_store(q, t, i, out, out64);
+  }
+  return RMH_OK;
+}
+
+int rmh_cross_stats_to_f32(const rmh_cross_stats* in, void* out, uint32_t n, void* stream) {
+  if (n == 0) return RMH_OK;
+  if (!in || !out) return fail(RMH_ERR_INVALID, "rmh_cross_stats_to_f32: NULL argument");
+  k_stats_to_f32<<<grid1(n), kBlk, 0, static_cast<hipStream_t>(stream)>>>(in, static_cast<CrossStatsF32*>(out), n);
+  RMH_HIP_TRY(hipGetLastError());
+  return RMH_OK;
+}
+
+int rmh_cross_stats_pack(const rmh_vec3* dataset_means, const rmh_vec3* model_means, const float* covariances,
+                         const uint32_t* n_meas, rmh_cross_stats* out, uint32_t n, void* stream) {
+  if (n == 0) return RMH_OK;
+  if (!dataset_means || !model_means || !covariances || !out)
+    return fail(RMH_ERR_INVALID, "rmh_cross_stats_pack: NULL argument");
+  k_stats_pack<<<grid1(n), kBlk, 0, static_cast<hipStream_t>(stream)>>>(dataset_means, model_means, covariances,
+                                                                           n_meas, out, n);
   RMH_HIP_TRY(hipGetLastError());
   return RMH_OK;
 }

@@ -655,13 +655,23 @@ __device__ __forceinline__ V3 map_origin(const Tf& Tsm, const V3& os) {
 // kernel, and as plain stores their lines (6.8 GB per terrain launch) displaced BVH nodes
 // and triangles from the 4 MB per-XCD L2 (terrain: 10.04 -> 9.47 ms with streaming stores;
 // the bytes written are the same).
-template <class T>
-__device__ __forceinline__ void stream_out(T* p, T v) { __builtin_nontemporal_store(v, p); }
+// Ray lists (O1Dn, OnDn) are traced in direction-sorted order, so each lane writes to a
+// scattered buffer index: there plain stores let the L2 merge the partial lines that
+// neighbouring waves write before they go to HBM.
+template <int KIND>
+constexpr bool kStreamOut = !(KIND == RMH_MODEL_O1DN || KIND == RMH_MODEL_ONDN);
+
+template <int KIND, class T>
+__device__ __forceinline__ void stream_out(T* p, T v) {
+  if constexpr (kStreamOut<KIND>) __builtin_nontemporal_store(v, p);
+  else *p = v;
+}
+template <int KIND>
 __device__ __forceinline__ void stream_out3(rmh_vec3* p, float x, float y, float z) {
   float* f = &p->x;
-  __builtin_nontemporal_store(x, f);
-  __builtin_nontemporal_store(y, f + 1);
-  __builtin_nontemporal_store(z, f + 2);
+  stream_out<KIND>(f, x);
+  stream_out<KIND>(f + 1, y);
+  stream_out<KIND>(f + 2, z);
 }
 
 // Attribute epilogue (SphereSimulatorEmbree.tcc:104-249, O1DnSimulatorEmbree.tcc:132).
@@ -671,12 +681,12 @@ __device__ __forceinline__ void write_outputs(const rmh_outputs& out, uint64_t g
                                               const Hit& h) {
   if (h.slot != 0xFFFFFFFFu) {
     const float t = static_cast<float>(h.t);
-    if (out.hits) stream_out(&out.hits[glob], static_cast<uint8_t>((t >= g.range_min) ? 1 : 0));
-    if (out.ranges) stream_out(&out.ranges[glob], t);
+    if (out.hits) stream_out<KIND>(&out.hits[glob], static_cast<uint8_t>((t >= g.range_min) ? 1 : 0));
+    if (out.ranges) stream_out<KIND>(&out.ranges[glob], t);
     if (out.points) {
       V3 p = scale(ds, t);
       if constexpr (KIND == RMH_MODEL_O1DN || KIND == RMH_MODEL_ONDN) p = add(p, os);
-      stream_out3(&out.points[glob], p.x, p.y, p.z);
+      stream_out3<KIND>(&out.points[glob], p.x, p.y, p.z);
     }
     float4 nn = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     if (out.normals || out.object_ids) nn = map.normals[h.slot];  // w = object id
@@ -685,26 +695,26 @@ __device__ __forceinline__ void write_outputs(const rmh_outputs& out, uint64_t g
       V3 n = qrot(qconj(Tsm.R), v3(nn.x, nn.y, nn.z));
       if (dot(ds, n) > 0.0f) n = scale(n, -1.0f);
       n = normalize(n);
-      stream_out3(&out.normals[glob], n.x, n.y, n.z);
+      stream_out3<KIND>(&out.normals[glob], n.x, n.y, n.z);
     }
     if (out.face_ids || out.geom_ids || out.object_ids) {
       const size_t s = static_cast<size_t>(h.slot) * 3;
       const uint32_t face = __float_as_uint(map.tris[s].w);
       const uint32_t geom = __float_as_uint(map.tris[s + 1].w);
-      if (out.face_ids) stream_out(&out.face_ids[glob], face);
-      if (out.geom_ids) stream_out(&out.geom_ids[glob], geom);
+      if (out.face_ids) stream_out<KIND>(&out.face_ids[glob], face);
+      if (out.geom_ids) stream_out<KIND>(&out.geom_ids[glob], geom);
       // object id: the instance id for meshes of a multi-mesh instance, else the geom id
-      if (out.object_ids) stream_out(&out.object_ids[glob], __float_as_uint(nn.w));
+      if (out.object_ids) stream_out<KIND>(&out.object_ids[glob], __float_as_uint(nn.w));
     }
   } else {
     const float qnan = __builtin_nanf("");
-    if (out.hits) stream_out(&out.hits[glob], static_cast<uint8_t>(0));
-    if (out.ranges) stream_out(&out.ranges[glob], g.range_max + 1.0f);
-    if (out.points) stream_out3(&out.points[glob], qnan, qnan, qnan);
-    if (out.normals) stream_out3(&out.normals[glob], qnan, qnan, qnan);
-    if (out.face_ids) stream_out(&out.face_ids[glob], 0xFFFFFFFFu);
-    if (out.geom_ids) stream_out(&out.geom_ids[glob], 0xFFFFFFFFu);
-    if (out.object_ids) stream_out(&out.object_ids[glob], 0xFFFFFFFFu);
+    if (out.hits) stream_out<KIND>(&out.hits[glob], static_cast<uint8_t>(0));
+    if (out.ranges) stream_out<KIND>(&out.ranges[glob], g.range_max + 1.0f);
+    if (out.points) stream_out3<KIND>(&out.points[glob], qnan, qnan, qnan);
+    if (out.normals) stream_out3<KIND>(&out.normals[glob], qnan, qnan, qnan);
+    if (out.face_ids) stream_out<KIND>(&out.face_ids[glob], 0xFFFFFFFFu);
+    if (out.geom_ids) stream_out<KIND>(&out.geom_ids[glob], 0xFFFFFFFFu);
+    if (out.object_ids) stream_out<KIND>(&out.object_ids[glob], 0xFFFFFFFFu);
   }
 }
 

@@ -9,7 +9,7 @@ pytestmark = pytest.mark.gpu
 BUILD = Path(__file__).resolve().parent / "cpp" / "_build"
 
 
-@pytest.mark.parametrize("prog", ["simulation_kat"])
+@pytest.mark.parametrize("prog", ["simulation_kat", "correction_rcc"])
 def test_cpp_program(prog):
     exe = BUILD / prog
     assert exe.exists(), f"{exe} not built (run __graft_entry__.build())"
