@@ -98,3 +98,23 @@ def closest_hits_mt(V, F, orig, dirs, tmax):
             ts[i] = tt[j]
             fs[i] = idx[j]
     return ts, fs
+
+
+def qrot_many(q, P):
+    """qrot of every row of P (float32 (n, 3)) by one quaternion, the same FP32 operations as
+    qrot, vectorised."""
+    P = np.asarray(P, np.float32)
+    qx, qy, qz, qw = (f4(v) for v in q)
+    px, py, pz = P[:, 0], P[:, 1], P[:, 2]
+    pw = np.zeros_like(px)
+    # q * (p, 0)
+    ax = ((qw * px + qx * pw) + qy * pz) - qz * py
+    ay = ((qw * py - qx * pz) + qy * pw) + qz * px
+    az = ((qw * pz + qx * py) - qy * px) + qz * pw
+    aw = ((qw * pw - qx * px) - qy * py) - qz * pz
+    # (...) * q^-1
+    bx, by, bz, bw = -qx, -qy, -qz, qw
+    rx = ((aw * bx + ax * bw) + ay * bz) - az * by
+    ry = ((aw * by - ax * bz) + ay * bw) + az * bx
+    rz = ((aw * bz + ax * by) - ay * bx) + az * bw
+    return np.stack([rx, ry, rz], 1).astype(np.float32)

@@ -16,6 +16,7 @@ def _check(hm, V, F, Q, max_distance=C.FLT_MAX, label="", offsets=None):
     r = hm.closest_points(torch.as_tensor(Q, device="cuda"), max_distance)
     torch.cuda.synchronize()
     d, p, f = C.closest_points(V, F, Q, max_distance)
+    f_glob = f.copy()
     if offsets is not None:  # the library reports the face index within its mesh (primID)
         off = np.asarray(offsets)
         f = np.where(f >= 0, f - off[np.searchsorted(off, f, side="right") - 1], f)
@@ -26,8 +27,25 @@ def _check(hm, V, F, Q, max_distance=C.FLT_MAX, label="", offsets=None):
     np.testing.assert_allclose(gd[ok], d[ok], rtol=2e-7, atol=1e-7, err_msg=label)
     np.testing.assert_allclose(r["points"].cpu().numpy()[ok], p[ok], atol=2e-6, err_msg=label)
     np.testing.assert_array_equal(gf[~ok], -1)
-    same = gf[ok] == f[ok]  # a different face can only be an equal-distance tie
-    assert same.mean() > 0.99, label
+    # a different face than the oracle's is allowed only where the GPU's face is at the same
+    # FP64 distance (a tie up to the rounding of the two FP64 evaluations, 1e-12 relative) and
+    # is the lowest index among the tied faces
+    V64 = np.asarray(V, np.float32).astype(np.float64)
+    F64 = np.asarray(F, np.int64)
+    if offsets is not None:  # compare in global face indices
+        off = np.asarray(offsets)
+        geom = r["geom_ids"].cpu().numpy().astype(np.int64)
+        gf_glob = np.where(gf >= 0, gf + off[np.clip(geom, 0, len(off) - 1)], -1)
+    else:
+        gf_glob = gf
+    diff = np.nonzero(ok & (gf_glob != f_glob))[0]
+    for i in diff:
+        P = C.closest_on_triangles(Q[i].astype(np.float64), V64[F64[:, 0]], V64[F64[:, 1]], V64[F64[:, 2]])
+        d2 = ((P - Q[i].astype(np.float64)) ** 2).sum(1)
+        tied = np.nonzero(d2 <= d2.min() * (1 + 1e-12) + 1e-300)[0]
+        assert gf_glob[i] in tied, (label, i, gf_glob[i], f_glob[i], d2[gf_glob[i]], d2.min())
+        assert gf_glob[i] == tied.min(), (label, i, gf_glob[i], tied)
+    print(f"{label}: {ok.sum()} queries, {len(diff)} face ids differ from the oracle, all equal-distance ties")
     return r
 
 
