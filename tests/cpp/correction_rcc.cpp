@@ -7,9 +7,12 @@
 // reference test, the same run checks the other drop-ins it touches against each other:
 // partial statistics (a view of 7 outputs) merged with CrossStatistics' formula
 // (CrossStatistics.tcc:9-58) against the total, the batched device umeyama_transform
-// against the host one, multNxN / invert / pack / setIdentity.  Throws on failure.
+// against the host one, multNxN / invert / pack / setIdentity; and, where the reference test
+// prints them side by side, the device statistics against the host ones on RAM copies of the
+// clouds (correction_rcc.cpp:112-119,169-177).  Throws on failure.
 #include <rmagine/map/HipMap.hpp>
 #include <rmagine/math/memory_math_hip.hpp>
+#include <rmagine/math/statistics.h>
 #include <rmagine/math/statistics_hip.hpp>
 #include <rmagine/simulation/SimulatorHipTraits.hpp>
 #include <rmagine/types/PointCloud.hpp>
@@ -105,6 +108,11 @@ int main() {
 
   rm::PointCloudView_<rm::VRAM_HIP> cloud_dataset = {.points = dataset.points, .mask = dataset.hits};
 
+  // only for the CPU comparison (correction_rcc.cpp:112-119)
+  rm::PointCloud_<rm::RAM> cloud_dataset_data_ram = rm::transfer<rm::RAM>(cloud_dataset);
+  rm::PointCloudView_<rm::RAM> cloud_dataset_ram = {.points = cloud_dataset_data_ram.points,
+                                                    .mask = cloud_dataset_data_ram.mask};
+
   // MICP params (correction_rcc.cpp:132-139)
   const size_t n_outer = 10;
   const size_t n_inner = 5;
@@ -125,9 +133,28 @@ int main() {
     sim->simulate(rm::make_view(Tbm_est), model);
     rm::PointCloudView_<rm::VRAM_HIP> cloud_model = {.points = model.points, .mask = model.hits,
                                                      .normals = model.normals};
+    rm::PointCloud_<rm::RAM> cloud_model_ram_data = rm::transfer<rm::RAM>(cloud_model);
+    rm::PointCloudView_<rm::RAM> cloud_model_ram = {.points = cloud_model_ram_data.points,
+                                                    .mask = cloud_model_ram_data.mask,
+                                                    .normals = cloud_model_ram_data.normals};
     rm::Transform Tpre = rm::Transform::Identity();
     for (size_t j = 0; j < n_inner; j++) {
       rm::CrossStatistics stats = rm::statistics_p2l(Tpre, cloud_dataset, cloud_model, params);
+      rm::CrossStatistics stats2 = rm::statistics_p2l(Tpre, cloud_dataset_ram, cloud_model_ram, params);
+      {
+        // device (FP64 accumulation, FP64 gate) against host (FP32 gate, FP64 accumulation)
+        require(stats.n_meas == stats2.n_meas, "GPU and CPU statistics count different correspondences");
+        double err = 0.0, scale = 1e-6;
+        for (int k = 0; k < 9; k++) {
+          err = std::max(err, std::fabs(double(stats.covariance.data[k]) - stats2.covariance.data[k]));
+          scale = std::max(scale, std::fabs(double(stats2.covariance.data[k])));
+        }
+        const rm::Vector dd = stats.dataset_mean - stats2.dataset_mean, dmm = stats.model_mean - stats2.model_mean;
+        std::ostringstream ss;
+        ss << "GPU vs CPU statistics: covariance differs by " << err << " (scale " << scale << "), means by "
+           << dd.l2norm() << " / " << dmm.l2norm();
+        require(err <= 1e-5 * scale && dd.l2norm() <= 1e-5f && dmm.l2norm() <= 1e-5f, ss.str());
+      }
       if (j == 0) {
         // partial results over the reference's block partition, merged, equal the total
         rm::Memory<rm::CrossStatistics, rm::VRAM_HIP> parts_d(7);
