@@ -1,4 +1,6 @@
-// The *SimulatorHip classes: Rmagine's GPU Simulator API on librmh.
+// SimulatorHip: the base of Rmagine's GPU Simulator API on librmh (map, Tsb, simulate).  The
+// model classes derive from it in SphereSimulatorHip.hpp, PinholeSimulatorHip.hpp,
+// O1DnSimulatorHip.hpp and OnDnSimulatorHip.hpp.
 //
 // API shape of SimulatorOptix / SphereSimulatorOptix / PinholeSimulatorOptix /
 // O1DnSimulatorOptix / OnDnSimulatorOptix
@@ -164,83 +166,6 @@ class SimulatorHip {
   uint32_t m_width = 0, m_height = 0;
 };
 
-class SphereSimulatorHip : public SimulatorHip {
- public:
-  using SimulatorHip::SimulatorHip;
-  void setModel(const SphericalModel& model) {
-    m_model = model;
-    setSensor(RMH_MODEL_SPHERICAL, &model);
-  }
-  void setModel(const MemoryView<SphericalModel, RAM>& model) { setModel(model[0]); }
-  const SphericalModel& model() const { return m_model; }
-
- private:
-  SphericalModel m_model{};
-};
-
-class PinholeSimulatorHip : public SimulatorHip {
- public:
-  using SimulatorHip::SimulatorHip;
-  void setModel(const PinholeModel& model) {
-    m_model = model;
-    setSensor(RMH_MODEL_PINHOLE, &model);
-  }
-  void setModel(const MemoryView<PinholeModel, RAM>& model) { setModel(model[0]); }
-  const PinholeModel& model() const { return m_model; }
-
- private:
-  PinholeModel m_model{};
-};
-
-class O1DnSimulatorHip : public SimulatorHip {
- public:
-  using SimulatorHip::SimulatorHip;
-  // uploads the directions, like O1DnSimulatorOptix::setModel (O1DnSimulatorOptix.cpp:41-83)
-  void setModel(const O1DnModel_<RAM>& model) {
-    Memory<Vector, VRAM_HIP> dirs(model.dirs);
-    setDev(model, dirs);
-  }
-  void setModel(const O1DnModel_<VRAM_HIP>& model) { setDev(model, model.dirs); }
-
- private:
-  template <typename M>
-  void setDev(const M& model, const MemoryView<Vector, VRAM_HIP>& dirs) {
-    rmh_o1dn_model m{};
-    m.width = model.width;
-    m.height = model.height;
-    m.range = {model.range.min, model.range.max};
-    m.orig = {model.orig.x, model.orig.y, model.orig.z};
-    m.dirs_dev = reinterpret_cast<const rmh_vec3*>(dirs.raw());
-    setSensor(RMH_MODEL_O1DN, &m);
-  }
-};
-
-class OnDnSimulatorHip : public SimulatorHip {
- public:
-  using SimulatorHip::SimulatorHip;
-  void setModel(const OnDnModel_<RAM>& model) {
-    Memory<Vector, VRAM_HIP> dirs(model.dirs), origs(model.origs);
-    setDev(model, origs, dirs);
-  }
-  void setModel(const OnDnModel_<VRAM_HIP>& model) { setDev(model, model.origs, model.dirs); }
-
- private:
-  template <typename M>
-  void setDev(const M& model, const MemoryView<Vector, VRAM_HIP>& origs, const MemoryView<Vector, VRAM_HIP>& dirs) {
-    rmh_ondn_model m{};
-    m.width = model.width;
-    m.height = model.height;
-    m.range = {model.range.min, model.range.max};
-    m.origs_dev = reinterpret_cast<const rmh_vec3*>(origs.raw());
-    m.dirs_dev = reinterpret_cast<const rmh_vec3*>(dirs.raw());
-    setSensor(RMH_MODEL_ONDN, &m);
-  }
-};
-
 using SimulatorHipPtr = std::shared_ptr<SimulatorHip>;
-using SphereSimulatorHipPtr = std::shared_ptr<SphereSimulatorHip>;
-using PinholeSimulatorHipPtr = std::shared_ptr<PinholeSimulatorHip>;
-using O1DnSimulatorHipPtr = std::shared_ptr<O1DnSimulatorHip>;
-using OnDnSimulatorHipPtr = std::shared_ptr<OnDnSimulatorHip>;
 
 }  // namespace rmagine

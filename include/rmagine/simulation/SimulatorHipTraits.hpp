@@ -5,7 +5,10 @@
 #pragma once
 
 #include <rmagine/simulation/Simulator.hpp>
-#include <rmagine/simulation/SimulatorHip.hpp>
+#include <rmagine/simulation/O1DnSimulatorHip.hpp>
+#include <rmagine/simulation/OnDnSimulatorHip.hpp>
+#include <rmagine/simulation/PinholeSimulatorHip.hpp>
+#include <rmagine/simulation/SphereSimulatorHip.hpp>
 
 namespace rmagine {
 
