@@ -61,8 +61,11 @@ struct SimArgs {
 template <int KIND, int STACK, bool WIDE>
 __global__ __launch_bounds__(kWave, 8) void k_simulate(const SimArgs a) {
   __shared__ int2 stk[kLdsStack][kWave];
+  // block, pose and tile indices are wave-uniform: divided on the scalar unit, they stay in
+  // SGPRs through the traversal instead of taking VGPRs (the scarce registers at 8 waves
+  // per SIMD; `/` would run on the vector unit and leave its quotient there)
   const uint32_t lb = xcd_contiguous(blockIdx.x, gridDim.x);
-  const uint32_t pl = lb / a.tl.tiles;
+  const uint32_t pl = fast_div(lb, a.tl.div_tiles);
   const uint32_t tile = lb - pl * a.tl.tiles;
   uint32_t vid, hid;
   if (!tile_ray(a.tl, a.g.grid_w, a.g.grid_h, tile, threadIdx.x, vid, hid)) return;
@@ -132,7 +135,7 @@ template <int KIND, int STACK>
 __global__ __launch_bounds__(kWave, 8) void k_visit_stats(const VisitArgs a) {
   __shared__ int2 stk[kLdsStack][kWave];
   const uint32_t lb = xcd_contiguous(blockIdx.x, gridDim.x);
-  const uint32_t pl = lb / a.tl.tiles;
+  const uint32_t pl = fast_div(lb, a.tl.div_tiles);
   const uint32_t tile = lb - pl * a.tl.tiles;
   uint32_t vid, hid;
   VisitCount vc = {0u, 0u, 0u, 0u, 0u, 0u};
@@ -267,10 +270,10 @@ __global__ __launch_bounds__(BLOCK, 8) void k_batch(const BatchArgs a) {
     const uint64_t n_tiles = static_cast<uint64_t>(e.n_poses) * e.tl.tiles;
     const int kind = e.g.kind;
     for (uint32_t k = 0; k < a.chunk_rays / 64; k++) {
-      const uint64_t tg = tile0 + k;  // wave-uniform
-      if (tg >= n_tiles) break;
-      const uint32_t pose = static_cast<uint32_t>(tg / e.tl.tiles);
-      const uint32_t tile = static_cast<uint32_t>(tg - static_cast<uint64_t>(pose) * e.tl.tiles);
+      if (tile0 + k >= n_tiles) break;
+      const uint32_t tg = static_cast<uint32_t>(tile0 + k);  // wave-uniform; n_tiles < 2^32
+      const uint32_t pose = fast_div(tg, e.tl.div_tiles);
+      const uint32_t tile = tg - pose * e.tl.tiles;
       uint32_t vid, hid;
       if (!tile_ray(e.tl, e.g.grid_w, e.g.grid_h, tile, lane, vid, hid)) continue;
       batch_ray<STACK, BLOCK, WIDE, LDS, SPEC>(a, e, kind, pose, vid, hid, stk);
@@ -358,6 +361,8 @@ int batch_upload(rmh_batch* b, const std::vector<BatchEntryDev>& host) {
   std::vector<uint32_t> prefix(b->n_entries + 1, 0);
   uint64_t chunks = 0;
   for (uint32_t i = 0; i < b->n_entries; i++) {
+    // the kernel enumerates an entry's tiles with 32-bit indices
+    if (b->entry_rays[i] / 64u >= (1ull << 32)) return fail(RMH_ERR_UNSUPPORTED, "rmh_batch: an entry holds 2^32 or more ray tiles");
     prefix[i] = static_cast<uint32_t>(chunks);
     chunks += (b->entry_rays[i] + b->params.chunk_rays - 1) / b->params.chunk_rays;
     if (chunks >= (1ull << 31)) return fail(RMH_ERR_UNSUPPORTED, "rmh_batch: too many ray chunks");

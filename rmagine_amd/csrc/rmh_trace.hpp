@@ -4,6 +4,7 @@
 // epilogue.  References for each piece are given in rmh_simulate.hip.
 #pragma once
 
+#include "rmh_fastdiv.hpp"
 #include "rmh_internal.hpp"
 
 namespace rmh {
@@ -110,7 +111,11 @@ __device__ __forceinline__ double tri_test(const TriRay& r, const float4& v0, co
   const double U = Cx * By - Cy * Bx;
   const double V = Ax * Cy - Ay * Cx;
   const double W = Bx * Ay - By * Ax;
-  if ((U < 0.0 || V < 0.0 || W < 0.0) && (U > 0.0 || V > 0.0 || W > 0.0)) return -1.0;
+  // non-short-circuit form: six compares into scalar masks, where the short-circuit form
+  // became FP64 min/max chains with canonicalising moves and a branch
+  const bool neg = (U < 0.0) | (V < 0.0) | (W < 0.0);
+  const bool pos = (U > 0.0) | (V > 0.0) | (W > 0.0);
+  if (neg & pos) return -1.0;
   const double det = U + V + W;
   if (det == 0.0) return -1.0;
   const double T = U * (r.Sz * Akz) + V * (r.Sz * Bkz) + W * (r.Sz * Ckz);
@@ -198,6 +203,13 @@ struct SlabRay {
   float hx, hy, hz;      // -(o - P) / d: offsets of the hi planes
 };
 
+// max(v, 0) and min(v, hi) of slab values as one v_med3_f32 each.  fminf/fmaxf would first
+// canonicalise an operand the compiler cannot prove canonical (tbf, which lives across the
+// node loop), one more VALU instruction per node visit.  The slab values are never NaN
+// (finite planes, finite clamped reciprocals), where med3 and min/max agree.
+__device__ __forceinline__ float clamp_lo0(float v) { return __builtin_amdgcn_fmed3f(v, 0.0f, 3.40282347e38f); }
+__device__ __forceinline__ float clamp_hi(float v, float hi) { return __builtin_amdgcn_fmed3f(v, hi, -3.40282347e38f); }
+
 // Entry/exit distances of both children of one node.  OCT >= 0: the sign of every
 // direction component is known for the whole wave (bit 0/1/2 set = x/y/z negative), so
 // the near and far plane of each axis are known and min/max per axis are not needed.
@@ -213,16 +225,16 @@ __device__ __forceinline__ void slab2(const SlabRay& r, float tbf, const float4&
   const float y10 = __fmaf_rn(nb.z, r.idy, r.ly), y11 = __fmaf_rn(nb.w, r.idy, r.hy);
   const float z10 = __fmaf_rn(nc.z, r.idz, r.lz), z11 = __fmaf_rn(nc.w, r.idz, r.hz);
   if constexpr (OCT < 0) {
-    tmin0 = fmaxf(fmaxf(fminf(x00, x01), fminf(y00, y01)), fmaxf(fminf(z00, z01), 0.0f));
-    tmax0 = fminf(fminf(fmaxf(x00, x01), fmaxf(y00, y01)), fminf(fmaxf(z00, z01), tbf));
-    tmin1 = fmaxf(fmaxf(fminf(x10, x11), fminf(y10, y11)), fmaxf(fminf(z10, z11), 0.0f));
-    tmax1 = fminf(fminf(fmaxf(x10, x11), fmaxf(y10, y11)), fminf(fmaxf(z10, z11), tbf));
+    tmin0 = fmaxf(fmaxf(fminf(x00, x01), fminf(y00, y01)), clamp_lo0(fminf(z00, z01)));
+    tmax0 = fminf(fminf(fmaxf(x00, x01), fmaxf(y00, y01)), clamp_hi(fmaxf(z00, z01), tbf));
+    tmin1 = fmaxf(fmaxf(fminf(x10, x11), fminf(y10, y11)), clamp_lo0(fminf(z10, z11)));
+    tmax1 = fminf(fminf(fmaxf(x10, x11), fmaxf(y10, y11)), clamp_hi(fmaxf(z10, z11), tbf));
   } else {
     constexpr bool nx = OCT & 1, ny = OCT & 2, nz = OCT & 4;
-    tmin0 = fmaxf(fmaxf(nx ? x01 : x00, ny ? y01 : y00), fmaxf(nz ? z01 : z00, 0.0f));
-    tmax0 = fminf(fminf(nx ? x00 : x01, ny ? y00 : y01), fminf(nz ? z00 : z01, tbf));
-    tmin1 = fmaxf(fmaxf(nx ? x11 : x10, ny ? y11 : y10), fmaxf(nz ? z11 : z10, 0.0f));
-    tmax1 = fminf(fminf(nx ? x10 : x11, ny ? y10 : y11), fminf(nz ? z10 : z11, tbf));
+    tmin0 = fmaxf(fmaxf(nx ? x01 : x00, ny ? y01 : y00), clamp_lo0(nz ? z01 : z00));
+    tmax0 = fminf(fminf(nx ? x00 : x01, ny ? y00 : y01), clamp_hi(nz ? z00 : z01, tbf));
+    tmin1 = fmaxf(fmaxf(nx ? x11 : x10, ny ? y11 : y10), clamp_lo0(nz ? z11 : z10));
+    tmax1 = fminf(fminf(nx ? x10 : x11, ny ? y10 : y11), clamp_hi(nz ? z10 : z11, tbf));
   }
 }
 
@@ -314,11 +326,12 @@ __device__ __forceinline__ void walk(const MapData& m, const SlabRay& r, float t
       const float4 nb = make_float4(gf[4], gf[5], gf[6], gf[7]);
       const float4 nc = make_float4(gf[8], gf[9], gf[10], gf[11]);
       // only the two child references of the last 16 bytes are used; hiding the
-      // offset's alignment keeps the compiler from widening this read to 16 bytes
-      uint32_t roff = off + 48u;
+      // offset's alignment keeps the compiler from widening this read to 16 bytes, and
+      // adding the 48 after the barrier lets it go into the load's immediate offset
+      uint32_t roff = off;
       asm volatile("" : "+v"(roff));
       __builtin_assume((roff & 7u) == 0u);
-      const GInt* gr = (const GInt*)((const GChar*)(m.nodes) + roff);
+      const GInt* gr = (const GInt*)((const GChar*)(m.nodes) + roff + 48u);
       nr = make_int2(gr[0], gr[1]);
       slab2<OCT>(r, tbf, na, nb, nc, tmin0, tmax0, tmin1, tmax1);
     }
@@ -373,12 +386,12 @@ __device__ __forceinline__ void slab4(const SlabRay& r, float tbf, uint32_t w0, 
     const float z0 = __fmaf_rn(lz, r.idz, r.lz), z1 = __fmaf_rn(hz, r.idz, r.hz);
     float t0, t1;
     if constexpr (OCT < 0) {
-      t0 = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), 0.0f));
-      t1 = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), tbf));
+      t0 = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), clamp_lo0(fminf(z0, z1)));
+      t1 = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), clamp_hi(fmaxf(z0, z1), tbf));
     } else {
       constexpr bool nx = OCT & 1, ny = OCT & 2, nz = OCT & 4;
-      t0 = fmaxf(fmaxf(nx ? x1 : x0, ny ? y1 : y0), fmaxf(nz ? z1 : z0, 0.0f));
-      t1 = fminf(fminf(nx ? x0 : x1, ny ? y0 : y1), fminf(nz ? z0 : z1, tbf));
+      t0 = fmaxf(fmaxf(nx ? x1 : x0, ny ? y1 : y0), clamp_lo0(nz ? z1 : z0));
+      t1 = fminf(fminf(nx ? x0 : x1, ny ? y0 : y1), clamp_hi(nz ? z0 : z1, tbf));
     }
     tmin[k] = t0;
     hit[k] = t0 <= t1 * 1.0000005f && (k < 2 || static_cast<uint32_t>(k) < nch);
@@ -567,6 +580,8 @@ constexpr int spec_mode() { return speculative_walk(KIND) ? kSpecOn : kSpecOff; 
 // O1Dn/OnDn images of height 1 use 1 x 64.
 struct Tiling {
   uint32_t tw, th, tiles_w, tiles;  // tiles per pose image
+  uint32_t tw_log2;                 // tw is a power of two
+  FastDiv div_tiles, div_tiles_w;   // division by tiles and by tiles_w
 };
 
 constexpr uint32_t kDefaultTileRows = 4;
@@ -579,6 +594,10 @@ inline Tiling make_tiling(uint32_t W, uint32_t H, uint32_t rows = kDefaultTileRo
   t.tw = 64u / t.th;
   t.tiles_w = (W + t.tw - 1) / t.tw;
   t.tiles = t.tiles_w * ((H + t.th - 1) / t.th);
+  t.tw_log2 = 0;
+  while ((1u << t.tw_log2) < t.tw) t.tw_log2++;
+  t.div_tiles = make_fastdiv(t.tiles > 0 ? t.tiles : 1u);
+  t.div_tiles_w = make_fastdiv(t.tiles_w > 0 ? t.tiles_w : 1u);
   return t;
 }
 
@@ -601,11 +620,12 @@ __device__ __forceinline__ uint32_t xcd_contiguous(uint32_t b, uint32_t n) {
 // size in work-items as a 32-bit count, so blocks x 64 stays below 2^31.
 constexpr uint64_t kMaxTileBlocks = 1ull << 25;
 
+// `tile` is wave-uniform in every caller, so its division stays on the scalar unit.
 __device__ __forceinline__ bool tile_ray(const Tiling& t, uint32_t W, uint32_t H, uint32_t tile, uint32_t lane,
                                          uint32_t& vid, uint32_t& hid) {
-  const uint32_t ty = tile / t.tiles_w, tx = tile - ty * t.tiles_w;
-  vid = ty * t.th + lane / t.tw;
-  hid = tx * t.tw + lane % t.tw;
+  const uint32_t ty = fast_div(tile, t.div_tiles_w), tx = tile - ty * t.tiles_w;
+  vid = ty * t.th + (lane >> t.tw_log2);
+  hid = tx * t.tw + (lane & (t.tw - 1u));
   return vid < H && hid < W;
 }
 
