@@ -189,14 +189,20 @@ typedef __attribute__((address_space(1))) const float GFloat;
 typedef __attribute__((address_space(1))) const int GInt;
 
 // Slab test.  A slab value is fma(plane, 1/d, -(o +- P) / d) with a per-ray margin
-//   P = 2^-21 max|o| + 2^-62 range_max + 1e-30:
-// the rounding of (o + P)/d stays below P, so each value is, up to the one rounding of
-// the fma, the exact entry/exit distance of a plane moved outward by at most ~2P (the lo
-// plane with o + P, the hi plane with o - P).  P also covers replacing a direction
-// component below 2^-64 by +-2^-64 (position error <= range_max * 2^-64).  The remaining
-// relative errors (fma, reciprocal: 2 roundings) are absorbed by tmin <= tmax (1 + 5e-7),
-// so a box that the exact ray meets within [0, tbf] is never culled.  SlabRay holds the
-// per-ray constants of that test.
+//   P = 2^-21 (max|o| + range_max) + 1e-30.
+// Written out, a computed value is t' = (plane - o -+ P'') / d * (1 + r), where
+//   * P'' lies in [0.74 P, 1.26 P]: o +- P and its product with 1/d are rounded, at most
+//     2^-23 (|o| + P) <= P / 4 in position (the lo plane takes o + P, the hi plane o - P,
+//     so both planes move outward);
+//   * |r| <= 1.2e-7: the roundings of the reciprocal and of the fma.
+// Replacing a direction component below 2^-64 by +-2^-64 moves the ray by at most
+// range_max 2^-64, far below P.  Let the exact ray meet the box at some t in [0, tbf]
+// (tbf <= range_max).  On each axis the near plane's exact crossing is <= t, so its moved
+// plane crosses at <= t - P''/|d| and the computed near value is <= t - P''/|d| + 1.2e-7 t
+// <= t, because P''/|d| >= 0.74 P >= 3.5e-7 range_max (|d| <= 1, t <= range_max); in the
+// same way every computed far value is >= t.  So tmin <= tmax holds as computed, with no
+// relative slack, and the box is never culled; boxes grow by about 2^-21 range_max
+// (0.05 mm at 100 m).  SlabRay holds the per-ray constants of that test.
 struct SlabRay {
   float idx, idy, idz;   // reciprocal direction (components clamped away from zero)
   float lx, ly, lz;      // -(o + P) / d: offsets of the lo planes
@@ -268,8 +274,9 @@ __device__ __forceinline__ void count_visit(VisitCount* vc, bool uniform) {
 
 // Pops the next subtree of a lane, kDone when its stack runs empty.  When culling, entries
 // whose box the slab test would no longer accept against tbf (the current closest hit,
-// rounded up: entry <= tbf with the test's relative slack) are skipped: skipping is the
-// slab test of the entry's parent repeated with the shorter interval.
+// rounded up: entry <= tbf) are skipped: skipping is the slab test of the entry's parent
+// repeated with the shorter interval (the stored entry is the computed near value, at
+// most the distance of any point of the box the ray meets within the old interval).
 template <int STACK, int BLOCK, int LDS, bool CULL>
 __device__ __forceinline__ int pop_ref(int2 (*stk)[BLOCK], volatile int* spill, int& sp, float tbf) {
   if constexpr (!CULL) {
@@ -287,7 +294,7 @@ __device__ __forceinline__ int pop_ref(int2 (*stk)[BLOCK], volatile int* spill, 
       e.x = spill[2 * (sp - LDS)];
       e.y = spill[2 * (sp - LDS) + 1];
     }
-    if (__int_as_float(e.y) <= tbf * 1.0000005f) return e.x;
+    if (__int_as_float(e.y) <= tbf) return e.x;
   }
   return kDone;
 }
@@ -335,8 +342,8 @@ __device__ __forceinline__ void walk(const MapData& m, const SlabRay& r, float t
       nr = make_int2(gr[0], gr[1]);
       slab2<OCT>(r, tbf, na, nb, nc, tmin0, tmax0, tmin1, tmax1);
     }
-    const bool h0 = tmin0 <= tmax0 * 1.0000005f;
-    const bool h1 = tmin1 <= tmax1 * 1.0000005f;
+    const bool h0 = tmin0 <= tmax0;
+    const bool h1 = tmin1 <= tmax1;
     if (h0 && h1) {
       int near = nr.x, far = nr.y;
       float tfar = tmin1;
@@ -394,7 +401,7 @@ __device__ __forceinline__ void slab4(const SlabRay& r, float tbf, uint32_t w0, 
       t1 = fminf(fminf(nx ? x0 : x1, ny ? y0 : y1), clamp_hi(nz ? z0 : z1, tbf));
     }
     tmin[k] = t0;
-    hit[k] = t0 <= t1 * 1.0000005f && (k < 2 || static_cast<uint32_t>(k) < nch);
+    hit[k] = t0 <= t1 && (k < 2 || static_cast<uint32_t>(k) < nch);
   }
 }
 
@@ -492,7 +499,7 @@ __device__ __forceinline__ void trace(const MapData& m, const V3& o, const V3& d
   r.idy = 1.0f / (fabsf(d.y) > kMinDir ? d.y : copysignf(kMinDir, d.y));
   r.idz = 1.0f / (fabsf(d.z) > kMinDir ? d.z : copysignf(kMinDir, d.z));
   const float omax = fmaxf(fmaxf(fabsf(o.x), fabsf(o.y)), fabsf(o.z));
-  const float P = omax * 0x1p-21f + range_max * 0x1p-62f + 1e-30f;
+  const float P = (omax + range_max) * 0x1p-21f + 1e-30f;
   r.lx = -((o.x + P) * r.idx); r.hx = -((o.x - P) * r.idx);
   r.ly = -((o.y + P) * r.idy); r.hy = -((o.y - P) * r.idy);
   r.lz = -((o.z + P) * r.idz); r.hz = -((o.z - P) * r.idz);
