@@ -74,12 +74,10 @@ inline TravSel trav_select(const rmh_map* m) {
   return t;
 }
 
-__device__ __forceinline__ float sel3(const float4& v, int k) { return k == 0 ? v.x : (k == 1 ? v.y : v.z); }
-
 struct TriRay {
   double okx, oky, okz;
   double Sx, Sy, Sz;
-  int kx, ky, kz;
+  int perm;  // 2 kz + swap (one register; the axes are derived at the leaves)
 };
 
 __device__ __forceinline__ TriRay make_tri_ray(const V3& o, const V3& d) {
@@ -92,7 +90,7 @@ __device__ __forceinline__ TriRay make_tri_ray(const V3& o, const V3& d) {
   if (dkz < 0.0f) { const int t = kx; kx = ky; ky = t; }
   const double dd[3] = {(double)d.x, (double)d.y, (double)d.z};
   const double oo[3] = {(double)o.x, (double)o.y, (double)o.z};
-  r.kx = kx; r.ky = ky; r.kz = kz;
+  r.perm = 2 * kz + (dkz < 0.0f ? 1 : 0);
   r.Sx = dd[kx] / dd[kz];
   r.Sy = dd[ky] / dd[kz];
   r.Sz = 1.0 / dd[kz];
@@ -100,11 +98,33 @@ __device__ __forceinline__ TriRay make_tri_ray(const V3& o, const V3& d) {
   return r;
 }
 
+// Axis permutation of the watertight test as lane masks: kz is the dominant axis of the
+// direction, kx and ky follow it cyclically and are swapped when d[kz] < 0.  Choosing a
+// component by an index (v[k]) compiles to a loop over the distinct indices of the wave
+// with branches over the execution mask; two selects on masks computed once per leaf are
+// two v_cndmask instructions per component.
+struct TriAxes {
+  bool x0, x1, y0, y1, z0, z1;  // kx == 0, kx == 1, ...
+};
+
+__device__ __forceinline__ TriAxes tri_axes(int perm) {
+  const int kz = perm >> 1;
+  const int kx0 = kz == 2 ? 0 : kz + 1, ky0 = kx0 == 2 ? 0 : kx0 + 1;
+  const int kx = (perm & 1) ? ky0 : kx0, ky = (perm & 1) ? kx0 : ky0;
+  return TriAxes{kx == 0, kx == 1, ky == 0, ky == 1, kz == 0, kz == 1};
+}
+
+__device__ __forceinline__ float pick(const float4& v, bool c0, bool c1) { return c0 ? v.x : (c1 ? v.y : v.z); }
+
 // Watertight test in FP64.  Returns t >= 0 of the hit or a negative value for no hit.
-__device__ __forceinline__ double tri_test(const TriRay& r, const float4& v0, const float4& v1, const float4& v2) {
-  const double Akx = (double)sel3(v0, r.kx) - r.okx, Aky = (double)sel3(v0, r.ky) - r.oky, Akz = (double)sel3(v0, r.kz) - r.okz;
-  const double Bkx = (double)sel3(v1, r.kx) - r.okx, Bky = (double)sel3(v1, r.ky) - r.oky, Bkz = (double)sel3(v1, r.kz) - r.okz;
-  const double Ckx = (double)sel3(v2, r.kx) - r.okx, Cky = (double)sel3(v2, r.ky) - r.oky, Ckz = (double)sel3(v2, r.kz) - r.okz;
+__device__ __forceinline__ double tri_test(const TriRay& r, const TriAxes& k, const float4& v0, const float4& v1,
+                                           const float4& v2) {
+  const double Akx = (double)pick(v0, k.x0, k.x1) - r.okx, Aky = (double)pick(v0, k.y0, k.y1) - r.oky;
+  const double Akz = (double)pick(v0, k.z0, k.z1) - r.okz;
+  const double Bkx = (double)pick(v1, k.x0, k.x1) - r.okx, Bky = (double)pick(v1, k.y0, k.y1) - r.oky;
+  const double Bkz = (double)pick(v1, k.z0, k.z1) - r.okz;
+  const double Ckx = (double)pick(v2, k.x0, k.x1) - r.okx, Cky = (double)pick(v2, k.y0, k.y1) - r.oky;
+  const double Ckz = (double)pick(v2, k.z0, k.z1) - r.okz;
   const double Ax = Akx - r.Sx * Akz, Ay = Aky - r.Sy * Akz;
   const double Bx = Bkx - r.Sx * Bkz, By = Bky - r.Sy * Bkz;
   const double Cx = Ckx - r.Sx * Ckz, Cy = Cky - r.Sy * Ckz;
@@ -131,12 +151,13 @@ struct Hit {
 // The triangles of leaf reference `leaf` against the ray; keeps the nearest hit, equal
 // distances going to the lower global face index.
 __device__ __forceinline__ void test_leaf(const MapData& m, const TriRay& tr, int leaf, Hit& h) {
+  const TriAxes ax = tri_axes(tr.perm);
   const uint32_t e = ~static_cast<uint32_t>(leaf);
   const uint32_t first = e >> 3, cnt = (e & 7u) + 1u;
   for (uint32_t k = 0; k < cnt; k++) {
     const size_t s = static_cast<size_t>(first + k) * 3;
     const float4 v0 = m.tris[s], v1 = m.tris[s + 1], v2 = m.tris[s + 2];
-    const double t = tri_test(tr, v0, v1, v2);
+    const double t = tri_test(tr, ax, v0, v1, v2);
     if (t >= 0.0 && t <= h.t) {
       const uint32_t face = __float_as_uint(v2.w);
       if (t < h.t || face < h.face) {
