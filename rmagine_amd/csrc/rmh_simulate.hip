@@ -223,17 +223,25 @@ __device__ __forceinline__ void with_kind(int kind, Fn&& fn) {
 // k_simulate, the ray set-up is recomputed for the writers rather than held (spilled)
 // through the traversal.
 template <int STACK, int BLOCK, bool WIDE, int LDS, int SPEC>
-__device__ __forceinline__ void batch_ray(const BatchArgs& a, const BatchEntryDev& e, int kind, uint32_t pose,
-                                          uint32_t vid, uint32_t hid, int2 (*stk)[BLOCK]) {
+__device__ __forceinline__ void batch_ray(const BatchArgs& a, const BatchEntryDev* ep, int kind, uint32_t pose,
+                                          uint32_t tile, uint32_t vid, uint32_t hid, int2 (*stk)[BLOCK]) {
   Hit h;
   {
+    const BatchEntryDev& e = *ep;
     const Tf Tsm = tmul(load_tf(e.Tbm[pose]), e.Tsb);
     uint32_t lid;
     V3 ds, os, dm, om;
     with_kind<SPEC>(kind, [&](auto k) { batch_gen<decltype(k)::value>(e, vid, hid, lid, Tsm, ds, os, dm, om); });
     trace<STACK, BLOCK, WIDE, SPEC, false, LDS>(a.map, om, dm, e.g.range_max, stk, h, speculative_walk(kind));
   }
-  asm volatile("" : "+v"(vid), "+v"(hid));
+  // the entry's fields are read again after the traversal (the barrier hides that they are
+  // the same), and the ray's tile coordinates recomputed from the lane: holding either
+  // through the traversal spilled them to scratch once per tile
+  asm volatile("" : "+s"(ep));
+  uint32_t lane = threadIdx.x & 63u;
+  asm volatile("" : "+v"(lane));
+  const BatchEntryDev& e = *ep;
+  tile_ray(e.tl, e.g.grid_w, e.g.grid_h, tile, lane, vid, hid);
   const Tf Tsm = tmul(load_tf(e.Tbm[pose]), e.Tsb);
   uint32_t lid;
   V3 ds, os;
@@ -276,7 +284,7 @@ __global__ __launch_bounds__(BLOCK, 8) void k_batch(const BatchArgs a) {
       const uint32_t tile = tg - pose * e.tl.tiles;
       uint32_t vid, hid;
       if (!tile_ray(e.tl, e.g.grid_w, e.g.grid_h, tile, lane, vid, hid)) continue;
-      batch_ray<STACK, BLOCK, WIDE, LDS, SPEC>(a, e, kind, pose, vid, hid, stk);
+      batch_ray<STACK, BLOCK, WIDE, LDS, SPEC>(a, &e, kind, pose, tile, vid, hid, stk);
     }
   }
 }
