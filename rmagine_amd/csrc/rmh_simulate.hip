@@ -45,6 +45,23 @@ using namespace rmh;
 // occupancy follows the live waves instead of the slowest wave of a larger workgroup.
 constexpr int kWave = 64;
 
+// The launch arguments of a kernel whose only argument is a T, read through the kernarg
+// segment pointer (the argument starts the segment) hidden behind a barrier.  Reads made
+// through it after a traversal are scalar loads again (served by the scalar cache), where
+// the compiler would otherwise load the fields at the kernel's start and hold them in SGPRs
+// throughout: k_simulate's arguments outgrow the SGPRs, and the excess was spilled into VGPR
+// lanes, taking VGPRs from the traversal (terrain 8.80 -> 8.67 ms without them; the batched
+// kernel, whose entries live in device memory, measured slower with it and does not use it).
+// (The pointer is returned as a generic one; the cast from the constant address space stays
+// visible to the compiler, which reads through it with scalar loads.)
+template <class T>
+__device__ __forceinline__ const T* kernel_args() {
+  typedef const __attribute__((address_space(4))) T KArg;
+  KArg* p = (KArg*)__builtin_amdgcn_kernarg_segment_ptr();
+  asm volatile("" : "+s"(p));
+  return (const T*)p;
+}
+
 struct SimArgs {
   MapData map;
   RayGen g;
@@ -56,8 +73,8 @@ struct SimArgs {
 };
 
 
-// Launch bounds ask for 8 waves per SIMD (64 VGPRs); the few values that do not fit are
-// set-up constants spilled outside the node loop.
+// Launch bounds ask for 8 waves per SIMD (64 VGPRs); with the writers' values recomputed or
+// re-read after the traversal, nothing spills (62-64 VGPRs, 62 SGPRs).
 template <int KIND, int STACK, bool WIDE>
 __global__ __launch_bounds__(kWave, 8) void k_simulate(const SimArgs a) {
   __shared__ int2 stk[kLdsStack][kWave];
@@ -81,14 +98,16 @@ __global__ __launch_bounds__(kWave, 8) void k_simulate(const SimArgs a) {
     trace<STACK, kWave, WIDE, spec_mode<KIND>()>(a.map, om, dm, a.g.range_max, stk, h);
   }
   // The ray set-up is recomputed for the attribute writers instead of being held through
-  // the traversal: at 64 VGPRs those values would otherwise be spilled to scratch.
+  // the traversal: at 64 VGPRs those values would otherwise be spilled to scratch.  The
+  // launch arguments the writers use are read again (kernel_args).
   asm volatile("" : "+v"(vid), "+v"(hid));
+  const SimArgs* ap = kernel_args<SimArgs>();
   uint32_t lid;
   V3 ds, os;
-  gen_ray<KIND>(a.g, vid, hid, lid, ds, os);
-  const Tf Tsm = tmul(load_tf(a.Tbm[pose]), a.Tsb);
-  const uint64_t glob = static_cast<uint64_t>(pose) * a.g.ray_count + lid;
-  write_outputs<KIND>(a.out, glob, a.g, a.map, Tsm, ds, os, h);
+  gen_ray<KIND>(ap->g, vid, hid, lid, ds, os);
+  const Tf Tsm = tmul(load_tf(ap->Tbm[pose]), ap->Tsb);
+  const uint64_t glob = static_cast<uint64_t>(pose) * ap->g.ray_count + lid;
+  write_outputs<KIND>(ap->out, glob, ap->g, ap->map, Tsm, ds, os, h);
 }
 
 template <int KIND, int STACK, bool WIDE>

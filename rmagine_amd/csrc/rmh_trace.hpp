@@ -331,7 +331,6 @@ __device__ __forceinline__ int pop_ref(int2 (*stk)[BLOCK], volatile int* spill, 
 template <int OCT, int STACK, int BLOCK, int SPEC, bool COUNT, int LDS>
 __device__ __forceinline__ void walk(const MapData& m, const SlabRay& r, float tbf, int2 (*stk)[BLOCK],
                                      volatile int* spill, int& ref, int& sp, int& leaf, bool spec, VisitCount* vc) {
-  const int tid = threadIdx.x;
   while (ref >= 0 && ref != kDone) {
     float tmin0, tmax0, tmin1, tmax1;
     int2 nr;
@@ -444,7 +443,6 @@ __device__ __forceinline__ int pick_ref(uint32_t key, int r0, int r1, int r2, in
 template <int OCT, int STACK, int BLOCK, int LDS, bool CULL>
 __device__ __forceinline__ void walk4(const MapData& m, const SlabRay& r, float tbf, int2 (*stk)[BLOCK],
                                       volatile int* spill, int& ref, int& sp) {
-  const int tid = threadIdx.x;
   while (ref >= 0) {
     float tmin[4];
     bool hit[4];
@@ -513,7 +511,6 @@ __device__ __forceinline__ void trace(const MapData& m, const V3& o, const V3& d
   h.slot = 0xFFFFFFFFu;
   h.face = 0xFFFFFFFFu;
   if (m.n_faces == 0) return;
-  const int tid = threadIdx.x;
   constexpr float kMinDir = 0x1p-64f;
   SlabRay r;
   r.idx = 1.0f / (fabsf(d.x) > kMinDir ? d.x : copysignf(kMinDir, d.x));
