@@ -217,8 +217,8 @@ typedef __attribute__((address_space(1))) const int GInt;
 //     so both planes move outward);
 //   * |r| <= 1.2e-7: the roundings of the reciprocal and of the fma.
 // Replacing a direction component below 2^-64 by +-2^-64 moves the ray by at most
-// range_max 2^-64, far below P.  Let the exact ray meet the box at some t in [0, tbf]
-// (tbf <= range_max).  On each axis the near plane's exact crossing is <= t, so its moved
+// range_max 2^-64, far below P.  Let the exact ray meet the box at some t <= range_max in
+// [0, tbf] (tbf: an FP32 bound at least the closest hit found so far).  On each axis the near plane's exact crossing is <= t, so its moved
 // plane crosses at <= t - P''/|d| and the computed near value is <= t - P''/|d| + 1.2e-7 t
 // <= t, because P''/|d| >= 0.74 P >= 3.5e-7 range_max (|d| <= 1, t <= range_max); in the
 // same way every computed far value is >= t.  So tmin <= tmax holds as computed, with no
@@ -265,6 +265,12 @@ __device__ __forceinline__ void slab2(const SlabRay& r, float tbf, const float4&
   }
 }
 
+// An FP32 bound at least as large as a non-negative FP64 distance, for the culling tests:
+// the nearest float plus one ulp (two instructions, where the correctly rounded-up
+// conversion takes about fourteen).  It exceeds the rounded-up value by at most one ulp,
+// so the tests keep at most the boxes that start within that ulp beyond the hit.
+__device__ __forceinline__ float f32_above(double t) { return __int_as_float(__float_as_int(static_cast<float>(t)) + 1); }
+
 constexpr int kDone = 0x7fffffff;
 
 // Walks interior nodes from `ref` until a leaf is reached (ref < 0) or the stack runs
@@ -275,6 +281,15 @@ constexpr int kDone = 0x7fffffff;
 // well costs registers and made a batched kernel 50 % slower).  Stack entries carry entry
 // distances (pop skipping) in every mode but kSpecOn.
 constexpr int kSpecOff = 0, kSpecOn = 1, kSpecRuntime = 2;
+
+// The culling bound after a leaf test.  The run-time mode (one batched kernel for all sensor
+// kinds) keeps the rounded-up conversion: with the cheaper bound its register allocation
+// came out slower (batched workload 19.26 -> 19.39 ms), while the terrain ran 0.7 % faster.
+template <int SPEC>
+__device__ __forceinline__ float leaf_bound(double t) {
+  if constexpr (SPEC == kSpecRuntime) return __double2float_ru(t);
+  else return f32_above(t);
+}
 
 // Per-lane traversal counts of the diagnostic build of trace() (COUNT = true): node
 // visits through the vector and the scalar path, leaves, triangle tests, and (on the
@@ -294,8 +309,8 @@ __device__ __forceinline__ void count_visit(VisitCount* vc, bool uniform) {
 }
 
 // Pops the next subtree of a lane, kDone when its stack runs empty.  When culling, entries
-// whose box the slab test would no longer accept against tbf (the current closest hit,
-// rounded up: entry <= tbf) are skipped: skipping is the slab test of the entry's parent
+// whose box the slab test would no longer accept against tbf (an FP32 bound at least the
+// current closest hit: entry <= tbf) are skipped: skipping is the slab test of the entry's parent
 // repeated with the shorter interval (the stored entry is the computed near value, at
 // most the distance of any point of the box the ray meets within the old interval).
 template <int STACK, int BLOCK, int LDS, bool CULL>
@@ -566,7 +581,7 @@ __device__ __forceinline__ void trace(const MapData& m, const V3& o, const V3& d
       if (ref == kDone) break;
       if constexpr (COUNT) { vc->leaves++; vc->tris += (~static_cast<uint32_t>(ref) & 7u) + 1u; }
       test_leaf(m, tr, ref, h);
-      tbf = __double2float_ru(h.t);
+      tbf = leaf_bound<SPEC>(h.t);
       ref = pop_ref<STACK, BLOCK, LDS, SPEC != kSpecOn>(stk, spill, sp, tbf);
       if (ref == kDone) break;
       continue;
@@ -580,7 +595,7 @@ __device__ __forceinline__ void trace(const MapData& m, const V3& o, const V3& d
     while (leaf < 0) {
       if constexpr (COUNT) { vc->leaves++; vc->tris += (~static_cast<uint32_t>(leaf) & 7u) + 1u; }
       test_leaf(m, tr, leaf, h);
-      tbf = __double2float_ru(h.t);
+      tbf = leaf_bound<SPEC>(h.t);
       if (ref < 0) {
         leaf = ref;
         ref = pop_ref<STACK, BLOCK, LDS, SPEC != kSpecOn>(stk, spill, sp, tbf);

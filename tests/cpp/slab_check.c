@@ -12,7 +12,8 @@
  * (degenerate boxes, rays that touch a face, an edge or a corner).  Origins reach 1e4,
  * directions include tiny, zero and axis-aligned components, range_max reaches 1e4, and tbf
  * is either range_max or t* rounded up to FP32 (the bound after a hit at t*: a box holding
- * an equal-distance triangle must survive).
+ * an equal-distance triangle must survive; the kernels' bound, the nearest float plus one
+ * ulp, is at least that, so it culls no more).
  *
  * usage: slab_check [n]  -- prints the numbers of checked and culled boxes; exit status 0
  * when none is culled.  Built with the C compiler without -ffast-math: fmaf is the
