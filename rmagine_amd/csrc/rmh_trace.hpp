@@ -356,6 +356,7 @@ __device__ __forceinline__ void walk(const MapData& m, const SlabRay& r, float t
       // through the scalar cache, which keeps the vector memory return path free; the
       // slab code sits in both branches so that it reads the node words from SGPRs
       const CFloat* cf = (const CFloat*)(m.nodes) + static_cast<size_t>(r0) * 16;
+      __builtin_amdgcn_s_setprio(0);
       slab2<OCT>(r, tbf, make_float4(cf[0], cf[1], cf[2], cf[3]), make_float4(cf[4], cf[5], cf[6], cf[7]),
                  make_float4(cf[8], cf[9], cf[10], cf[11]), tmin0, tmax0, tmin1, tmax1);
       nr = make_int2(__float_as_int(cf[12]), __float_as_int(cf[13]));
@@ -375,8 +376,15 @@ __device__ __forceinline__ void walk(const MapData& m, const SlabRay& r, float t
       __builtin_assume((roff & 7u) == 0u);
       const GInt* gr = (const GInt*)((const GChar*)(m.nodes) + roff + 48u);
       nr = make_int2(gr[0], gr[1]);
+      __builtin_amdgcn_s_setprio(0);
       slab2<OCT>(r, tbf, na, nb, nc, tmin0, tmax0, tmin1, tmax1);
     }
+    // Wave priority: the slab tests run at priority 0, everything else (the choice of the
+    // next node, stack work, the loads that follow, and the leaf tests) at 1, so that a wave
+    // that has its box results issues its next dependent loads ahead of waves still testing
+    // boxes (terrain 8.58 -> 8.53 ms, room 0.124 -> 0.119 ms; the leaf tests at priority 0 or
+    // 2, or the whole walk at 0, were 1-5 % slower).
+    __builtin_amdgcn_s_setprio(1);
     const bool h0 = tmin0 <= tmax0;
     const bool h1 = tmin1 <= tmax1;
     if (h0 && h1) {
