@@ -131,8 +131,8 @@ __device__ __forceinline__ double tri_test(const TriRay& r, const TriAxes& k, co
   const double U = Cx * By - Cy * Bx;
   const double V = Ax * Cy - Ay * Cx;
   const double W = Bx * Ay - By * Ax;
-  // non-short-circuit form: six compares into scalar masks, where the short-circuit form
-  // became FP64 min/max chains with canonicalising moves and a branch
+  // non-short-circuit form: each triple of compares becomes one FP64 min or max and one
+  // compare; the short-circuit form also took canonicalising moves and a branch
   const bool neg = (U < 0.0) | (V < 0.0) | (W < 0.0);
   const bool pos = (U > 0.0) | (V > 0.0) | (W > 0.0);
   if (neg & pos) return -1.0;
