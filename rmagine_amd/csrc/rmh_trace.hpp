@@ -337,9 +337,11 @@ __device__ __forceinline__ int pop_ref(int2 (*stk)[BLOCK], volatile int* spill, 
 
 // SPEC == kSpecOn (or kSpecRuntime with spec): speculative walk (Aila and Laine, "Understanding the Efficiency of
 // Ray Traversal on GPUs", HPG 2009): a lane that reaches its first leaf holds it in `leaf`
-// and keeps walking; the wave leaves the loop once a quarter of its walking lanes hold a
-// leaf (on 2,000 x 32k random directions: 25.9 ms when waiting for all lanes, 24.1 for
-// half, 22.2 for a quarter, 22.5 for an eighth; 34.5 ms without speculation).  This fills
+// and keeps walking; the wave leaves the loop once an eighth of its walking lanes hold a
+// leaf (on 2,000 x 32k random directions with the current kernel: 20.7 ms for an eighth,
+// 20.8 for a sixth, 21.0 for a twelfth, 21.3 for a quarter, 21.4 for a sixteenth, 22.0 for
+// a third; on earlier kernels 25.9 ms when waiting for all lanes, 24.1 for half, and
+// 34.5 ms without speculation).  This fills
 // the node iterations of incoherent waves; coherent image tiles reach their leaves
 // together and lose by it (5 %: a held leaf shrinks the ray's interval later), so they
 // walk without it.
@@ -405,7 +407,12 @@ __device__ __forceinline__ void walk(const MapData& m, const SlabRay& r, float t
         leaf = ref;
         ref = pop_ref<STACK, BLOCK, LDS, SPEC != kSpecOn>(stk, spill, sp, tbf);
       }
-      if (__popcll(__ballot(leaf != 0)) * 4 >= __popcll(__ballot(1))) break;
+      // an eighth for the ray-list kernels (the ray-list launch of a batch split in two
+      // included: 19.10 -> 19.27 ms there, against ICP 21.38 -> 20.74 ms); the batched
+      // run-time mode, whose chunks mix sensor kinds, keeps a quarter (with an eighth: the
+      // batched workload's one launch 19.27 -> 19.48 ms)
+      constexpr int kHoldShare = SPEC == kSpecOn ? 8 : 4;
+      if (__popcll(__ballot(leaf != 0)) * kHoldShare >= __popcll(__ballot(1))) break;
     }
   }
 }
