@@ -119,3 +119,40 @@ def test_batch_one_or_two_launches():
     with pytest.raises(Exception):
         batch.set_params(launches=3)
     assert batch.params()["launches"] == 2
+
+
+def test_batch_refuses_entries_beyond_32_bit_tile_indices():
+    """The batched kernel enumerates an entry's 64-ray tiles with 32-bit indices: an entry of
+    2^32 tiles or more is refused with RMH_ERR_UNSUPPORTED when the batch is built, before any
+    launch; one tile pose fewer is accepted.  Only the device table is built (no outputs are
+    allocated and nothing is launched), so the pose pointer is never read."""
+    import ctypes
+
+    from rmagine_amd import _lib
+    from rmagine_amd.simulation import _SensorHandle
+
+    V, F, _ = scenes.room_workload(n_sensors=1)
+    hm = HipMap(V, F)
+    sh = _SensorHandle(scenes.ouster_workload_model(), torch.device("cuda"))
+    Tbm = torch.zeros(1, 8, dtype=torch.float32, device="cuda")
+    tiles = (128 // 4) * (1024 // 16)  # tile_rows 4: 4 x 16 rays per tile
+    prm = T.rmh_batch_params(0, 0, 4, 0, 0)
+
+    def create(n_poses):
+        table = (T.rmh_batch_entry * 1)()
+        table[0] = T.rmh_batch_entry(sh.handle.value, T.transform_to_c(T.identity_transforms(1)[0]),
+                                     int(Tbm.data_ptr()), n_poses, 0, T.rmh_outputs())
+        h = ctypes.c_void_p()
+        rc = _lib.lib().rmh_batch_create(hm.handle, table, 1, ctypes.byref(prm), ctypes.byref(h))
+        return rc, h
+
+    rc, h = create((1 << 32) // tiles)
+    assert rc == 5 and not h.value  # RMH_ERR_UNSUPPORTED, nothing returned
+    assert b"2^32" in _lib.lib().rmh_last_error()
+    rc, h = create((1 << 32) // tiles - 1)
+    assert rc == 0 and h.value
+    rays, chunks = ctypes.c_uint64(), ctypes.c_uint32()
+    _lib.call("rmh_batch_get_info", h, ctypes.byref(rays), ctypes.byref(chunks), None)
+    assert rays.value == ((1 << 32) // tiles - 1) * 128 * 1024
+    assert chunks.value == rays.value // 256
+    _lib.call("rmh_batch_destroy", h)
