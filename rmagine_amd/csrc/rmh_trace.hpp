@@ -184,9 +184,16 @@ __device__ __forceinline__ void test_leaf(const MapData& m, const TriRay& tr, in
 //     rays keep deeper stacks and could skip 2 % of visits: the deeper LDS stack pays more.
 constexpr int kLdsStack = 10;
 
+// Element of the per-lane spill array behind the LDS stack, addressed as private memory
+// (scratch instructions).  Through a generic pointer the accesses were flat instructions,
+// which count against both the vector-memory and the LDS counters, and a flat access that
+// may still be in flight makes the compiler wait for every outstanding vector load before
+// the first node word is used.
+typedef __attribute__((address_space(5))) volatile int PrivInt;
+
 // pushes a subtree (and, when culling, its entry distance)
 template <int STACK, int BLOCK, int LDS, bool CULL>
-__device__ __forceinline__ void push_ref(int2 (*stk)[BLOCK], volatile int* spill, int& sp, int ref, float t) {
+__device__ __forceinline__ void push_ref(int2 (*stk)[BLOCK], PrivInt* spill, int& sp, int ref, float t) {
   if constexpr (CULL) {
     if (sp < LDS) {
       stk[sp][threadIdx.x] = make_int2(ref, __float_as_int(t));
@@ -314,7 +321,7 @@ __device__ __forceinline__ void count_visit(VisitCount* vc, bool uniform) {
 // repeated with the shorter interval (the stored entry is the computed near value, at
 // most the distance of any point of the box the ray meets within the old interval).
 template <int STACK, int BLOCK, int LDS, bool CULL>
-__device__ __forceinline__ int pop_ref(int2 (*stk)[BLOCK], volatile int* spill, int& sp, float tbf) {
+__device__ __forceinline__ int pop_ref(int2 (*stk)[BLOCK], PrivInt* spill, int& sp, float tbf) {
   if constexpr (!CULL) {
     if (sp == 0) return kDone;
     --sp;
@@ -347,7 +354,7 @@ __device__ __forceinline__ int pop_ref(int2 (*stk)[BLOCK], volatile int* spill, 
 // walk without it.
 template <int OCT, int STACK, int BLOCK, int SPEC, bool COUNT, int LDS>
 __device__ __forceinline__ void walk(const MapData& m, const SlabRay& r, float tbf, int2 (*stk)[BLOCK],
-                                     volatile int* spill, int& ref, int& sp, int& leaf, bool spec, VisitCount* vc) {
+                                     PrivInt* spill, int& ref, int& sp, int& leaf, bool spec, VisitCount* vc) {
   while (ref >= 0 && ref != kDone) {
     float tmin0, tmax0, tmin1, tmax1;
     int2 nr;
@@ -472,7 +479,7 @@ __device__ __forceinline__ int pick_ref(uint32_t key, int r0, int r1, int r2, in
 // picks STACK >= 3 x the number of wide levels (at most 3 entries are pushed per level).
 template <int OCT, int STACK, int BLOCK, int LDS, bool CULL>
 __device__ __forceinline__ void walk4(const MapData& m, const SlabRay& r, float tbf, int2 (*stk)[BLOCK],
-                                      volatile int* spill, int& ref, int& sp) {
+                                      PrivInt* spill, int& ref, int& sp) {
   while (ref >= 0) {
     float tmin[4];
     bool hit[4];
@@ -558,7 +565,8 @@ __device__ __forceinline__ void trace(const MapData& m, const V3& o, const V3& d
   const TriRay tr = make_tri_ray(o, d);
   // volatile keeps the rare deep entries in scratch and stops the compiler from merging
   // the LDS and scratch pops into one flat load
-  volatile int spill[STACK > LDS ? 2 * (STACK - LDS) : 2];
+  volatile int spill_mem[STACK > LDS ? 2 * (STACK - LDS) : 2];
+  PrivInt* spill = (PrivInt*)spill_mem;
   float tbf = range_max;
   int ref = WIDE ? m.root4 : m.root;
   int sp = 0;
