@@ -82,7 +82,7 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the batched / pose-gradient / ICP workloads")
-    ap.add_argument("--extra-only", choices=["batched", "pose_gradient", "icp"],
+    ap.add_argument("--extra-only", choices=["batched", "pose_gradient", "icp", "next_rows"],
                     help="only run one secondary workload with default launch parameters (for profiling)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy 5 times (scale factor for HBM counters)")
@@ -432,6 +432,223 @@ def extra_icp() -> dict:
             "umeyama_fp64_max_abs_err_vs_cpu_oracle": umerr}
 
 
+def _timed_ms(fn, reps: int = 5) -> float:
+    """Mean launch time of fn() (one untimed call first), HIP events on the current stream."""
+    import torch
+    fn()
+    torch.cuda.synchronize()
+    a, b = _events()
+    ms = []
+    for _ in range(reps):
+        a.record()
+        fn()
+        b.record()
+        b.synchronize()
+        ms.append(a.elapsed_time(b))
+    return float(np.mean(ms))
+
+
+def ouster_ondn_model():
+    """128 x 1024 Ouster-style ray list with per-ray origins: the directions of the default
+    spherical sensor, each beam starting 15.8 mm from the sensor centre along its azimuth (the
+    beam-origin offset of Ouster sensor metadata)."""
+    from rmagine_amd import types as T
+    rows, cols = 128, 1024
+    phi = (np.float32(-22.5 * T.DEG_TO_RAD) + np.arange(rows, dtype=np.float64) * np.float64(T.f32(45.0 / 127.0 * T.DEG_TO_RAD)))
+    th = (np.float32(-np.pi) + np.arange(cols, dtype=np.float64) * np.float64(T.f32(2.0 * np.pi / 1024.0)))
+    P, Th = np.meshgrid(phi, th, indexing="ij")
+    dirs = np.stack([np.cos(P) * np.cos(Th), np.cos(P) * np.sin(Th), np.sin(P)], -1).reshape(-1, 3).astype(np.float32)
+    origs = (0.0158 * np.stack([np.cos(Th), np.sin(Th), np.zeros_like(Th)], -1)).reshape(-1, 3).astype(np.float32)
+    return T.OnDnModel(cols, rows, T.Interval(0.1, 100.0), origs, dirs)
+
+
+def extra_next_rows(do_cpu: bool, cpu_seconds: float = 3.0) -> dict:
+    """The parts either side of the simulate path, each timed with HIP events (mean of 5
+    launches after an untimed one) on the terrain of the default workload (1024^2 grid at
+    0.1 m, 2,093,058 triangles, 1,000 sensor poses):
+    * range noise (rmh_noise_apply, Gaussian / relative Gaussian / uniform dust) over
+      131,072,000 ranges, the size of one default step's output: 8 algorithmic bytes per range
+      (read and write), against 8 TB/s;
+    * OnDn simulation (per-ray origins): 1,000 poses x the 128 x 1024 Ouster-style ray list,
+      all seven attributes: 41 B written + 24 B of direction and origin read per ray; timed
+      with the sensor's defaults (direction-sorted, speculative walk) and after
+      autotune.tune_sensor picked its ray order and walk;
+    * closest-point queries (rmh_closest_points): 4,194,304 points within 2 m of the surface,
+      unbounded search: 12 B read + 40 B written per query;
+    * BVH refit (rmh_map_refit) after moving every vertex, synchronous (its one small
+      allocation included): per triangle 48 B of triangle and 16 B of normal written, 36 B of
+      vertices read; per node 64 B written;
+    * map file loading on the host (rmh_scene_load): the terrain as binary PLY.
+    With do_cpu, the oracle checks samples of the GPU results (OnDn poses, closest-point
+    queries, noise prefixes) and the OnDn poses it casts give its throughput."""
+    import tempfile
+
+    import torch
+
+    from rmagine_amd import noise as NZ
+    from rmagine_amd import scenes
+    from rmagine_amd.simulation import HipMap, OnDnSimulatorHip, load_map_file
+
+    dev = torch.device("cuda:0")
+    V, F, Tbm = scenes.terrain_workload(n_grid=1024, n_sensors=1000)
+    hm = HipMap(V, F, device=dev)
+    info = hm.info()
+    out = {"workload": f"terrain 1024x1024 fBm at 0.1 m ({len(F)} tris, {info['n_nodes']} nodes), "
+                       f"{len(Tbm)} sensor poses"}
+
+    # range noise
+    n = 131_072_000
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(5)
+    ranges = torch.rand(n, generator=gen, device=dev) * 99.9 + 0.1
+    kinds = (("gaussian", NZ.GaussianNoiseHip, (0.0, 0.01), 0), ("rel_gaussian", NZ.RelGaussianNoiseHip, (0.0, 0.002, 1.0), 1),
+             ("uniform_dust", NZ.UniformDustNoiseHip, (0.001, 0.5), 2))
+    noise = {"elements": n}
+    for name, cls, prm, kind in kinds:
+        nz = cls(*prm)
+        ms = _timed_ms(lambda: nz.apply(ranges))
+        noise[name] = roofline(f"k_noise<{name}>", 8 * n, ms, note="algorithmic bytes: one range read and written")
+        noise[name]["gelem_s"] = round(n / ms / 1e6, 2)
+        if do_cpu:
+            from oracle import noise as ON
+            k = 1 << 20
+            x = ranges[:k].clone()
+            host = x.cpu().numpy()
+            cls(*prm).apply(x)  # call count 0
+            t = time.perf_counter()
+            want = ON.apply(host, kind, *(list(prm) + [0.0] * (3 - len(prm))), max_range=10000.0, seed=42, epoch=0)
+            cel = time.perf_counter() - t
+            got = x.cpu().numpy()
+            if kind == 2:
+                bad = int((got != want).sum())
+            else:
+                bad = int((np.abs(got.astype(np.float64) - want) > np.spacing(np.abs(want))).sum())
+            noise[name]["cpu_oracle_gelem_s"] = round(k / cel / 1e9, 4)
+            noise[name]["oracle_check"] = {"elements": k, "outside_1_ulp" if kind != 2 else "differing": bad}
+    del ranges
+    torch.cuda.empty_cache()
+    out["range_noise"] = noise
+
+    # OnDn simulation
+    model = ouster_ondn_model()
+    sim = OnDnSimulatorHip(hm, device=dev)
+    sim.setModel(model)
+    Tbm_d = torch.as_tensor(Tbm, device=dev)
+    res = sim.simulate(Tbm_d)
+    rays = model.size() * len(Tbm)
+    ms_default = _timed_ms(lambda: sim.simulate(Tbm_d, res))
+    from rmagine_amd import autotune
+    tuned = autotune.tune_sensor(sim, Tbm_d, res)
+    ms = _timed_ms(lambda: sim.simulate(Tbm_d, res))
+    ondn = {"rays": rays, "ms": round(ms, 3), "mrays_s": round(rays / ms / 1e3, 1),
+            "walk": "speculative" if tuned["speculative"] else "classic",
+            "ray_order": "sorted" if tuned["rays_sorted"] else "list",
+            "untuned_ms": round(ms_default, 3), "tuner_trials_ms": tuned["trials"],
+            "roofline": roofline("k_simulate<OnDn> (traversal + attribute writers)", (BYTES_PER_RAY_ALL + 24) * rays, ms,
+                                 note="algorithmic bytes: 41 B of outputs written, 24 B of direction and origin read per ray")}
+    if do_cpu:
+        import oracle
+        from oracle import diff as D
+        om = oracle.OracleMap(V, F)
+        per = model.size()
+        acc, crays, cel, p = None, 0, 0.0, 0
+        while cel < cpu_seconds and p < len(Tbm):
+            t = time.perf_counter()
+            o = oracle.simulate(om, model, Tbm[p:p + 1], threads=os.cpu_count() or 1, flags=False)
+            cel += time.perf_counter() - t
+            sl = slice(p * per, (p + 1) * per)
+            g = {"hits": res.hits[sl].cpu().numpy(), "ranges": res.ranges[sl].cpu().numpy(),
+                 "points": res.points[sl].cpu().numpy(), "normals": res.normals[sl].cpu().numpy(),
+                 "face_ids": res.face_ids[sl].cpu().numpy().view(np.uint32),
+                 "geom_ids": res.geom_ids[sl].cpu().numpy().view(np.uint32),
+                 "object_ids": res.object_ids[sl].cpu().numpy().view(np.uint32)}
+            r = D.compare(g, {**o, "flags": np.zeros(len(o["ranges"]), np.uint8)}, model.range.max)
+            acc = D.merge(acc, {**r, "flagged": {}})
+            crays += per
+            p += 1
+        ondn["cpu_oracle_mrays_s"] = round(crays / cel / 1e6, 2)
+        ondn["oracle_check"] = {"poses": p, "rays": crays, "mismatching_rays": acc["mismatching_rays"],
+                                "range_max_err": acc["diff"]["ranges"]["max_diff"]}
+        del om
+    del res
+    torch.cuda.empty_cache()
+    out["ondn_simulate"] = ondn
+
+    # closest points
+    rng = np.random.default_rng(11)
+    nq = 1 << 22
+    lo, hi = V.min(0), V.max(0)
+    q = rng.uniform(lo, hi, size=(nq, 3)).astype(np.float32)
+    q[:, 2] = scenes.terrain_height_at(scenes.terrain_heights(1024, seed=scenes.SEED_BASE + 1), 0.1,
+                                       q[:, 0], q[:, 1]) + rng.uniform(-2.0, 2.0, nq).astype(np.float32)
+    qd = torch.as_tensor(q, device=dev)
+    cp = hm.closest_points(qd)
+    ms = _timed_ms(lambda: hm.closest_points(qd))
+    closest = {"queries": nq, "ms": round(ms, 3), "mqueries_s": round(nq / ms / 1e3, 1),
+               "roofline": roofline("k_closest", 52 * nq, ms, note="algorithmic bytes: 12 B query read, 40 B written")}
+    if do_cpu:
+        from oracle import closest as OC
+        k = 2
+        t = time.perf_counter()
+        d, _, f = OC.closest_points(V, F, q[:k])
+        cel = time.perf_counter() - t
+        gd = cp["dist"][:k].cpu().numpy()
+        gf = cp["face_ids"][:k].cpu().numpy().astype(np.int64)
+        closest["oracle_check"] = {"queries": k, "dist_max_rel_err": float(np.max(np.abs(gd - d) / np.maximum(d, 1e-30))),
+                                   "face_ids_equal": int((gf == f).sum()), "oracle_s_per_query": round(cel / k, 3),
+                                   "note": "oracle: brute force over every triangle (a check, not a baseline)"}
+    del cp, qd
+    out["closest_points"] = closest
+
+    # refit
+    Vm = V.copy()
+    Vm[:, 2] += (0.05 * np.sin(V[:, 0] * 0.7) * np.cos(V[:, 1] * 0.3)).astype(np.float32)
+    Vs = [torch.as_tensor(Vm, device=dev), torch.as_tensor(V, device=dev)]
+    it = [0]
+
+    def refit():
+        hm.refit(Vs[it[0] & 1])
+        it[0] += 1
+    ms = _timed_ms(refit, reps=6)  # ends on the moved vertices
+    alg = (48 + 16 + 36) * len(F) + 64 * info["n_nodes"]
+    refit_out = {"ms": round(ms, 3), "roofline": roofline("k_refit_tris + k_refit_nodes (rmh_map_refit)", alg, ms,
+                                                          note="algorithmic bytes: triangles, normals and vertices of "
+                                                               "every face, 64 B per node")}
+    # the refitted map casts exactly what a map built from the moved vertices casts
+    fresh = HipMap(Vm, F, device=dev)
+    sims = []
+    for m_ in (hm, fresh):
+        s_ = OnDnSimulatorHip(m_, device=dev)
+        s_.setModel(model)
+        sims.append(s_.simulate(Tbm_d[:16], attrs=("hits", "ranges", "face_ids")))
+    refit_out["vs_rebuilt_map_mismatches"] = int(sum(int((getattr(sims[0], k) != getattr(sims[1], k)).sum().item())
+                                                     for k in ("hits", "ranges", "face_ids")))
+    refit_out["rays_compared"] = 16 * model.size()
+    del fresh, sims
+    out["refit"] = refit_out
+
+    # map file loading (host)
+    with tempfile.TemporaryDirectory() as tmp:
+        path = os.path.join(tmp, "terrain.ply")
+        head = (f"ply\nformat binary_little_endian 1.0\nelement vertex {len(V)}\nproperty float x\nproperty float y\n"
+                f"property float z\nelement face {len(F)}\nproperty list uchar int vertex_indices\nend_header\n")
+        fb = np.empty(len(F), dtype=[("n", "u1"), ("i", "<i4", 3)])
+        fb["n"] = 3
+        fb["i"] = F.astype(np.int32)
+        with open(path, "wb") as fh:
+            fh.write(head.encode())
+            fh.write(np.ascontiguousarray(V, "<f4").tobytes())
+            fh.write(fb.tobytes())
+        size = os.path.getsize(path)
+        t = time.perf_counter()
+        s_ = load_map_file(path)
+        sec = time.perf_counter() - t
+        same = bool(np.array_equal(s_["vertices"], V) and np.array_equal(s_["faces"], F.astype(np.uint32)))
+    out["map_loading"] = {"format": "binary PLY", "bytes": size, "s": round(sec, 3), "mb_s": round(size / sec / 1e6, 1),
+                          "mtris_s": round(len(F) / sec / 1e6, 2), "arrays_equal_to_the_mesh": same}
+    return out
+
+
 def compare_layouts(names) -> None:
     """Binary against 4-wide quantised node layout on the same map: ms per simulate() (min of
     5 HIP-event timings after 2 warm-ups) and a bit-for-bit comparison of every output."""
@@ -534,7 +751,7 @@ def main():
 
     if args.extra_only:
         fn = {"batched": lambda: extra_batched(0.0, False, tune=False), "pose_gradient": extra_pose_grad,
-              "icp": extra_icp}[args.extra_only]
+              "icp": extra_icp, "next_rows": lambda: extra_next_rows(not args.no_cpu)}[args.extra_only]
         print(json.dumps({args.extra_only: fn()}))
         return
 
@@ -645,7 +862,8 @@ def main():
         del res
         torch.cuda.empty_cache()
         for name, fn in (("batched_heterogeneous", lambda: extra_batched(8.0, not args.no_cpu)),
-                         ("pose_gradient", extra_pose_grad), ("icp_graph", extra_icp)):
+                         ("pose_gradient", extra_pose_grad), ("icp_graph", extra_icp),
+                         ("next_rows", lambda: extra_next_rows(not args.no_cpu))):
             t0 = time.time()
             extra[name] = fn()
             log(f"{name}: {extra[name]} ({time.time() - t0:.1f} s)")

@@ -147,6 +147,14 @@ class SimulatorHip {
     RM_RMH_CHECK(rmh_sensor_set_ray_order(m_sensor.get(), sorted ? 1 : 0));
   }
 
+  // O1Dn/OnDn: walk the BVH speculatively (the default, fastest for scattered rays) or the
+  // classic way (fastest for a scanner's coherent ray grid); the results are identical, only
+  // the speed differs (image models ignore it)
+  void setWalk(bool speculative) {
+    if (!m_sensor) throw std::runtime_error("[SimulatorHip] setWalk(): no sensor model set");
+    RM_RMH_CHECK(rmh_sensor_set_walk(m_sensor.get(), speculative ? 1 : 0));
+  }
+
   // the librmh sensor of the current model (shared: a batch keeps it alive) and Tsb, for
   // BatchSimulatorHip and poseGradHip
   std::shared_ptr<rmh_sensor> sensorHandle() const { return m_sensor; }

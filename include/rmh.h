@@ -228,6 +228,17 @@ int rmh_sensor_destroy(rmh_sensor* sensor);
  * Image sensors (spherical, pinhole) ignore it and report 0. */
 int rmh_sensor_set_ray_order(rmh_sensor* sensor, int sorted);
 int rmh_sensor_get_ray_order(const rmh_sensor* sensor, int* sorted);
+/* Traversal walk of an O1Dn/OnDn sensor's rays in rmh_simulate and
+ * rmh_simulate_visit_stats (no counterpart in the reference, whose ray casts are one
+ * library call per ray): speculative != 0 = a lane that reaches a leaf holds it while the
+ * wave walks on, and the wave tests leaves once an eighth of its lanes hold one (the
+ * default; fastest for scattered rays such as ICP scans), 0 = the classic walk, in which
+ * the wave walks until every lane holds a leaf (fastest for a scanner's coherent ray
+ * grid).  Results are identical either way; a tuning choice of the autotuner.  A flag read
+ * at launch: no device work.  Batches and the pose gradient walk ray lists speculatively
+ * whatever the sensor holds.  Image sensors always walk the classic way and report 0. */
+int rmh_sensor_set_walk(rmh_sensor* sensor, int speculative);
+int rmh_sensor_get_walk(const rmh_sensor* sensor, int* speculative);
 int rmh_sensor_dims(const rmh_sensor* sensor, uint32_t* width, uint32_t* height);
 
 /* ---- simulation --------------------------------------------------------------------- */

@@ -66,6 +66,8 @@ _SIGNATURES = {
     "rmh_sensor_dims": (c_int, [c_void_p, ctypes.POINTER(c_u32), ctypes.POINTER(c_u32)]),
     "rmh_sensor_set_ray_order": (c_int, [c_void_p, c_int]),
     "rmh_sensor_get_ray_order": (c_int, [c_void_p, ctypes.POINTER(c_int)]),
+    "rmh_sensor_set_walk": (c_int, [c_void_p, c_int]),
+    "rmh_sensor_get_walk": (c_int, [c_void_p, ctypes.POINTER(c_int)]),
     "rmh_simulate_visit_stats": (c_int, [c_void_p, c_void_p, ctypes.POINTER(T.rmh_transform), c_void_p, c_u32,
                                          ctypes.POINTER(ctypes.c_uint64), c_void_p]),
     "rmh_simulate": (c_int, [c_void_p, c_void_p, ctypes.POINTER(T.rmh_transform), c_void_p, c_u32,

@@ -20,6 +20,11 @@ never results.  Results are cached as JSON keyed by device architecture, mesh
 size (power of two bucket of the face count) and sensor mix, in $RMH_TUNE_CACHE or
 ~/.cache/rmagine_amd/autotune.json.
 
+`tune_sensor` does the same for one O1Dn/OnDn simulator: it times the ray order (sorted or
+list) and the walk (speculative or classic) of its sensor on the caller's poses and keeps
+the fastest pair (a scanner's coherent ray grid walks fastest the classic way, often in its
+own order; scattered rays such as ICP scans speculatively and sorted).
+
 The reference has no counterpart (its launch shape is fixed by optixLaunch).
 """
 from __future__ import annotations
@@ -177,3 +182,38 @@ def tune_batch(batch, key: str | None = None, chunks=DEFAULT_CHUNKS, waves=DEFAU
         data[key] = {k: rec[k] for k in keys}
         _store(path, data)
     return dict(rec, cached=False)
+
+
+def tune_sensor(sim, Tbm, bundle=None, reps: int = 3, stream=None) -> dict:
+    """Times every (ray order, walk) pair of an O1Dn/OnDn simulator's sensor on `Tbm` with
+    HIP events (median of `reps` launches after one warm-up), applies the fastest and
+    returns {"rays_sorted", "speculative", "ms", "trials"}.  Image simulators have one choice
+    and return it untimed.  Results never change; no launch of the sensor may be in flight."""
+    from . import types as T
+    if not isinstance(sim.model, (T.O1DnModel, T.OnDnModel)):
+        return {"rays_sorted": False, "speculative": False, "ms": None, "trials": {}}
+    stream = stream or torch.cuda.current_stream()
+    if bundle is None:
+        bundle = sim.simulate(Tbm, stream=stream)
+    trials = {}
+    for sorted_ in (True, False):
+        sim.set_ray_order(sorted_)
+        for spec in (True, False):
+            sim.set_walk(spec)
+            sim.simulate(Tbm, bundle, stream=stream)
+            ms = []
+            for _ in range(reps):
+                a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                a.record(stream)
+                sim.simulate(Tbm, bundle, stream=stream)
+                b.record(stream)
+                b.synchronize()
+                ms.append(a.elapsed_time(b))
+            ms.sort()
+            trials[(sorted_, spec)] = ms[len(ms) // 2]
+    (best_sorted, best_spec), best = min(trials.items(), key=lambda kv: kv[1])
+    sim.set_ray_order(best_sorted)
+    sim.set_walk(best_spec)
+    return {"rays_sorted": best_sorted, "speculative": best_spec, "ms": best,
+            "trials": {f"{'sorted' if k[0] else 'list'}/{'speculative' if k[1] else 'classic'}": round(v, 3)
+                       for k, v in trials.items()}}

@@ -177,5 +177,6 @@ struct rmh_sensor {
   rmh_vec3* origs_list = nullptr;
   uint32_t* perm = nullptr;
   int sorted = 1;
+  int speculative = 1;            // O1Dn / OnDn walk in rmh_simulate: speculative or classic
   int device = 0;
 };

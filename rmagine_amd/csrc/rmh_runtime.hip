@@ -275,6 +275,20 @@ int rmh_sensor_get_ray_order(const rmh_sensor* s, int* sorted) {
   return RMH_OK;
 }
 
+// A flag read when rmh_simulate launches: no device work, nothing to wait for.
+int rmh_sensor_set_walk(rmh_sensor* s, int speculative) {
+  if (!s) return fail(RMH_ERR_INVALID, "rmh_sensor_set_walk: NULL sensor");
+  if (s->kind != RMH_MODEL_O1DN && s->kind != RMH_MODEL_ONDN) return RMH_OK;  // images: classic walk
+  s->speculative = speculative ? 1 : 0;
+  return RMH_OK;
+}
+
+int rmh_sensor_get_walk(const rmh_sensor* s, int* speculative) {
+  if (!s || !speculative) return fail(RMH_ERR_INVALID, "rmh_sensor_get_walk: NULL argument");
+  *speculative = (s->kind == RMH_MODEL_O1DN || s->kind == RMH_MODEL_ONDN) ? s->speculative : 0;
+  return RMH_OK;
+}
+
 int rmh_sensor_create(int kind, const void* model, rmh_sensor** out) {
   if (!model || !out) return fail(RMH_ERR_INVALID, "rmh_sensor_create: NULL argument");
   *out = nullptr;

@@ -75,7 +75,7 @@ struct SimArgs {
 
 // Launch bounds ask for 8 waves per SIMD (64 VGPRs); with the writers' values recomputed or
 // re-read after the traversal, nothing spills (62-64 VGPRs, 62 SGPRs).
-template <int KIND, int STACK, bool WIDE>
+template <int KIND, int STACK, bool WIDE, int SPEC>
 __global__ __launch_bounds__(kWave, 8) void k_simulate(const SimArgs a) {
   __shared__ int2 stk[kLdsStack][kWave];
   // block, pose and tile indices are wave-uniform: divided on the scalar unit, they stay in
@@ -95,7 +95,7 @@ __global__ __launch_bounds__(kWave, 8) void k_simulate(const SimArgs a) {
     const Tf Tsm = tmul(load_tf(a.Tbm[pose]), a.Tsb);  // Tsm = Tbm * Tsb (SphereSimulatorEmbree.tcc:69-70)
     const V3 dm = qrot(Tsm.R, ds);
     const V3 om = map_origin<KIND>(Tsm, os);
-    trace<STACK, kWave, WIDE, spec_mode<KIND>()>(a.map, om, dm, a.g.range_max, stk, h);
+    trace<STACK, kWave, WIDE, SPEC>(a.map, om, dm, a.g.range_max, stk, h);
   }
   // The ray set-up is recomputed for the attribute writers instead of being held through
   // the traversal: at 64 VGPRs those values would otherwise be spilled to scratch.  The
@@ -110,31 +110,39 @@ __global__ __launch_bounds__(kWave, 8) void k_simulate(const SimArgs a) {
   write_outputs<KIND>(ap->out, glob, ap->g, ap->map, Tsm, ds, os, h);
 }
 
+// Image models walk the classic way; ray lists speculatively or the classic way, as the
+// sensor is set (rmh_sensor_set_walk).
 template <int KIND, int STACK, bool WIDE>
-hipError_t launch_kind(const SimArgs& a, uint32_t n_poses, hipStream_t stream) {
-  k_simulate<KIND, STACK, WIDE><<<n_poses * a.tl.tiles, kWave, 0, stream>>>(a);
+hipError_t launch_kind(const SimArgs& a, bool spec, uint32_t n_poses, hipStream_t stream) {
+  if constexpr (speculative_walk(KIND)) {
+    if (spec) {
+      k_simulate<KIND, STACK, WIDE, kSpecOn><<<n_poses * a.tl.tiles, kWave, 0, stream>>>(a);
+      return hipGetLastError();
+    }
+  }
+  k_simulate<KIND, STACK, WIDE, kSpecOff><<<n_poses * a.tl.tiles, kWave, 0, stream>>>(a);
   return hipGetLastError();
 }
 
 template <int STACK, bool WIDE>
-hipError_t launch_stack(int kind, const SimArgs& a, uint32_t n_poses, hipStream_t stream) {
+hipError_t launch_stack(int kind, bool spec, const SimArgs& a, uint32_t n_poses, hipStream_t stream) {
   switch (kind) {
-    case RMH_MODEL_SPHERICAL: return launch_kind<RMH_MODEL_SPHERICAL, STACK, WIDE>(a, n_poses, stream);
-    case RMH_MODEL_PINHOLE: return launch_kind<RMH_MODEL_PINHOLE, STACK, WIDE>(a, n_poses, stream);
-    case RMH_MODEL_O1DN: return launch_kind<RMH_MODEL_O1DN, STACK, WIDE>(a, n_poses, stream);
-    default: return launch_kind<RMH_MODEL_ONDN, STACK, WIDE>(a, n_poses, stream);
+    case RMH_MODEL_SPHERICAL: return launch_kind<RMH_MODEL_SPHERICAL, STACK, WIDE>(a, spec, n_poses, stream);
+    case RMH_MODEL_PINHOLE: return launch_kind<RMH_MODEL_PINHOLE, STACK, WIDE>(a, spec, n_poses, stream);
+    case RMH_MODEL_O1DN: return launch_kind<RMH_MODEL_O1DN, STACK, WIDE>(a, spec, n_poses, stream);
+    default: return launch_kind<RMH_MODEL_ONDN, STACK, WIDE>(a, spec, n_poses, stream);
   }
 }
 
-hipError_t launch_sim(const TravSel& ts, int kind, const SimArgs& a, uint32_t n_poses, hipStream_t stream) {
+hipError_t launch_sim(const TravSel& ts, int kind, bool spec, const SimArgs& a, uint32_t n_poses, hipStream_t stream) {
   if (ts.wide) {
-    if (ts.stack == 32) return launch_stack<32, true>(kind, a, n_poses, stream);
-    if (ts.stack == 64) return launch_stack<64, true>(kind, a, n_poses, stream);
-    return launch_stack<128, true>(kind, a, n_poses, stream);
+    if (ts.stack == 32) return launch_stack<32, true>(kind, spec, a, n_poses, stream);
+    if (ts.stack == 64) return launch_stack<64, true>(kind, spec, a, n_poses, stream);
+    return launch_stack<128, true>(kind, spec, a, n_poses, stream);
   }
-  if (ts.stack == 32) return launch_stack<32, false>(kind, a, n_poses, stream);
-  if (ts.stack == 64) return launch_stack<64, false>(kind, a, n_poses, stream);
-  return launch_stack<128, false>(kind, a, n_poses, stream);
+  if (ts.stack == 32) return launch_stack<32, false>(kind, spec, a, n_poses, stream);
+  if (ts.stack == 64) return launch_stack<64, false>(kind, spec, a, n_poses, stream);
+  return launch_stack<128, false>(kind, spec, a, n_poses, stream);
 }
 
 // ---- traversal counts (diagnostic) ------------------------------------------------------
@@ -150,7 +158,7 @@ struct VisitArgs {
 
 // k_simulate's rays and traversal (binary nodes, same speculation mode), counting instead
 // of writing: per-lane counts are summed over the wave and added to the totals.
-template <int KIND, int STACK>
+template <int KIND, int STACK, int SPEC>
 __global__ __launch_bounds__(kWave, 8) void k_visit_stats(const VisitArgs a) {
   __shared__ int2 stk[kLdsStack][kWave];
   const uint32_t lb = xcd_contiguous(blockIdx.x, gridDim.x);
@@ -164,7 +172,7 @@ __global__ __launch_bounds__(kWave, 8) void k_visit_stats(const VisitArgs a) {
     gen_ray<KIND>(a.g, vid, hid, lid, ds, os);
     const Tf Tsm = tmul(load_tf(a.Tbm[pl]), a.Tsb);
     Hit h;
-    trace<STACK, kWave, false, spec_mode<KIND>(), true>(a.map, map_origin<KIND>(Tsm, os), qrot(Tsm.R, ds),
+    trace<STACK, kWave, false, SPEC, true>(a.map, map_origin<KIND>(Tsm, os), qrot(Tsm.R, ds),
                                                        a.g.range_max, stk, h, false, &vc);
   }
   uint32_t v[6] = {vc.vec_nodes, vc.uni_nodes, vc.leaves, vc.tris, vc.wave_iters, vc.wave_uni_iters};
@@ -175,13 +183,19 @@ __global__ __launch_bounds__(kWave, 8) void k_visit_stats(const VisitArgs a) {
 }
 
 template <int STACK>
-hipError_t launch_visit(int kind, const VisitArgs& a, uint32_t n_poses, hipStream_t stream) {
+hipError_t launch_visit(int kind, bool spec, const VisitArgs& a, uint32_t n_poses, hipStream_t stream) {
   const uint32_t blocks = n_poses * a.tl.tiles;
   switch (kind) {
-    case RMH_MODEL_SPHERICAL: k_visit_stats<RMH_MODEL_SPHERICAL, STACK><<<blocks, kWave, 0, stream>>>(a); break;
-    case RMH_MODEL_PINHOLE: k_visit_stats<RMH_MODEL_PINHOLE, STACK><<<blocks, kWave, 0, stream>>>(a); break;
-    case RMH_MODEL_O1DN: k_visit_stats<RMH_MODEL_O1DN, STACK><<<blocks, kWave, 0, stream>>>(a); break;
-    default: k_visit_stats<RMH_MODEL_ONDN, STACK><<<blocks, kWave, 0, stream>>>(a); break;
+    case RMH_MODEL_SPHERICAL: k_visit_stats<RMH_MODEL_SPHERICAL, STACK, kSpecOff><<<blocks, kWave, 0, stream>>>(a); break;
+    case RMH_MODEL_PINHOLE: k_visit_stats<RMH_MODEL_PINHOLE, STACK, kSpecOff><<<blocks, kWave, 0, stream>>>(a); break;
+    case RMH_MODEL_O1DN:
+      if (spec) k_visit_stats<RMH_MODEL_O1DN, STACK, kSpecOn><<<blocks, kWave, 0, stream>>>(a);
+      else k_visit_stats<RMH_MODEL_O1DN, STACK, kSpecOff><<<blocks, kWave, 0, stream>>>(a);
+      break;
+    default:
+      if (spec) k_visit_stats<RMH_MODEL_ONDN, STACK, kSpecOn><<<blocks, kWave, 0, stream>>>(a);
+      else k_visit_stats<RMH_MODEL_ONDN, STACK, kSpecOff><<<blocks, kWave, 0, stream>>>(a);
+      break;
   }
   return hipGetLastError();
 }
@@ -441,7 +455,7 @@ int rmh_simulate(const rmh_map* map, const rmh_sensor* sensor, const rmh_transfo
   for (uint32_t p0 = 0; p0 < n_poses; p0 += poses_per_launch) {
     const uint32_t np = std::min<uint32_t>(poses_per_launch, n_poses - p0);
     a.pose_offset = p0;
-    const hipError_t e = launch_sim(ts, sensor->kind, a, np, stream);
+    const hipError_t e = launch_sim(ts, sensor->kind, sensor->speculative != 0, a, np, stream);
     if (e != hipSuccess) return hip_fail(e, "rmh_simulate launch");
   }
   return RMH_OK;
@@ -474,9 +488,9 @@ int rmh_simulate_visit_stats(const rmh_map* map, const rmh_sensor* sensor, const
   hipError_t e = hipMemsetAsync(dev, 0, sizeof(unsigned long long) * 6, stream);
   a.counts = dev;
   if (e == hipSuccess) {
-    if (depth <= 32) e = launch_visit<32>(sensor->kind, a, n_poses, stream);
-    else if (depth <= 64) e = launch_visit<64>(sensor->kind, a, n_poses, stream);
-    else e = launch_visit<128>(sensor->kind, a, n_poses, stream);
+    if (depth <= 32) e = launch_visit<32>(sensor->kind, sensor->speculative != 0, a, n_poses, stream);
+    else if (depth <= 64) e = launch_visit<64>(sensor->kind, sensor->speculative != 0, a, n_poses, stream);
+    else e = launch_visit<128>(sensor->kind, sensor->speculative != 0, a, n_poses, stream);
   }
   unsigned long long h[6] = {0, 0, 0, 0, 0, 0};
   if (e == hipSuccess) e = hipMemcpyAsync(h, dev, sizeof(h), hipMemcpyDeviceToHost, stream);

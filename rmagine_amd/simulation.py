@@ -252,6 +252,14 @@ class _SensorHandle:
         _lib.call("rmh_sensor_get_ray_order", self.handle, ctypes.byref(v))
         return bool(v.value)
 
+    def set_walk(self, speculative: bool) -> None:
+        _lib.call("rmh_sensor_set_walk", self.handle, 1 if speculative else 0)
+
+    def walk_speculative(self) -> bool:
+        v = ctypes.c_int()
+        _lib.call("rmh_sensor_get_walk", self.handle, ctypes.byref(v))
+        return bool(v.value)
+
     def __del__(self):
         h = getattr(self, "handle", None)
         if h is not None and h.value:
@@ -300,6 +308,18 @@ class SimulatorHip:
 
     def ray_order_sorted(self) -> bool:
         return self._sensor is not None and self._sensor.ray_order_sorted()
+
+    def set_walk(self, speculative: bool = True) -> None:
+        """O1Dn/OnDn: walk the BVH speculatively (default: lanes that reach a leaf hold it
+        while the wave walks on; fastest for scattered rays) or the classic way (the wave
+        walks until every lane holds a leaf; fastest for a scanner's coherent ray grid).  The
+        results are identical; only the speed differs.  Image models ignore it."""
+        if self._sensor is None:
+            raise RuntimeError("set_walk: no sensor model set")
+        self._sensor.set_walk(speculative)
+
+    def walk_speculative(self) -> bool:
+        return self._sensor is not None and self._sensor.walk_speculative()
 
     def simulate(self, Tbm, bundle: Bundle | None = None, attrs=IntAttrAll, stream=None) -> Bundle:
         """Casts every ray of the model from every pose in Tbm ((N,8) float32 tensor).
