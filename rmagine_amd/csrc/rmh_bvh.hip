@@ -142,63 +142,92 @@ __device__ inline float half_area(float4 a, float4 b) {
   return dx * dy + dy * dz + dz * dx;
 }
 
-// Step 5.  node_lo.w = SAH cost (area weighted), node_hi.w = primitive count.
-__global__ __launch_bounds__(kBlock) void k_bottom_up(int n, const uint32_t* __restrict__ vals,
-                                                      const float4* __restrict__ flo, const float4* __restrict__ fhi,
-                                                      const uint32_t* __restrict__ child,
-                                                      const int* __restrict__ parent_int,
-                                                      const int* __restrict__ parent_leaf,
-                                                      float4* node_lo, float4* node_hi,
-                                                      uint32_t* arrive, uint8_t* collapse, uint32_t* subtree,
-                                                      uint32_t max_leaf, float c_node, float c_tri) {
-  const int k = blockIdx.x * kBlock + threadIdx.x;
-  if (k >= n) return;
-  int p = parent_leaf[k];
-  while (p >= 0) {
-    // acquire-release at agent scope: the node data this thread wrote is visible before
-    // the count, and the sibling's data after it (explicit fences around it only
-    // repeated the L2 write-back and invalidate of the atomic itself)
-    const uint32_t old = __hip_atomic_fetch_add(&arrive[p], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    if (old == 0u) return;  // first arrival: the sibling's thread finishes this node
-    float4 lo[2], hi[2];
-    float cost[2], cnt[2];
-    uint32_t below = 0;  // interior nodes of the final tree in the children's subtrees
-    for (int c = 0; c < 2; c++) {
-      const uint32_t ch = child[2 * p + c];
-      if (ch & kLeafBit) {
-        const uint32_t f = vals[ch & ~kLeafBit];
-        lo[c] = flo[f];
-        hi[c] = fhi[f];
-        cnt[c] = 1.0f;
-        cost[c] = c_tri * half_area(lo[c], hi[c]);
-      } else {
-        // written by the sibling's thread before its release; read after our acquire
-        lo[c] = node_lo[ch];
-        hi[c] = node_hi[ch];
-        cost[c] = lo[c].w;
-        cnt[c] = hi[c].w;
-        below += subtree[ch];
-      }
+// Step 5.  node_lo.w = SAH cost (area weighted), node_hi.w = primitive count.  One launch
+// per depth of the radix tree, deepest first (make_levels): every node of the level reads
+// its children, which earlier launches finished, so no arrival counters are needed (an
+// agent-scope acquire-release atomic writes back and invalidates the L2 of an XCD each time,
+// which made the counter-driven pass 7 ms on the 2.1M-triangle terrain).
+__global__ __launch_bounds__(kBlock) void k_bottom_up_level(const uint32_t* __restrict__ order, uint32_t lo, uint32_t hi,
+                                                            const uint32_t* __restrict__ vals,
+                                                            const float4* __restrict__ flo, const float4* __restrict__ fhi,
+                                                            const uint32_t* __restrict__ child,
+                                                            float4* node_lo, float4* node_hi,
+                                                            uint8_t* collapse, uint32_t* subtree,
+                                                            uint32_t max_leaf, float c_node, float c_tri) {
+  const uint32_t k = lo + blockIdx.x * kBlock + threadIdx.x;
+  if (k >= hi) return;
+  const uint32_t p = order[k];
+  float4 lo2[2], hi2[2];
+  float cost[2], cnt[2];
+  uint32_t below = 0;  // interior nodes of the final tree in the children's subtrees
+  for (int c = 0; c < 2; c++) {
+    const uint32_t ch = child[2 * p + c];
+    if (ch & kLeafBit) {
+      const uint32_t f = vals[ch & ~kLeafBit];
+      lo2[c] = flo[f];
+      hi2[c] = fhi[f];
+      cnt[c] = 1.0f;
+      cost[c] = c_tri * half_area(lo2[c], hi2[c]);
+    } else {
+      lo2[c] = node_lo[ch];
+      hi2[c] = node_hi[ch];
+      cost[c] = lo2[c].w;
+      cnt[c] = hi2[c].w;
+      below += subtree[ch];
     }
-    float4 L = make_float4(fminf(lo[0].x, lo[1].x), fminf(lo[0].y, lo[1].y), fminf(lo[0].z, lo[1].z), 0.0f);
-    float4 H = make_float4(fmaxf(hi[0].x, hi[1].x), fmaxf(hi[0].y, hi[1].y), fmaxf(hi[0].z, hi[1].z), 0.0f);
-    const float count = cnt[0] + cnt[1];
-    const float area = half_area(L, H);
-    const float split_cost = c_node * area + cost[0] + cost[1];
-    float node_cost = split_cost;
-    uint8_t col = 0;
-    if (count <= static_cast<float>(max_leaf)) {
-      const float leaf_cost = c_tri * area * count;
-      if (leaf_cost <= split_cost) { node_cost = leaf_cost; col = 1; }
-    }
-    L.w = node_cost;
-    H.w = count;
-    node_lo[p] = L;
-    node_hi[p] = H;
-    collapse[p] = col;
-    subtree[p] = col ? 0u : 1u + below;  // a collapsed node becomes a leaf of the final tree
-    p = parent_int[p];
   }
+  float4 L = make_float4(fminf(lo2[0].x, lo2[1].x), fminf(lo2[0].y, lo2[1].y), fminf(lo2[0].z, lo2[1].z), 0.0f);
+  float4 H = make_float4(fmaxf(hi2[0].x, hi2[1].x), fmaxf(hi2[0].y, hi2[1].y), fmaxf(hi2[0].z, hi2[1].z), 0.0f);
+  const float count = cnt[0] + cnt[1];
+  const float area = half_area(L, H);
+  const float split_cost = c_node * area + cost[0] + cost[1];
+  float node_cost = split_cost;
+  uint8_t col = 0;
+  if (count <= static_cast<float>(max_leaf)) {
+    const float leaf_cost = c_tri * area * count;
+    if (leaf_cost <= split_cost) { node_cost = leaf_cost; col = 1; }
+  }
+  L.w = node_cost;
+  H.w = count;
+  node_lo[p] = L;
+  node_hi[p] = H;
+  collapse[p] = col;
+  subtree[p] = col ? 0u : 1u + below;  // a collapsed node becomes a leaf of the final tree
+}
+
+// Sort key of every interior node of the radix tree for the level-by-level passes: its
+// depth (the root, node 0, has parent -1), deepest first, and within a depth the nodes
+// that get a treelet (count >= gamma; none when count is NULL) before the others.  A parent
+// chain that leaves the node range or runs longer than kMaxTreeWalk steps (a cycle) sets err
+// bit 8.
+__global__ __launch_bounds__(kBlock) void k_level_keys(int n, const int* __restrict__ parent_int,
+                                                       const uint32_t* __restrict__ count, uint32_t gamma,
+                                                       uint32_t* __restrict__ keys, uint32_t* __restrict__ idx,
+                                                       uint32_t* err) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n - 1) return;
+  uint32_t d = 0;
+  for (int p = parent_int[i]; p >= 0; p = parent_int[p]) {
+    if (++d > static_cast<uint32_t>(kMaxTreeWalk) || p >= n - 1) { atomicOr(err, 8u); d = 0; break; }
+  }
+  const bool need = count && count[i] >= gamma;
+  keys[i] = 2u * (static_cast<uint32_t>(kMaxTreeWalk) - d) + (need ? 0u : 1u);
+  idx[i] = static_cast<uint32_t>(i);
+}
+
+// start[key] = first position of `key` in the sorted keys (untouched for absent keys)
+__global__ __launch_bounds__(kBlock) void k_level_starts(uint32_t ni, const uint32_t* __restrict__ skeys,
+                                                         uint32_t* __restrict__ start) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= ni) return;
+  if (i == 0 || skeys[i] != skeys[i - 1]) start[skeys[i]] = i;
+}
+
+// Triangles below every interior node of the radix tree: its range of sorted leaves.
+__global__ __launch_bounds__(kBlock) void k_range_counts(uint32_t ni, const uint32_t* __restrict__ range,
+                                                         uint32_t* __restrict__ count) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < ni) count[i] = range[2 * i + 1] - range[2 * i] + 1u;
 }
 
 // Outward padding so that the FP32 slab test stays conservative for boxes of zero
@@ -405,50 +434,30 @@ __device__ void treelet_wave(const TreeArrays& t, int root, float c_node, float 
   __syncthreads();
 }
 
-// One bottom-up pass, one wave per workgroup, arrival counters as in k_bottom_up: the
-// second lane to reach an interior node sets its box, count and cost from its children,
-// whose subtrees are final.  Nodes with at least `gamma` triangles below are then
-// restructured one after the other by the whole wave (treelet_wave); lanes that are done
-// keep taking part until every lane of the wave is done.  A treelet only involves nodes of
-// its subtree, so no other wave touches them.
-__global__ __launch_bounds__(64) void k_treelet(int n, TreeArrays t, uint32_t* arrive, uint32_t gamma, float c_node,
-                                                float c_tri, uint32_t* err) {
+// One bottom-up treelet pass runs level by level over the current tree, deepest first
+// (make_levels): at each depth, k_refresh_nodes sets the box, count and cost of the nodes
+// with fewer than `gamma` triangles below from their children (thread per node), and
+// k_treelet_nodes does the same for the others and restructures each with one wave
+// (treelet_wave).  Children are final when their parent's level runs (earlier launches), a
+// treelet only involves nodes of its subtree, and the nodes of one depth have disjoint
+// subtrees, so no counters or fences are needed.  Which nodes get a treelet is known in
+// advance: a node's triangle count depends only on its leaf set, which restructuring below
+// it does not change.
+__global__ __launch_bounds__(kBlock) void k_refresh_nodes(const uint32_t* __restrict__ order, uint32_t lo, uint32_t hi,
+                                                          TreeArrays t, float c_node, float c_tri) {
+  const uint32_t k = lo + blockIdx.x * kBlock + threadIdx.x;
+  if (k < hi) refresh_node(t, static_cast<int>(order[k]), c_node, c_tri);
+}
+
+__global__ __launch_bounds__(64) void k_treelet_nodes(const uint32_t* __restrict__ order, uint32_t lo, TreeArrays t,
+                                                      float c_node, float c_tri) {
   __shared__ float s_area[1 << kTreeletLeaves];
   __shared__ float s_copt[1 << kTreeletLeaves];
   __shared__ uint8_t s_part[1 << kTreeletLeaves];
-  const int k = blockIdx.x * 64 + threadIdx.x;
-  bool active = k < n;
-  int p = active ? t.parent_leaf[k] : -1;
-  int steps = 0;
-  while (__any(active)) {
-    bool need = false;
-    if (active) {
-      if (p < 0) {
-        active = false;
-      } else if (steps++ >= kMaxTreeWalk || p >= n - 1) {
-        atomicOr(err, 8u);
-        active = false;
-      } else {
-        const uint32_t old = __hip_atomic_fetch_add(&arrive[p], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        if (old == 0u) {
-          active = false;  // first arrival: the sibling's lane finishes this node
-        } else {
-          refresh_node(t, p, c_node, c_tri);
-          need = t.count[p] >= gamma;
-        }
-      }
-    }
-    uint64_t todo = __ballot(need);
-    if (todo) {
-      while (todo) {
-        const int l = __ffsll(static_cast<unsigned long long>(todo)) - 1;
-        todo &= todo - 1;
-        treelet_wave(t, __shfl(p, l, 64), c_node, c_tri, s_area, s_copt, s_part);
-      }
-      __threadfence();  // the treelet writes of lane 0 before any lane's next arrival
-    }
-    if (active) p = t.parent_int[p];
-  }
+  const int p = static_cast<int>(order[lo + blockIdx.x]);
+  if (threadIdx.x == 0) refresh_node(t, p, c_node, c_tri);
+  __syncthreads();  // lane 0's box and cost before the wave reads them
+  treelet_wave(t, p, c_node, c_tri, s_area, s_copt, s_part);
 }
 
 // Structure check of the refined tree before anything walks it: every child slot points
@@ -676,6 +685,65 @@ struct DevBuf {
 
 inline uint32_t blocks_for(uint64_t n) { return static_cast<uint32_t>((n + kBlock - 1) / kBlock); }
 
+// One launch range of the level-by-level passes: order[lo, hi) are nodes of one depth, with
+// or without a treelet.
+struct LevelClass {
+  uint32_t lo, hi;
+  bool treelet;
+};
+
+// Scratch of make_levels for a tree of ni interior nodes.
+struct LevelBufs {
+  DevBuf keys, keys2, idx, order, start, tmp;
+  size_t tmp_bytes = 0;
+};
+
+constexpr uint32_t kLevelKeys = 2u * (static_cast<uint32_t>(kMaxTreeWalk) + 1u);
+
+hipError_t level_bufs(LevelBufs& b, uint32_t ni, hipStream_t stream) {
+  hipError_t e;
+  if ((e = hipMalloc(&b.keys.p, sizeof(uint32_t) * ni)) != hipSuccess) return e;
+  if ((e = hipMalloc(&b.keys2.p, sizeof(uint32_t) * ni)) != hipSuccess) return e;
+  if ((e = hipMalloc(&b.idx.p, sizeof(uint32_t) * ni)) != hipSuccess) return e;
+  if ((e = hipMalloc(&b.order.p, sizeof(uint32_t) * ni)) != hipSuccess) return e;
+  if ((e = hipMalloc(&b.start.p, sizeof(uint32_t) * kLevelKeys)) != hipSuccess) return e;
+  e = rocprim::radix_sort_pairs(nullptr, b.tmp_bytes, b.keys.as<uint32_t>(), b.keys2.as<uint32_t>(),
+                                b.idx.as<uint32_t>(), b.order.as<uint32_t>(), ni, 0u, 14u, stream);
+  if (e != hipSuccess) return e;
+  return hipMalloc(&b.tmp.p, b.tmp_bytes);
+}
+
+// Interior nodes of the radix tree (n leaves, n - 1 interior nodes) grouped by depth,
+// deepest first, into b.order; `classes` receives the launch ranges in that order (within a
+// depth the treelet nodes first).  count = NULL: no treelets.  Synchronous: the range sizes
+// size the launches.  A broken tree sets err bit 8.
+hipError_t make_levels(int n, const int* parent_int, const uint32_t* count, uint32_t gamma, LevelBufs& b,
+                       uint32_t* err, hipStream_t stream, std::vector<LevelClass>& classes) {
+  const uint32_t ni = static_cast<uint32_t>(n - 1);
+  hipError_t e;
+  k_level_keys<<<blocks_for(ni), kBlock, 0, stream>>>(n, parent_int, count, gamma, b.keys.as<uint32_t>(),
+                                                      b.idx.as<uint32_t>(), err);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  if ((e = rocprim::radix_sort_pairs(b.tmp.p, b.tmp_bytes, b.keys.as<uint32_t>(), b.keys2.as<uint32_t>(),
+                                     b.idx.as<uint32_t>(), b.order.as<uint32_t>(), ni, 0u, 14u, stream)) != hipSuccess)
+    return e;
+  if ((e = hipMemsetAsync(b.start.p, 0xFF, sizeof(uint32_t) * kLevelKeys, stream)) != hipSuccess) return e;
+  k_level_starts<<<blocks_for(ni), kBlock, 0, stream>>>(ni, b.keys2.as<uint32_t>(), b.start.as<uint32_t>());
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  std::vector<uint32_t> st(kLevelKeys);
+  if ((e = hipMemcpyAsync(st.data(), b.start.p, sizeof(uint32_t) * kLevelKeys, hipMemcpyDeviceToHost, stream)) !=
+      hipSuccess)
+    return e;
+  if ((e = hipStreamSynchronize(stream)) != hipSuccess) return e;
+  classes.clear();
+  for (uint32_t k = 0; k < kLevelKeys; k++) {
+    if (st[k] == 0xFFFFFFFFu) continue;
+    if (!classes.empty()) classes.back().hi = st[k];
+    classes.push_back(LevelClass{st[k], ni, (k & 1u) == 0u});
+  }
+  return hipSuccess;
+}
+
 void map_free(rmh_map* m) {
   if (!m) return;
   if (m->vertices) (void)hipFree(m->vertices);
@@ -686,6 +754,7 @@ void map_free(rmh_map* m) {
   if (m->parent) (void)hipFree(m->parent);
   if (m->reach) (void)hipFree(m->reach);
   if (m->nodes4) (void)hipFree(m->nodes4);
+  if (m->refit_order) (void)hipFree(m->refit_order);
   delete m;
 }
 
@@ -899,8 +968,10 @@ static int map_build(const float* vertices, uint32_t nv, const uint32_t* faces, 
 
   const int n = static_cast<int>(nf);
   const uint32_t ni = nf > 1 ? nf - 1 : 0;
-  DevBuf flo, fhi, bounds, err, keys, keys2, vals, vals2, child, range, pint, pleaf, nlo, nhi, arrive, coll,
-      stats, moff, tmp, subtree;
+  DevBuf flo, fhi, bounds, err, keys, keys2, vals, vals2, child, range, pint, pleaf, nlo, nhi, coll, stats, moff, tmp,
+      subtree;
+  LevelBufs lb;                    // scratch of the level-by-level passes (make_levels)
+  std::vector<LevelClass> levels;  // their launch ranges
   BUILD_TRY(hipMalloc(&flo.p, sizeof(float4) * nf));
   BUILD_TRY(hipMalloc(&fhi.p, sizeof(float4) * nf));
   BUILD_TRY(hipMalloc(&bounds.p, sizeof(float) * 8));
@@ -915,9 +986,9 @@ static int map_build(const float* vertices, uint32_t nv, const uint32_t* faces, 
   BUILD_TRY(hipMalloc(&pleaf.p, sizeof(int) * nf));
   BUILD_TRY(hipMalloc(&nlo.p, sizeof(float4) * (ni + 1)));
   BUILD_TRY(hipMalloc(&nhi.p, sizeof(float4) * (ni + 1)));
-  BUILD_TRY(hipMalloc(&arrive.p, sizeof(uint32_t) * (ni + 1)));
   BUILD_TRY(hipMalloc(&coll.p, sizeof(uint8_t) * (ni + 1)));
   BUILD_TRY(hipMalloc(&subtree.p, sizeof(uint32_t) * (ni + 1)));
+  if (ni > 0) BUILD_TRY(level_bufs(lb, ni, stream));
   BUILD_TRY(hipMalloc(&stats.p, sizeof(uint32_t) * 4));
   BUILD_TRY(hipMalloc(&moff.p, sizeof(uint32_t) * (n_meshes + 1)));
   BUILD_TRY(hipMalloc(&m->tris, sizeof(float4) * 3 * static_cast<size_t>(nf)));
@@ -978,11 +1049,23 @@ static int map_build(const float* vertices, uint32_t nv, const uint32_t* faces, 
       k_leaf_boxes<<<blocks_for(nf), kBlock, 0, stream>>>(nf, svals, flo.as<float4>(), fhi.as<float4>(), lbox.as<float>());
       BUILD_TRY(hipGetLastError());
       BUILD_TRY(hipMemsetAsync(err.p, 0, sizeof(uint32_t), stream));
+      // triangle counts of the radix tree decide the first round's treelets; each round
+      // leaves every count current for the next
+      k_range_counts<<<blocks_for(ni), kBlock, 0, stream>>>(ni, range.as<uint32_t>(), ta.count);
+      BUILD_TRY(hipGetLastError());
       for (int r = 0; r < kTreeletRounds; r++) {
-        BUILD_TRY(hipMemsetAsync(arrive.p, 0, sizeof(uint32_t) * ni, stream));
-        k_treelet<<<(nf + 63) / 64, 64, 0, stream>>>(n, ta, arrive.as<uint32_t>(), static_cast<uint32_t>(kTreeletLeaves),
-                                                         prm.sah_cost_node, prm.sah_cost_tri, err.as<uint32_t>());
-        BUILD_TRY(hipGetLastError());
+        // the depths change with every round's restructuring
+        BUILD_TRY(make_levels(n, pint.as<int>(), ta.count, static_cast<uint32_t>(kTreeletLeaves), lb, err.as<uint32_t>(),
+                              stream, levels));
+        for (const LevelClass& c : levels) {
+          if (c.treelet)
+            k_treelet_nodes<<<c.hi - c.lo, 64, 0, stream>>>(lb.order.as<uint32_t>(), c.lo, ta, prm.sah_cost_node,
+                                                            prm.sah_cost_tri);
+          else
+            k_refresh_nodes<<<blocks_for(c.hi - c.lo), kBlock, 0, stream>>>(lb.order.as<uint32_t>(), c.lo, c.hi, ta,
+                                                                             prm.sah_cost_node, prm.sah_cost_tri);
+          BUILD_TRY(hipGetLastError());
+        }
       }
       {
         DevBuf refs;
@@ -1006,13 +1089,16 @@ static int map_build(const float* vertices, uint32_t nv, const uint32_t* faces, 
       svals = vals3.as<uint32_t>();
       std::swap(pleaf.p, pleaf2.p);
     }
-    BUILD_TRY(hipMemsetAsync(arrive.p, 0, sizeof(uint32_t) * ni, stream));
-    k_bottom_up<<<blocks_for(nf), kBlock, 0, stream>>>(n, svals, flo.as<float4>(), fhi.as<float4>(), child.as<uint32_t>(),
-                                                       pint.as<int>(), pleaf.as<int>(), nlo.as<float4>(), nhi.as<float4>(),
-                                                       arrive.as<uint32_t>(), coll.as<uint8_t>(), subtree.as<uint32_t>(),
-                                                       prm.max_leaf_size,
-                                                       prm.sah_cost_node, prm.sah_cost_tri);
-    BUILD_TRY(hipGetLastError());
+    BUILD_TRY(make_levels(n, pint.as<int>(), nullptr, 0u, lb, err.as<uint32_t>(), stream, levels));
+    for (const LevelClass& c : levels) {
+      const uint32_t cnt = c.hi - c.lo;
+      k_bottom_up_level<<<blocks_for(cnt), kBlock, 0, stream>>>(lb.order.as<uint32_t>(), c.lo, c.hi, svals,
+                                                                flo.as<float4>(), fhi.as<float4>(), child.as<uint32_t>(),
+                                                                nlo.as<float4>(), nhi.as<float4>(), coll.as<uint8_t>(),
+                                                                subtree.as<uint32_t>(), prm.max_leaf_size,
+                                                                prm.sah_cost_node, prm.sah_cost_tri);
+      BUILD_TRY(hipGetLastError());
+    }
     k_layout<<<blocks_for(ni), kBlock, 0, stream>>>(n, svals, flo.as<float4>(), fhi.as<float4>(), child.as<uint32_t>(),
                                                     range.as<uint32_t>(), pint.as<int>(), nlo.as<float4>(),
                                                     nhi.as<float4>(), coll.as<uint8_t>(), m->nodes, stats.as<uint32_t>(),
@@ -1140,11 +1226,6 @@ __global__ __launch_bounds__(kBlock) void k_refit_tris(uint32_t nf, const float*
   normals[k] = nrm;
 }
 
-__global__ __launch_bounds__(kBlock) void k_fill_u32(uint32_t* p, uint32_t n, uint32_t v) {
-  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-  if (i < n) p[i] = v;
-}
-
 __device__ inline void set_child_box(Node2& nd, int c, float4 lo, float4 hi) {
   if (c == 0) {
     nd.a.x = pad_lo(lo.x); nd.a.y = pad_hi(hi.x); nd.a.z = pad_lo(lo.y); nd.a.w = pad_hi(hi.y);
@@ -1155,54 +1236,88 @@ __device__ inline void set_child_box(Node2& nd, int c, float4 lo, float4 hi) {
   }
 }
 
-// Bottom-up box update of the final tree: leaf slots are boxed from their triangles; the
-// second arrival at a node unions its two (already padded) child boxes, pads once more
-// and moves up.  Boxes stay conservative at every level.
-__global__ __launch_bounds__(kBlock) void k_refit_nodes(uint32_t ni, const uint8_t* __restrict__ reach,
-                                                        const int* __restrict__ parent, const float4* __restrict__ tris,
-                                                        Node2* nodes, uint32_t* arrive) {
+// Depth of every interior node of the final tree (0 at the root; unreachable slots get
+// UINT32_MAX).  The parent chain of a valid tree ends at the root.
+__global__ __launch_bounds__(kBlock) void k_node_depth(uint32_t ni, const uint8_t* __restrict__ reach,
+                                                       const int* __restrict__ parent, uint32_t* __restrict__ depth) {
   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-  if (i >= ni || !reach[i]) return;
+  if (i >= ni) return;
+  if (!reach[i]) { depth[i] = 0xFFFFFFFFu; return; }
+  uint32_t d = 0;
+  for (int p = parent[i]; p >= 0; p = parent[p]) d++;
+  depth[i] = d;
+}
+
+// One depth of the level-by-level refit: every node of the level boxes its two children,
+// leaf slots from their triangles, interior slots as the union of the child's own two
+// (already padded) boxes, refitted by the previous launch.  Each node is written by one
+// thread and read only by later launches, so no atomics or fences are needed (agent-scope
+// release and acquire write back and invalidate the L2 of an XCD on every such atomic,
+// which made the bottom-up pass with arrival counters 4 ms on the terrain).
+__global__ __launch_bounds__(kBlock) void k_refit_level(const uint32_t* __restrict__ order, uint32_t lo, uint32_t hi,
+                                                        const float4* __restrict__ tris, Node2* nodes) {
+  const uint32_t k = lo + blockIdx.x * kBlock + threadIdx.x;
+  if (k >= hi) return;
+  const uint32_t i = order[k];
   Node2 nd = nodes[i];
-  const int refs[2] = {nd.d.x, nd.d.y};
-  uint32_t leaves = 0;
-  // only the words of leaf slots are written here: an interior slot belongs to the thread
-  // that finishes that child, which may already have stored its box
   for (int c = 0; c < 2; c++) {
-    if (refs[c] >= 0) continue;
-    const uint32_t e = ~static_cast<uint32_t>(refs[c]);
-    const uint32_t first = e >> 3, cnt = (e & 7u) + 1u;
-    float4 lo = make_float4(INFINITY, INFINITY, INFINITY, 0.0f), hi = make_float4(-INFINITY, -INFINITY, -INFINITY, 0.0f);
-    for (uint32_t k = first; k < first + cnt; k++)
-      for (int v = 0; v < 3; v++) {
-        const float4 p = tris[3 * static_cast<size_t>(k) + v];
-        lo.x = fminf(lo.x, p.x); lo.y = fminf(lo.y, p.y); lo.z = fminf(lo.z, p.z);
-        hi.x = fmaxf(hi.x, p.x); hi.y = fmaxf(hi.y, p.y); hi.z = fmaxf(hi.z, p.z);
-      }
-    set_child_box(nd, c, lo, hi);
-    if (c == 0) { nodes[i].a = nd.a; nodes[i].c.x = nd.c.x; nodes[i].c.y = nd.c.y; }
-    else { nodes[i].b = nd.b; nodes[i].c.z = nd.c.z; nodes[i].c.w = nd.c.w; }
-    leaves++;
+    const int ref = c ? nd.d.y : nd.d.x;
+    float4 blo, bhi;
+    if (ref < 0) {
+      const uint32_t e = ~static_cast<uint32_t>(ref);
+      const uint32_t first = e >> 3, cnt = (e & 7u) + 1u;
+      blo = make_float4(INFINITY, INFINITY, INFINITY, 0.0f);
+      bhi = make_float4(-INFINITY, -INFINITY, -INFINITY, 0.0f);
+      for (uint32_t t = first; t < first + cnt; t++)
+        for (int v = 0; v < 3; v++) {
+          const float4 p = tris[3 * static_cast<size_t>(t) + v];
+          blo.x = fminf(blo.x, p.x); blo.y = fminf(blo.y, p.y); blo.z = fminf(blo.z, p.z);
+          bhi.x = fmaxf(bhi.x, p.x); bhi.y = fmaxf(bhi.y, p.y); bhi.z = fmaxf(bhi.z, p.z);
+        }
+    } else {
+      const Node2 ch = nodes[ref];
+      blo = make_float4(fminf(ch.a.x, ch.b.x), fminf(ch.a.z, ch.b.z), fminf(ch.c.x, ch.c.z), 0.0f);
+      bhi = make_float4(fmaxf(ch.a.y, ch.b.y), fmaxf(ch.a.w, ch.b.w), fmaxf(ch.c.y, ch.c.w), 0.0f);
+    }
+    set_child_box(nd, c, blo, bhi);
   }
-  if (leaves == 0) return;  // both children interior: their threads finish this node
-  uint32_t old = __hip_atomic_fetch_add(&arrive[i], leaves, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-  if (old + leaves != 2u) return;
-  int j = static_cast<int>(i);
-  for (;;) {
-    const Node2 cur = nodes[j];
-    const float4 lo = make_float4(fminf(cur.a.x, cur.b.x), fminf(cur.a.z, cur.b.z), fminf(cur.c.x, cur.c.z), 0.0f);
-    const float4 hi = make_float4(fmaxf(cur.a.y, cur.b.y), fmaxf(cur.a.w, cur.b.w), fmaxf(cur.c.y, cur.c.w), 0.0f);
-    if (j == 0) return;  // the root
-    const int p = parent[j];
-    Node2 pn = nodes[p];
-    const int c = pn.d.x == j ? 0 : 1;
-    set_child_box(pn, c, lo, hi);
-    if (c == 0) { nodes[p].a = pn.a; nodes[p].c.x = pn.c.x; nodes[p].c.y = pn.c.y; }
-    else { nodes[p].b = pn.b; nodes[p].c.z = pn.c.z; nodes[p].c.w = pn.c.w; }
-    old = __hip_atomic_fetch_add(&arrive[p], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    if (old != 1u) return;  // first arrival: the sibling's thread continues
-    j = p;
-  }
+  nodes[i].a = nd.a;
+  nodes[i].b = nd.b;
+  nodes[i].c = nd.c;
+}
+
+// The refit schedule of a map (once: the tree's topology never changes after the build).
+hipError_t make_refit_schedule(rmh_map* m, hipStream_t stream) {
+  const uint32_t ni = m->n_internal;
+  uint32_t* depth = nullptr;
+  hipError_t e = hipMalloc(&depth, sizeof(uint32_t) * ni);
+  if (e != hipSuccess) return e;
+  k_node_depth<<<blocks_for(ni), kBlock, 0, stream>>>(ni, m->reach, m->parent, depth);
+  std::vector<uint32_t> hd(ni);
+  e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(hd.data(), depth, sizeof(uint32_t) * ni, hipMemcpyDeviceToHost, stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(stream);
+  (void)hipFree(depth);
+  if (e != hipSuccess) return e;
+  uint32_t max_d = 0, n = 0;
+  for (uint32_t d : hd)
+    if (d != 0xFFFFFFFFu) { max_d = std::max(max_d, d); n++; }
+  // counting sort, deepest level first
+  std::vector<uint32_t> start(max_d + 2, 0);
+  for (uint32_t d : hd)
+    if (d != 0xFFFFFFFFu) start[max_d - d + 1]++;
+  for (uint32_t l = 1; l < start.size(); l++) start[l] += start[l - 1];
+  std::vector<uint32_t> order(std::max<uint32_t>(1, n));
+  std::vector<uint32_t> fill(start.begin(), start.end() - 1);
+  for (uint32_t i = 0; i < ni; i++)
+    if (hd[i] != 0xFFFFFFFFu) order[fill[max_d - hd[i]]++] = i;
+  uint32_t* dev = nullptr;
+  e = hipMalloc(&dev, sizeof(uint32_t) * order.size());
+  if (e == hipSuccess) e = hipMemcpy(dev, order.data(), sizeof(uint32_t) * order.size(), hipMemcpyHostToDevice);
+  if (e != hipSuccess) { if (dev) (void)hipFree(dev); return e; }
+  m->refit_order = dev;
+  m->refit_levels = std::move(start);
+  return hipSuccess;
 }
 
 extern "C" int rmh_map_refit(rmh_map* map, const float* vertices, void* stream_) {
@@ -1216,14 +1331,20 @@ extern "C" int rmh_map_refit(rmh_map* map, const float* vertices, void* stream_)
                                                                  map->normals);
   RMH_HIP_TRY(hipGetLastError());
   if (map->root >= 0 && map->n_internal > 0) {
-    uint32_t* arrive = nullptr;
-    RMH_HIP_TRY(hipMalloc(&arrive, sizeof(uint32_t) * map->n_internal));
-    k_fill_u32<<<blocks_for(map->n_internal), kBlock, 0, stream>>>(arrive, map->n_internal, 0u);
-    k_refit_nodes<<<blocks_for(map->n_internal), kBlock, 0, stream>>>(map->n_internal, map->reach, map->parent,
-                                                                      map->tris, map->nodes, arrive);
-    hipError_t e = hipGetLastError();
+    if (!map->refit_order) {
+      const hipError_t e = make_refit_schedule(map, stream);
+      if (e != hipSuccess) return hip_fail(e, "rmh_map_refit: schedule");
+    }
+    // one launch per depth, deepest first: a level reads only what earlier launches wrote
+    const std::vector<uint32_t>& lv = map->refit_levels;
+    hipError_t e = hipSuccess;
+    for (size_t l = 0; l + 1 < lv.size() && e == hipSuccess; l++) {
+      if (lv[l + 1] == lv[l]) continue;
+      k_refit_level<<<blocks_for(lv[l + 1] - lv[l]), kBlock, 0, stream>>>(map->refit_order, lv[l], lv[l + 1],
+                                                                           map->tris, map->nodes);
+      e = hipGetLastError();
+    }
     if (e == hipSuccess) e = hipStreamSynchronize(stream);
-    (void)hipFree(arrive);
     if (e != hipSuccess) return hip_fail(e, "rmh_map_refit");
     // the 4-wide tree is collapsed again from the refitted binary boxes
     if (map->nodes4) {

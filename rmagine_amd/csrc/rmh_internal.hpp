@@ -156,6 +156,11 @@ struct rmh_map {
   rmh::Node4* nodes4 = nullptr;
   int root4 = 0;
   uint32_t wide_capacity = 0;          // allocated Node4 slots
+  // refit schedule, made at the first refit: the interior nodes of the final tree ordered by
+  // depth, deepest first, and where each depth starts (refit_levels[d] .. refit_levels[d+1]
+  // in reverse depth order)
+  uint32_t* refit_order = nullptr;
+  std::vector<uint32_t> refit_levels;
 };
 
 struct rmh_sensor {
