@@ -475,9 +475,9 @@ def extra_next_rows(do_cpu: bool, cpu_seconds: float = 3.0) -> dict:
       autotune.tune_sensor picked its ray order and walk;
     * closest-point queries (rmh_closest_points): 4,194,304 points within 2 m of the surface,
       unbounded search: 12 B read + 40 B written per query;
-    * BVH refit (rmh_map_refit) after moving every vertex, synchronous (its one small
-      allocation included): per triangle 48 B of triangle and 16 B of normal written, 36 B of
-      vertices read; per node 64 B written;
+    * BVH refit (rmh_map_refit) after moving every vertex, synchronous, one launch per tree
+      depth (the schedule made at the first, untimed refit): per triangle 48 B of triangle and
+      16 B of normal written, 36 B of vertices read; per node 64 B written;
     * map file loading on the host (rmh_scene_load): the terrain as binary PLY.
     With do_cpu, the oracle checks samples of the GPU results (OnDn poses, closest-point
     queries, noise prefixes) and the OnDn poses it casts give its throughput."""
@@ -611,7 +611,7 @@ def extra_next_rows(do_cpu: bool, cpu_seconds: float = 3.0) -> dict:
         it[0] += 1
     ms = _timed_ms(refit, reps=6)  # ends on the moved vertices
     alg = (48 + 16 + 36) * len(F) + 64 * info["n_nodes"]
-    refit_out = {"ms": round(ms, 3), "roofline": roofline("k_refit_tris + k_refit_nodes (rmh_map_refit)", alg, ms,
+    refit_out = {"ms": round(ms, 3), "roofline": roofline("k_refit_tris + k_refit_level per depth (rmh_map_refit)", alg, ms,
                                                           note="algorithmic bytes: triangles, normals and vertices of "
                                                                "every face, 64 B per node")}
     # the refitted map casts exactly what a map built from the moved vertices casts
