@@ -6,14 +6,16 @@
 //
 // One lane per query.  Depth-first BVH walk, nearer child first; a subtree is skipped when
 // the squared distance from the query to its (outward padded) box exceeds the best squared
-// distance so far.  Box distances and the triangle's closest point (Ericson, Real-Time
+// distance so far.  The walk's stack is the ray kernels' (rmh_trace.hpp): the top rows in
+// LDS, deeper entries in private memory; held in registers, the 32 entries and their
+// distances took the kernel to 190 VGPRs (two waves per SIMD).  Box distances and the triangle's closest point (Ericson, Real-Time
 // Collision Detection, 5.1.5: Voronoi regions of the vertices, edges and face) are
 // evaluated in FP64, so the result is the FP64 nearest triangle; equal distances go to the
 // lower global face index.  Outputs per query (each pointer may be NULL): distance
 // (FLT_MAX when nothing lies within max_distance), closest point, unit face normal (not
 // flipped), face id within its mesh, geom id, object id (0xFFFFFFFF for none; point and
 // normal NaN).
-#include "rmh_internal.hpp"
+#include "rmh_trace.hpp"
 
 #include <cfloat>
 
@@ -67,7 +69,8 @@ struct CPArgs {
 };
 
 template <int STACK>
-__global__ __launch_bounds__(64) void k_closest(const CPArgs a) {
+__global__ __launch_bounds__(64, 4) void k_closest(const CPArgs a) {
+  __shared__ int2 stk[kLdsStack][64];
   const uint32_t i = blockIdx.x * 64 + threadIdx.x;
   if (i >= a.n) return;
   const rmh_vec3 qv = a.q[i];
@@ -75,12 +78,11 @@ __global__ __launch_bounds__(64) void k_closest(const CPArgs a) {
   double best = a.best0;
   uint32_t best_slot = 0xFFFFFFFFu, best_face = 0xFFFFFFFFu;
   D3 best_p = d3(0, 0, 0);
-  int stk[STACK];
-  float stk_d[STACK];
+  volatile int spill_mem[STACK > kLdsStack ? 2 * (STACK - kLdsStack) : 2];
+  PrivInt* spill = (PrivInt*)spill_mem;
   int sp = 0;
-  int ref = a.map.n_faces ? a.map.root : 0x7fffffff;
-  if (a.map.n_faces == 0) sp = -1;
-  while (sp >= 0) {
+  int ref = a.map.n_faces ? a.map.root : kDone;
+  while (ref != kDone) {
     if (ref >= 0) {
       const Node2 nd = a.map.nodes[ref];
       const double b0 = box_d2(q, nd.a.x, nd.a.y, nd.a.z, nd.a.w, nd.c.x, nd.c.y);
@@ -88,11 +90,10 @@ __global__ __launch_bounds__(64) void k_closest(const CPArgs a) {
       const bool h0 = b0 <= best, h1 = b1 <= best;
       if (h0 && h1) {
         const bool first0 = b0 <= b1;
-        if (sp < STACK) {  // the host picks STACK >= tree depth
-          stk[sp] = first0 ? nd.d.y : nd.d.x;
-          stk_d[sp] = __double2float_rd(first0 ? b1 : b0);
-          sp++;
-        }
+        // one entry per level: the host picks STACK >= tree depth; the entry's distance is
+        // rounded down, so a pop never skips a box that could still hold the nearest point
+        push_ref<STACK, 64, kLdsStack, true>(stk, spill, sp, first0 ? nd.d.y : nd.d.x,
+                                             __double2float_rd(first0 ? b1 : b0));
         ref = first0 ? nd.d.x : nd.d.y;
         continue;
       }
@@ -116,13 +117,10 @@ __global__ __launch_bounds__(64) void k_closest(const CPArgs a) {
         }
       }
     }
-    // pop the next subtree that can still hold something nearer
-    ref = 0x7fffffff;
-    while (sp > 0) {
-      sp--;
-      if (static_cast<double>(stk_d[sp]) <= best) { ref = stk[sp]; break; }
-    }
-    if (ref == 0x7fffffff) break;
+    // pop the next subtree that can still hold something nearer: the bound rounded up to
+    // FP32 pops every entry with distance <= best (and at most a few more, whose children
+    // are tested again)
+    ref = pop_ref<STACK, 64, kLdsStack, true>(stk, spill, sp, __double2float_ru(best));
   }
   const rmh_closest_outputs& o = a.out;
   if (best_slot != 0xFFFFFFFFu) {
