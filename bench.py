@@ -264,7 +264,7 @@ def extra_batched(cpu_seconds: float, do_cpu: bool, tune: bool = True) -> dict:
            "per_sensor_launches_ms": round(plain_ms, 3), "per_sensor_mrays_s": round(rays / plain_ms / 1e3, 1),
            "tuned": {**{k: tuned[k] for k in ("chunk_rays", "waves_per_cu", "tile_rows", "lds_stack")},
                      "lds_stack_ms": {str(k): round(v, 3) for k, v in tuned["lds_stack_ms"].items()},
-                     "ray_order": "sorted" if tuned["rays_sorted"] else "list", "ray_order_ms": tuned["ray_order_ms"],
+                     "ray_order": tuned["ray_order"], "ray_order_ms": tuned["ray_order_ms"],
                      "layout": "4-wide" if tuned.get("wide") else "binary", "layout_ms": tuned.get("layout_ms")},
            "batched_vs_per_sensor_mismatches": mism}
     if do_cpu:
@@ -542,7 +542,7 @@ def extra_next_rows(do_cpu: bool, cpu_seconds: float = 3.0) -> dict:
     ms = _timed_ms(lambda: sim.simulate(Tbm_d, res))
     ondn = {"rays": rays, "ms": round(ms, 3), "mrays_s": round(rays / ms / 1e3, 1),
             "walk": "speculative" if tuned["speculative"] else "classic",
-            "ray_order": "sorted" if tuned["rays_sorted"] else "list",
+            "ray_order": tuned["ray_order"],
             "untuned_ms": round(ms_default, 3), "tuner_trials_ms": tuned["trials"],
             "roofline": roofline("k_simulate<OnDn> (traversal + attribute writers)", (BYTES_PER_RAY_ALL + 24) * rays, ms,
                                  note="algorithmic bytes: 41 B of outputs written, 24 B of direction and origin read per ray")}

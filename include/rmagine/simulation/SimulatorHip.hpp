@@ -142,9 +142,14 @@ class SimulatorHip {
 
   // O1Dn/OnDn: cast the rays in direction-sorted order (the default) or in list order; the
   // results are identical, only the speed differs (image models ignore it)
-  void setRayOrder(bool sorted) {
+  void setRayOrder(bool sorted) { setRayOrder(sorted ? RayOrder::Sorted : RayOrder::List); }
+
+  // O1Dn/OnDn traversal order: list, direction-sorted (the default) or the model's W x H
+  // grid in 4 x 16 tiles (as image sensors are cast); the results are identical
+  enum class RayOrder { List = RMH_RAY_ORDER_LIST, Sorted = RMH_RAY_ORDER_SORTED, Tiles = RMH_RAY_ORDER_TILES };
+  void setRayOrder(RayOrder order) {
     if (!m_sensor) throw std::runtime_error("[SimulatorHip] setRayOrder(): no sensor model set");
-    RM_RMH_CHECK(rmh_sensor_set_ray_order(m_sensor.get(), sorted ? 1 : 0));
+    RM_RMH_CHECK(rmh_sensor_set_ray_order(m_sensor.get(), static_cast<int>(order)));
   }
 
   // O1Dn/OnDn: walk the BVH speculatively (the default, fastest for scattered rays) or the

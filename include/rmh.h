@@ -220,14 +220,19 @@ int rmh_ouster_o1dn(const char* path, uint32_t* width, uint32_t* height, float* 
 int rmh_sensor_create(int kind, const void* model, rmh_sensor** out);
 int rmh_sensor_destroy(rmh_sensor* sensor);
 /* Traversal order of an O1Dn/OnDn sensor's rays (no counterpart in the reference, which
- * casts them in list order): sorted != 0 = by direction (the default: a wave's 64 rays
- * cover one patch of the sphere), 0 = list order (rays already ordered by the caller, e.g.
- * a scanner's row order).  Results are identical either way; a tuning choice of the
- * autotuner.  Rewrites the sensor's device arrays in place (batches and graphs that use
- * the sensor stay valid); synchronous; no launch using the sensor may be in flight.
- * Image sensors (spherical, pinhole) ignore it and report 0. */
-int rmh_sensor_set_ray_order(rmh_sensor* sensor, int sorted);
-int rmh_sensor_get_ray_order(const rmh_sensor* sensor, int* sorted);
+ * casts them in list order): RMH_RAY_ORDER_SORTED = by direction (the default: a wave's 64
+ * rays cover one patch of the sphere), RMH_RAY_ORDER_LIST = list order (rays already
+ * ordered by the caller), RMH_RAY_ORDER_TILES = the model's W x H grid (buffer index
+ * row * W + column) in 4 x 16 tiles, as image sensors are cast (for a scanner's ray grid).
+ * Results are identical in every order; a tuning choice of the autotuner.  Rewrites the
+ * sensor's device arrays in place (batches and graphs that use the sensor stay valid);
+ * synchronous; no launch using the sensor may be in flight.  Other values are refused with
+ * RMH_ERR_INVALID.  Image sensors (spherical, pinhole) ignore it and report the list order. */
+#define RMH_RAY_ORDER_LIST   0
+#define RMH_RAY_ORDER_SORTED 1
+#define RMH_RAY_ORDER_TILES  2
+int rmh_sensor_set_ray_order(rmh_sensor* sensor, int order);
+int rmh_sensor_get_ray_order(const rmh_sensor* sensor, int* order);
 /* Traversal walk of an O1Dn/OnDn sensor's rays in rmh_simulate and
  * rmh_simulate_visit_stats (no counterpart in the reference, whose ray casts are one
  * library call per ray): speculative != 0 = a lane that reaches a leaf holds it while the

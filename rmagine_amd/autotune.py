@@ -20,9 +20,9 @@ never results.  Results are cached as JSON keyed by device architecture, mesh
 size (power of two bucket of the face count) and sensor mix, in $RMH_TUNE_CACHE or
 ~/.cache/rmagine_amd/autotune.json.
 
-`tune_sensor` does the same for one O1Dn/OnDn simulator: it times the ray order (sorted or
-list) and the walk (speculative or classic) of its sensor on the caller's poses and keeps
-the fastest pair (a scanner's coherent ray grid walks fastest the classic way, often in its
+`tune_sensor` does the same for one O1Dn/OnDn simulator: it times the ray order (sorted,
+list, or the model's grid in image tiles) and the walk (speculative or classic) of its
+sensor on the caller's poses and keeps the fastest pair (a scanner's coherent ray grid walks fastest the classic way, often in its
 own order; scattered rays such as ICP scans speculatively and sorted).
 
 The reference has no counterpart (its launch shape is fixed by optixLaunch).
@@ -186,18 +186,20 @@ def tune_batch(batch, key: str | None = None, chunks=DEFAULT_CHUNKS, waves=DEFAU
 
 def tune_sensor(sim, Tbm, bundle=None, reps: int = 3, stream=None) -> dict:
     """Times every (ray order, walk) pair of an O1Dn/OnDn simulator's sensor on `Tbm` with
-    HIP events (median of `reps` launches after one warm-up), applies the fastest and
-    returns {"rays_sorted", "speculative", "ms", "trials"}.  Image simulators have one choice
-    and return it untimed.  Results never change; no launch of the sensor may be in flight."""
+    HIP events (median of `reps` launches after one warm-up): orders "sorted", "list" and
+    "tiles" (the model's W x H grid in 4 x 16 tiles), walks speculative and classic.  Applies
+    the fastest and returns {"ray_order", "rays_sorted", "speculative", "ms", "trials"}.  Image
+    simulators have one choice and return it untimed.  Results never change; no launch of the
+    sensor may be in flight."""
     from . import types as T
     if not isinstance(sim.model, (T.O1DnModel, T.OnDnModel)):
-        return {"rays_sorted": False, "speculative": False, "ms": None, "trials": {}}
+        return {"ray_order": "list", "rays_sorted": False, "speculative": False, "ms": None, "trials": {}}
     stream = stream or torch.cuda.current_stream()
     if bundle is None:
         bundle = sim.simulate(Tbm, stream=stream)
     trials = {}
-    for sorted_ in (True, False):
-        sim.set_ray_order(sorted_)
+    for order in ("sorted", "list", "tiles"):
+        sim.set_ray_order(order)
         for spec in (True, False):
             sim.set_walk(spec)
             sim.simulate(Tbm, bundle, stream=stream)
@@ -210,10 +212,9 @@ def tune_sensor(sim, Tbm, bundle=None, reps: int = 3, stream=None) -> dict:
                 b.synchronize()
                 ms.append(a.elapsed_time(b))
             ms.sort()
-            trials[(sorted_, spec)] = ms[len(ms) // 2]
-    (best_sorted, best_spec), best = min(trials.items(), key=lambda kv: kv[1])
-    sim.set_ray_order(best_sorted)
+            trials[(order, spec)] = ms[len(ms) // 2]
+    (best_order, best_spec), best = min(trials.items(), key=lambda kv: kv[1])
+    sim.set_ray_order(best_order)
     sim.set_walk(best_spec)
-    return {"rays_sorted": best_sorted, "speculative": best_spec, "ms": best,
-            "trials": {f"{'sorted' if k[0] else 'list'}/{'speculative' if k[1] else 'classic'}": round(v, 3)
-                       for k, v in trials.items()}}
+    return {"ray_order": best_order, "rays_sorted": best_order == "sorted", "speculative": best_spec, "ms": best,
+            "trials": {f"{k[0]}/{'speculative' if k[1] else 'classic'}": round(v, 3) for k, v in trials.items()}}

@@ -8,7 +8,10 @@
 // the squared distance from the query to its (outward padded) box exceeds the best squared
 // distance so far.  The walk's stack is the ray kernels' (rmh_trace.hpp): the top rows in
 // LDS, deeper entries in private memory; held in registers, the 32 entries and their
-// distances took the kernel to 190 VGPRs (two waves per SIMD).  Box distances and the triangle's closest point (Ericson, Real-Time
+// distances took the kernel to 190 VGPRs (two waves per SIMD).  An unbounded search pushes
+// a subtree at almost every level of its first descent, so the stack is deeper than a ray's:
+// 16 LDS rows (8 KB per wave; the kernel's 90 VGPRs allow five waves per SIMD, which the
+// 160 KB of LDS of a CU then just holds).  Box distances and the triangle's closest point (Ericson, Real-Time
 // Collision Detection, 5.1.5: Voronoi regions of the vertices, edges and face) are
 // evaluated in FP64, so the result is the FP64 nearest triangle; equal distances go to the
 // lower global face index.  Outputs per query (each pointer may be NULL): distance
@@ -68,9 +71,11 @@ struct CPArgs {
   rmh_closest_outputs out;
 };
 
+constexpr int kClosestLds = 16;
+
 template <int STACK>
 __global__ __launch_bounds__(64, 4) void k_closest(const CPArgs a) {
-  __shared__ int2 stk[kLdsStack][64];
+  __shared__ int2 stk[kClosestLds][64];
   const uint32_t i = blockIdx.x * 64 + threadIdx.x;
   if (i >= a.n) return;
   const rmh_vec3 qv = a.q[i];
@@ -78,7 +83,7 @@ __global__ __launch_bounds__(64, 4) void k_closest(const CPArgs a) {
   double best = a.best0;
   uint32_t best_slot = 0xFFFFFFFFu, best_face = 0xFFFFFFFFu;
   D3 best_p = d3(0, 0, 0);
-  volatile int spill_mem[STACK > kLdsStack ? 2 * (STACK - kLdsStack) : 2];
+  volatile int spill_mem[STACK > kClosestLds ? 2 * (STACK - kClosestLds) : 2];
   PrivInt* spill = (PrivInt*)spill_mem;
   int sp = 0;
   int ref = a.map.n_faces ? a.map.root : kDone;
@@ -92,7 +97,7 @@ __global__ __launch_bounds__(64, 4) void k_closest(const CPArgs a) {
         const bool first0 = b0 <= b1;
         // one entry per level: the host picks STACK >= tree depth; the entry's distance is
         // rounded down, so a pop never skips a box that could still hold the nearest point
-        push_ref<STACK, 64, kLdsStack, true>(stk, spill, sp, first0 ? nd.d.y : nd.d.x,
+        push_ref<STACK, 64, kClosestLds, true>(stk, spill, sp, first0 ? nd.d.y : nd.d.x,
                                              __double2float_rd(first0 ? b1 : b0));
         ref = first0 ? nd.d.x : nd.d.y;
         continue;
@@ -120,7 +125,7 @@ __global__ __launch_bounds__(64, 4) void k_closest(const CPArgs a) {
     // pop the next subtree that can still hold something nearer: the bound rounded up to
     // FP32 pops every entry with distance <= best (and at most a few more, whose children
     // are tested again)
-    ref = pop_ref<STACK, 64, kLdsStack, true>(stk, spill, sp, __double2float_ru(best));
+    ref = pop_ref<STACK, 64, kClosestLds, true>(stk, spill, sp, __double2float_ru(best));
   }
   const rmh_closest_outputs& o = a.out;
   if (best_slot != 0xFFFFFFFFu) {

@@ -181,7 +181,7 @@ struct rmh_sensor {
   rmh_vec3* dirs_list = nullptr;
   rmh_vec3* origs_list = nullptr;
   uint32_t* perm = nullptr;
-  int sorted = 1;
+  int sorted = 1;                 // traversal order: RMH_RAY_ORDER_LIST / _SORTED / _TILES
   int speculative = 1;            // O1Dn / OnDn walk in rmh_simulate: speculative or classic
   int device = 0;
 };

@@ -244,34 +244,51 @@ static hipError_t upload_sorted_rays(rmh_sensor* s, const rmh_vec3* dirs_in, con
   return e;
 }
 
+// Buffer indices of the W x H ray grid (row-major) in the order of 4 x 16 tiles, row of
+// tiles by row of tiles, each tile row by row; positions outside the grid are skipped, so
+// every index appears once and, when 4 divides H and 16 divides W, each run of 64 slots is
+// one tile.
+static std::vector<uint32_t> grid_tile_order(uint32_t W, uint32_t H) {
+  constexpr uint32_t th = 4, tw = 16;
+  std::vector<uint32_t> o;
+  o.reserve(static_cast<size_t>(W) * H);
+  for (uint32_t ty = 0; ty < H; ty += th)
+    for (uint32_t tx = 0; tx < W; tx += tw)
+      for (uint32_t r = ty; r < std::min(H, ty + th); r++)
+        for (uint32_t c = tx; c < std::min(W, tx + tw); c++) o.push_back(r * W + c);
+  return o;
+}
+
 // Rewrites the traversal arrays in place (same buffers, so batch tables and captured
-// graphs that point at them stay valid): sorted = the direction order, else list order.
-int rmh_sensor_set_ray_order(rmh_sensor* s, int sorted) {
+// graphs that point at them stay valid): list order, direction order, or the grid's tiles.
+int rmh_sensor_set_ray_order(rmh_sensor* s, int order) {
   if (!s) return fail(RMH_ERR_INVALID, "rmh_sensor_set_ray_order: NULL sensor");
+  if (order != RMH_RAY_ORDER_LIST && order != RMH_RAY_ORDER_SORTED && order != RMH_RAY_ORDER_TILES)
+    return fail(RMH_ERR_INVALID, "rmh_sensor_set_ray_order: order must be 0 (list), 1 (sorted) or 2 (tiles)");
   if (s->kind != RMH_MODEL_O1DN && s->kind != RMH_MODEL_ONDN) return RMH_OK;  // images: no list
-  sorted = sorted ? 1 : 0;
-  if (sorted == s->sorted) return RMH_OK;
+  if (order == s->sorted) return RMH_OK;
   const uint32_t n = s->width * s->height;
   const uint32_t blocks = (n + 255) / 256;
   RMH_HIP_TRY(hipDeviceSynchronize());  // no launch may read the arrays meanwhile
-  if (sorted) {
+  if (order == RMH_RAY_ORDER_SORTED) {
     RMH_HIP_TRY(hipMemcpy(s->order, s->perm, n * sizeof(uint32_t), hipMemcpyDeviceToDevice));
-    k_gather_rays<<<blocks, 256>>>(s->order, n, s->dirs_list, s->origs_list, s->dirs, s->origs);
-    RMH_HIP_TRY(hipGetLastError());
-  } else {
+  } else if (order == RMH_RAY_ORDER_LIST) {
     k_iota<<<blocks, 256>>>(s->order, n);
     RMH_HIP_TRY(hipGetLastError());
-    RMH_HIP_TRY(hipMemcpy(s->dirs, s->dirs_list, n * sizeof(rmh_vec3), hipMemcpyDeviceToDevice));
-    if (s->origs) RMH_HIP_TRY(hipMemcpy(s->origs, s->origs_list, n * sizeof(rmh_vec3), hipMemcpyDeviceToDevice));
+  } else {
+    const std::vector<uint32_t> t = grid_tile_order(s->width, s->height);
+    RMH_HIP_TRY(hipMemcpy(s->order, t.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice));
   }
+  k_gather_rays<<<blocks, 256>>>(s->order, n, s->dirs_list, s->origs_list, s->dirs, s->origs);
+  RMH_HIP_TRY(hipGetLastError());
   RMH_HIP_TRY(hipDeviceSynchronize());
-  s->sorted = sorted;
+  s->sorted = order;
   return RMH_OK;
 }
 
-int rmh_sensor_get_ray_order(const rmh_sensor* s, int* sorted) {
-  if (!s || !sorted) return fail(RMH_ERR_INVALID, "rmh_sensor_get_ray_order: NULL argument");
-  *sorted = (s->kind == RMH_MODEL_O1DN || s->kind == RMH_MODEL_ONDN) ? s->sorted : 0;
+int rmh_sensor_get_ray_order(const rmh_sensor* s, int* order) {
+  if (!s || !order) return fail(RMH_ERR_INVALID, "rmh_sensor_get_ray_order: NULL argument");
+  *order = (s->kind == RMH_MODEL_O1DN || s->kind == RMH_MODEL_ONDN) ? s->sorted : RMH_RAY_ORDER_LIST;
   return RMH_OK;
 }
 

@@ -224,6 +224,19 @@ def import_hip_map(path, polygons: str = "first", device="cuda", **bvh) -> HipMa
     return m
 
 
+RAY_ORDERS = ("list", "sorted", "tiles")  # rmh.h RMH_RAY_ORDER_LIST / _SORTED / _TILES
+
+
+def _ray_order_code(order) -> int:
+    """True / "sorted": by direction; False / "list": list order; "tiles": the W x H grid in
+    4 x 16 tiles."""
+    if isinstance(order, str):
+        if order not in RAY_ORDERS:
+            raise ValueError(f"ray order must be one of {RAY_ORDERS}")
+        return RAY_ORDERS.index(order)
+    return 1 if order else 0
+
+
 class _SensorHandle:
     def __init__(self, model, device):
         self.model = model
@@ -244,13 +257,16 @@ class _SensorHandle:
         _lib.call("rmh_sensor_create", kind, ctypes.byref(cmodel), ctypes.byref(self.handle))
         self.W, self.H = model.getWidth(), model.getHeight()
 
-    def set_ray_order(self, sorted_: bool) -> None:
-        _lib.call("rmh_sensor_set_ray_order", self.handle, 1 if sorted_ else 0)
+    def set_ray_order(self, order) -> None:
+        _lib.call("rmh_sensor_set_ray_order", self.handle, _ray_order_code(order))
 
-    def ray_order_sorted(self) -> bool:
+    def ray_order(self) -> str:
         v = ctypes.c_int()
         _lib.call("rmh_sensor_get_ray_order", self.handle, ctypes.byref(v))
-        return bool(v.value)
+        return RAY_ORDERS[v.value]
+
+    def ray_order_sorted(self) -> bool:
+        return self.ray_order() == "sorted"
 
     def set_walk(self, speculative: bool) -> None:
         _lib.call("rmh_sensor_set_walk", self.handle, 1 if speculative else 0)
@@ -299,12 +315,17 @@ class SimulatorHip:
     def model(self):
         return None if self._sensor is None else self._sensor.model
 
-    def set_ray_order(self, sorted_: bool = True) -> None:
-        """O1Dn/OnDn: cast the rays in direction-sorted order (default) or in list order.
-        The results are identical; only the speed differs.  Image models ignore it."""
+    def set_ray_order(self, sorted_=True) -> None:
+        """O1Dn/OnDn: cast the rays in direction-sorted order (True or "sorted", the
+        default), in list order (False or "list"), or as the model's W x H grid in 4 x 16
+        tiles ("tiles", as image sensors are cast).  The results are identical; only the
+        speed differs.  Image models ignore it."""
         if self._sensor is None:
             raise RuntimeError("set_ray_order: no sensor model set")
         self._sensor.set_ray_order(sorted_)
+
+    def ray_order(self) -> str:
+        return "list" if self._sensor is None else self._sensor.ray_order()
 
     def ray_order_sorted(self) -> bool:
         return self._sensor is not None and self._sensor.ray_order_sorted()

@@ -174,9 +174,10 @@ def test_ray_order_switch(room):
 
 def test_walk_switch_and_sensor_tuner(room):
     """The walk of O1Dn/OnDn rays (speculative, or the classic while-while walk) changes no
-    output byte in either ray order, on scattered rays and on a scanner-like coherent grid;
-    autotune.tune_sensor applies one of the four pairs and changes nothing either; image
-    models always walk the classic way."""
+    output byte in any ray order (direction-sorted, list, the W x H grid in 4 x 16 tiles), on
+    scattered rays and on a scanner-like coherent grid whose sides 4 and 16 do not divide;
+    autotune.tune_sensor applies one of the six pairs and changes nothing either; image
+    models always walk the classic way; unknown orders are refused."""
     from rmagine_amd import autotune
     V, F, Tbm, hm, om = room
     rng = np.random.default_rng(13)
@@ -184,41 +185,47 @@ def test_walk_switch_and_sensor_tuner(room):
     dirs = rng.normal(size=(n, 3))
     dirs = (dirs / np.linalg.norm(dirs, axis=1, keepdims=True)).astype(np.float32)
     origs = rng.uniform(-0.3, 0.3, size=(n, 3)).astype(np.float32)
-    # 16 x 90 grid of a spinning scanner, row by row, beams starting off the centre
-    ph, th = np.meshgrid(np.radians(np.linspace(-15.0, 15.0, 16)), np.radians(np.arange(90) * 4.0), indexing="ij")
+    # 15 x 90 grid of a spinning scanner, row by row, beams starting off the centre
+    ph, th = np.meshgrid(np.radians(np.linspace(-15.0, 15.0, 15)), np.radians(np.arange(90) * 4.0), indexing="ij")
     gd = np.stack([np.cos(ph) * np.cos(th), np.cos(ph) * np.sin(th), np.sin(ph)], -1).reshape(-1, 3).astype(np.float32)
     go = (0.015 * np.stack([np.cos(th), np.sin(th), np.zeros_like(th)], -1)).reshape(-1, 3).astype(np.float32)
     models = (T.O1DnModel(n, 1, T.Interval(0.1, 60.0), (0.05, -0.02, 0.1), dirs),
               T.OnDnModel(n, 1, T.Interval(0.1, 60.0), origs, dirs),
-              T.O1DnModel(90, 16, T.Interval(0.1, 60.0), (0.0, 0.0, 0.0), gd),
-              T.OnDnModel(90, 16, T.Interval(0.1, 60.0), go, gd))
+              T.O1DnModel(90, 15, T.Interval(0.1, 60.0), (0.0, 0.0, 0.0), gd),
+              T.OnDnModel(90, 15, T.Interval(0.1, 60.0), go, gd))
     P = torch.as_tensor(Tbm[:5], device="cuda")
     for m in models:
         sim = simulator_for(m, hm)
         assert sim.walk_speculative()  # the default for ray lists
         ref = sim.simulate(P, attrs=T.ATTRIBUTES).to_numpy()
         label = f"{type(m).__name__} {m.getWidth()}x{m.getHeight()}"
-        for sorted_ in (True, False):
+        for order in ("sorted", "list", "tiles"):
             for spec in (False, True):
-                sim.set_ray_order(sorted_)
+                sim.set_ray_order(order)
                 sim.set_walk(spec)
-                assert sim.walk_speculative() == spec
+                assert sim.walk_speculative() == spec and sim.ray_order() == order
                 got = sim.simulate(P, attrs=T.ATTRIBUTES).to_numpy()
                 torch.cuda.synchronize()
                 for k in ref:
                     np.testing.assert_array_equal(got[k].view(np.uint8), ref[k].view(np.uint8),
-                                                  err_msg=f"{label} {k} sorted={sorted_} speculative={spec}")
+                                                  err_msg=f"{label} {k} order={order} speculative={spec}")
         compare(got, oracle.simulate(om, m, Tbm[:5]), f"walk/{label}")
         rec = autotune.tune_sensor(sim, P, reps=1)
-        assert set(rec["trials"]) == {"sorted/speculative", "sorted/classic", "list/speculative", "list/classic"}
-        assert sim.ray_order_sorted() == rec["rays_sorted"] and sim.walk_speculative() == rec["speculative"]
+        assert set(rec["trials"]) == {f"{o}/{w}" for o in ("sorted", "list", "tiles") for w in ("speculative", "classic")}
+        assert sim.ray_order() == rec["ray_order"] and sim.walk_speculative() == rec["speculative"]
         assert rec["ms"] == min(rec["trials"].values()) or abs(rec["ms"] - min(rec["trials"].values())) < 1e-3
         got = sim.simulate(P, attrs=T.ATTRIBUTES).to_numpy()
         for k in ref:
             np.testing.assert_array_equal(got[k].view(np.uint8), ref[k].view(np.uint8), err_msg=f"{label} {k} tuned")
+    with pytest.raises(ValueError):
+        sim.set_ray_order("columns")
+    from rmagine_amd import _lib
+    assert _lib.lib().rmh_sensor_set_ray_order(sim._sensor.handle, 3) == 1  # RMH_ERR_INVALID
     sim = simulator_for(scenes.vlp16_workload_model(), hm)
     sim.set_walk(True)
     assert not sim.walk_speculative()
+    sim.set_ray_order("tiles")
+    assert sim.ray_order() == "list"
     assert autotune.tune_sensor(sim, P)["trials"] == {}
 
 
