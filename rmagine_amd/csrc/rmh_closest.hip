@@ -86,7 +86,41 @@ __global__ __launch_bounds__(64, 4) void k_closest(const CPArgs a) {
   volatile int spill_mem[STACK > kClosestLds ? 2 * (STACK - kClosestLds) : 2];
   PrivInt* spill = (PrivInt*)spill_mem;
   int sp = 0;
-  int ref = a.map.n_faces ? a.map.root : kDone;
+  // the triangles of leaf reference `leaf` against the query
+  auto leaf_test = [&](int leaf) {
+    const uint32_t e = ~static_cast<uint32_t>(leaf);
+    const uint32_t first = e >> 3, cnt = (e & 7u) + 1u;
+    for (uint32_t k = 0; k < cnt; k++) {
+      const size_t s = static_cast<size_t>(first + k) * 3;
+      const float4 v0 = a.map.tris[s], v1 = a.map.tris[s + 1], v2 = a.map.tris[s + 2];
+      const D3 p = closest_on_triangle(q, d3(v0.x, v0.y, v0.z), d3(v1.x, v1.y, v1.z), d3(v2.x, v2.y, v2.z));
+      const D3 dv = sub(p, q);
+      const double d2 = ddot(dv, dv);
+      const uint32_t face = __float_as_uint(v2.w);
+      if (d2 < best || (d2 == best && best_slot != 0xFFFFFFFFu && face < best_face)) {
+        best = d2;
+        best_slot = first + k;
+        best_face = face;
+        best_p = p;
+      }
+    }
+  };
+  int ref = kDone;
+  if (a.map.n_faces) {
+    // a bound before the walk: one descent to the nearer child at every level, then that
+    // leaf's triangles.  Without it the walk starts unbounded and pushes the farther child
+    // at every level of its first descent, deeper than the LDS rows hold.  The walk tests
+    // that leaf again, which changes nothing (same distance, same face).
+    int g = a.map.root;
+    while (g >= 0) {
+      const Node2 nd = a.map.nodes[g];
+      const double b0 = box_d2(q, nd.a.x, nd.a.y, nd.a.z, nd.a.w, nd.c.x, nd.c.y);
+      const double b1 = box_d2(q, nd.b.x, nd.b.y, nd.b.z, nd.b.w, nd.c.z, nd.c.w);
+      g = b0 <= b1 ? nd.d.x : nd.d.y;
+    }
+    leaf_test(g);
+    ref = a.map.root;
+  }
   while (ref != kDone) {
     if (ref >= 0) {
       const Node2 nd = a.map.nodes[ref];
@@ -98,29 +132,14 @@ __global__ __launch_bounds__(64, 4) void k_closest(const CPArgs a) {
         // one entry per level: the host picks STACK >= tree depth; the entry's distance is
         // rounded down, so a pop never skips a box that could still hold the nearest point
         push_ref<STACK, 64, kClosestLds, true>(stk, spill, sp, first0 ? nd.d.y : nd.d.x,
-                                             __double2float_rd(first0 ? b1 : b0));
+                                               __double2float_rd(first0 ? b1 : b0));
         ref = first0 ? nd.d.x : nd.d.y;
         continue;
       }
       if (h0) { ref = nd.d.x; continue; }
       if (h1) { ref = nd.d.y; continue; }
     } else {
-      const uint32_t e = ~static_cast<uint32_t>(ref);
-      const uint32_t first = e >> 3, cnt = (e & 7u) + 1u;
-      for (uint32_t k = 0; k < cnt; k++) {
-        const size_t s = static_cast<size_t>(first + k) * 3;
-        const float4 v0 = a.map.tris[s], v1 = a.map.tris[s + 1], v2 = a.map.tris[s + 2];
-        const D3 p = closest_on_triangle(q, d3(v0.x, v0.y, v0.z), d3(v1.x, v1.y, v1.z), d3(v2.x, v2.y, v2.z));
-        const D3 dv = sub(p, q);
-        const double d2 = ddot(dv, dv);
-        const uint32_t face = __float_as_uint(v2.w);
-        if (d2 < best || (d2 == best && best_slot != 0xFFFFFFFFu && face < best_face)) {
-          best = d2;
-          best_slot = first + k;
-          best_face = face;
-          best_p = p;
-        }
-      }
+      leaf_test(ref);
     }
     // pop the next subtree that can still hold something nearer: the bound rounded up to
     // FP32 pops every entry with distance <= best (and at most a few more, whose children
