@@ -264,7 +264,7 @@ def extra_batched(cpu_seconds: float, do_cpu: bool, tune: bool = True) -> dict:
            "per_sensor_launches_ms": round(plain_ms, 3), "per_sensor_mrays_s": round(rays / plain_ms / 1e3, 1),
            "tuned": {**{k: tuned[k] for k in ("chunk_rays", "waves_per_cu", "tile_rows", "lds_stack")},
                      "lds_stack_ms": {str(k): round(v, 3) for k, v in tuned["lds_stack_ms"].items()},
-                     "ray_order": tuned["ray_order"], "ray_order_ms": tuned["ray_order_ms"],
+                     "ray_order": "sorted" if tuned["rays_sorted"] else "list", "ray_order_ms": tuned["ray_order_ms"],
                      "layout": "4-wide" if tuned.get("wide") else "binary", "layout_ms": tuned.get("layout_ms")},
            "batched_vs_per_sensor_mismatches": mism}
     if do_cpu:
