@@ -11,9 +11,10 @@
 //   4b. SAH refinement: three bottom-up rounds of treelet restructuring (Karras and Aila,
 //      HPG 2013; rmh_treelet.hpp), then depth-first renumbering of the leaves so that
 //      every subtree is one range of the triangle array again;
-//   5. bottom-up pass with one atomic arrival counter per interior node: node boxes,
-//      primitive counts and the area-weighted SAH cost; a subtree of at most
-//      max_leaf_size triangles becomes a leaf when that is cheaper under SAH;
+//   5. bottom-up pass: node boxes, primitive counts and the area-weighted SAH cost; a
+//      subtree of at most max_leaf_size triangles becomes a leaf when that is cheaper
+//      under SAH.  The bottom-up passes (4b, 5) run one launch per tree depth, deepest
+//      first, over the interior nodes sorted by depth (make_levels);
 //   6. layout: 64-byte binary nodes that hold both child boxes (padded outward), plus
 //      depth statistics of the reachable tree;
 //   7. gather of the triangles in Morton order (3 x float4 per triangle with the face,
