@@ -846,6 +846,11 @@ def main():
             "traffic_gbs": round(traffic / avg_kernel_s / 1e9, 1) if traffic else None,
             "traffic_frac": round(traffic / avg_kernel_s / 1e9 / HBM_PEAK_GBS, 4) if traffic else None,
             "l2_hit_rate": l2_hit,
+            # what does bind the kernel (same profile): the texture data path that returns
+            # vector-load data is busy this share of the kernel, and the VALU issues this
+            # many instructions per SIMD and cycle (a wave64 FP32 fma takes 4 cycles: 0.25)
+            "td_busy": profiled(default_workload, "td_busy", float),
+            "valu_instr_per_simd_cycle": profiled(default_workload, "valu_instr_per_simd_cycle", float),
             "kernel": "k_simulate<spherical> (traversal + attribute writers)",
             "kernel_avg_ms": round(avg_kernel_s * 1e3, 4),
             "alg_bytes_per_launch": alg_bytes,
