@@ -5,6 +5,7 @@
 // allocations.  Sensors are the device-side copy of a sensor model that the reference's
 // setModel() uploads (src/rmagine_optix/src/simulation/SphereSimulatorOptix.cpp:20-60,
 // O1DnSimulatorOptix.cpp:41-83).
+#include "rmh_grid_order.hpp"
 #include "rmh_internal.hpp"
 
 #include <rocprim/device/device_radix_sort.hpp>
@@ -244,21 +245,6 @@ static hipError_t upload_sorted_rays(rmh_sensor* s, const rmh_vec3* dirs_in, con
   return e;
 }
 
-// Buffer indices of the W x H ray grid (row-major) in the order of 4 x 16 tiles, row of
-// tiles by row of tiles, each tile row by row; positions outside the grid are skipped, so
-// every index appears once and, when 4 divides H and 16 divides W, each run of 64 slots is
-// one tile.
-static std::vector<uint32_t> grid_tile_order(uint32_t W, uint32_t H) {
-  constexpr uint32_t th = 4, tw = 16;
-  std::vector<uint32_t> o;
-  o.reserve(static_cast<size_t>(W) * H);
-  for (uint32_t ty = 0; ty < H; ty += th)
-    for (uint32_t tx = 0; tx < W; tx += tw)
-      for (uint32_t r = ty; r < std::min(H, ty + th); r++)
-        for (uint32_t c = tx; c < std::min(W, tx + tw); c++) o.push_back(r * W + c);
-  return o;
-}
-
 // Rewrites the traversal arrays in place (same buffers, so batch tables and captured
 // graphs that point at them stay valid): list order, direction order, or the grid's tiles.
 int rmh_sensor_set_ray_order(rmh_sensor* s, int order) {
@@ -276,7 +262,7 @@ int rmh_sensor_set_ray_order(rmh_sensor* s, int order) {
     k_iota<<<blocks, 256>>>(s->order, n);
     RMH_HIP_TRY(hipGetLastError());
   } else {
-    const std::vector<uint32_t> t = grid_tile_order(s->width, s->height);
+    const std::vector<uint32_t> t = rmh::grid_tile_order(s->width, s->height);
     RMH_HIP_TRY(hipMemcpy(s->order, t.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice));
   }
   k_gather_rays<<<blocks, 256>>>(s->order, n, s->dirs_list, s->origs_list, s->dirs, s->origs);
