@@ -82,7 +82,7 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the batched / pose-gradient / ICP workloads")
-    ap.add_argument("--extra-only", choices=["batched", "pose_gradient", "icp", "next_rows"],
+    ap.add_argument("--extra-only", choices=["batched", "pose_gradient", "icp", "next_rows", "math_layer"],
                     help="only run one secondary workload with default launch parameters (for profiling)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy 5 times (scale factor for HBM counters)")
@@ -649,6 +649,74 @@ def extra_next_rows(do_cpu: bool, cpu_seconds: float = 3.0) -> dict:
     return out
 
 
+def extra_math_layer(do_cpu: bool) -> dict:
+    """The batched pose operations and moments of the math layer at sizes that fill the GPU,
+    each timed with HIP events (mean of 5 launches after an untimed one), with its algorithmic
+    bytes against 8 TB/s:
+    * rmh_transform_mul (multNxN), 16,777,216 transform pairs: 64 B read, 32 B written each;
+    * rmh_transform_inv, 16,777,216 transforms: 32 B read, 32 B written;
+    * rmh_transform_points, one transform applied to 67,108,864 points (mult1xN): 12 B read,
+      12 B written per point;
+    * rmh_transform_pack_quat, 16,777,216 (quaternion, translation) pairs: 28 B read, 28 B
+      written (the stamps are kept);
+    * rmh_vec3_moments (segmented FP64 sum, mean and covariance), 67,108,864 point pairs with
+      a mask in 2,048 segments: 25 B read per pair.
+    With do_cpu, the oracle's moments of the first 8 segments are compared with the GPU's."""
+    import torch
+
+    from rmagine_amd import math as M
+
+    dev = torch.device("cuda:0")
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(9)
+    nt, npts = 1 << 24, 1 << 26
+
+    def transforms(n):
+        T = torch.rand((n, 8), generator=gen, device=dev) * 2.0 - 1.0
+        T[:, :4] /= T[:, :4].norm(dim=1, keepdim=True)
+        T[:, 7] = 0.0
+        return T
+
+    A, B = transforms(nt), transforms(nt)
+    C = torch.empty_like(A)
+    out = {}
+
+    def leg(name, fn, nbytes, unit_count, unit):
+        ms = _timed_ms(fn)
+        out[name] = {"ms": round(ms, 4), unit: unit_count,
+                     "roofline": roofline(name, nbytes, ms, note=f"algorithmic bytes: {nbytes / unit_count:.0f} B per {unit[:-1]}")}
+
+    leg("k_tf_mul (rmh_transform_mul)", lambda: M.transform_mul(A, B, out=C), 96 * nt, nt, "transforms")
+    leg("k_tf_inv (rmh_transform_inv)", lambda: M.transform_inv(A, out=C), 64 * nt, nt, "transforms")
+    q = A[:, :4].contiguous()
+    t = A[:, 4:7].contiguous()
+    leg("k_tf_pack_quat (rmh_transform_pack_quat)", lambda: M.pack(q, t, out=C), 56 * nt, nt, "transforms")
+    del B, q, t
+    P = torch.rand((npts, 3), generator=gen, device=dev) * 100.0 - 50.0
+    Q = torch.empty_like(P)
+    leg("k_tf_points (rmh_transform_points)", lambda: M.transform_points(A[:1], P, out=Q), 24 * npts, npts, "points")
+    del A, C
+    mask = (torch.rand(npts, generator=gen, device=dev) > 0.1).to(torch.uint8)
+    seg = npts // 2048
+    res = {}
+
+    def moments():
+        res.update(M.vec3_moments(P, Q, mask, segment_size=seg, n_segments=2048))
+    leg("k_moments (rmh_vec3_moments)", moments, 25 * npts, npts, "pairs")
+    if do_cpu:
+        from oracle import stats as OS
+        k = 8
+        o = OS.moments(P[:k * seg].cpu().numpy(), Q[:k * seg].cpu().numpy(), mask[:k * seg].cpu().numpy(),
+                       segment_size=seg, n_segments=k)
+        g = {key: res[key][:k].cpu().numpy() for key in ("sum", "mean", "cov", "count")}
+        rel = max(float(np.max(np.abs(g[key] - o[key]) / np.maximum(np.abs(o[key]), 1.0))) for key in ("sum", "mean", "cov"))
+        out["k_moments (rmh_vec3_moments)"]["oracle_check"] = {"segments": k, "max_rel_err": rel,
+                                                                "counts_equal": bool(np.array_equal(g["count"], o["count"]))}
+    del P, Q, mask
+    torch.cuda.empty_cache()
+    return out
+
+
 def compare_layouts(names) -> None:
     """Binary against 4-wide quantised node layout on the same map: ms per simulate() (min of
     5 HIP-event timings after 2 warm-ups) and a bit-for-bit comparison of every output."""
@@ -751,7 +819,8 @@ def main():
 
     if args.extra_only:
         fn = {"batched": lambda: extra_batched(0.0, False, tune=False), "pose_gradient": extra_pose_grad,
-              "icp": extra_icp, "next_rows": lambda: extra_next_rows(not args.no_cpu)}[args.extra_only]
+              "icp": extra_icp, "next_rows": lambda: extra_next_rows(not args.no_cpu),
+              "math_layer": lambda: extra_math_layer(not args.no_cpu)}[args.extra_only]
         print(json.dumps({args.extra_only: fn()}))
         return
 
@@ -868,7 +937,8 @@ def main():
         torch.cuda.empty_cache()
         for name, fn in (("batched_heterogeneous", lambda: extra_batched(8.0, not args.no_cpu)),
                          ("pose_gradient", extra_pose_grad), ("icp_graph", extra_icp),
-                         ("next_rows", lambda: extra_next_rows(not args.no_cpu))):
+                         ("next_rows", lambda: extra_next_rows(not args.no_cpu)),
+                         ("math_layer", lambda: extra_math_layer(not args.no_cpu))):
             t0 = time.time()
             extra[name] = fn()
             log(f"{name}: {extra[name]} ({time.time() - t0:.1f} s)")
