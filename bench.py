@@ -471,8 +471,8 @@ def extra_next_rows(do_cpu: bool, cpu_seconds: float = 3.0) -> dict:
       (read and write), against 8 TB/s;
     * OnDn simulation (per-ray origins): 1,000 poses x the 128 x 1024 Ouster-style ray list,
       all seven attributes: 41 B written + 24 B of direction and origin read per ray; timed
-      with the sensor's defaults (direction-sorted, speculative walk) and after
-      autotune.tune_sensor picked its ray order and walk;
+      with the sensor's defaults (for a grid of this size: its 4 x 16 tiles, classic walk)
+      and after autotune.tune_sensor timed every ray order and walk;
     * closest-point queries (rmh_closest_points): 4,194,304 points within 2 m of the surface,
       unbounded search: 12 B read + 40 B written per query;
     * BVH refit (rmh_map_refit) after moving every vertex, synchronous, one launch per tree

@@ -220,10 +220,12 @@ int rmh_ouster_o1dn(const char* path, uint32_t* width, uint32_t* height, float* 
 int rmh_sensor_create(int kind, const void* model, rmh_sensor** out);
 int rmh_sensor_destroy(rmh_sensor* sensor);
 /* Traversal order of an O1Dn/OnDn sensor's rays (no counterpart in the reference, which
- * casts them in list order): RMH_RAY_ORDER_SORTED = by direction (the default: a wave's 64
- * rays cover one patch of the sphere), RMH_RAY_ORDER_LIST = list order (rays already
- * ordered by the caller), RMH_RAY_ORDER_TILES = the model's W x H grid (buffer index
- * row * W + column) in 4 x 16 tiles, as image sensors are cast (for a scanner's ray grid).
+ * casts them in list order): RMH_RAY_ORDER_SORTED = by direction (a wave's 64 rays cover
+ * one patch of the sphere), RMH_RAY_ORDER_LIST = list order (rays already ordered by the
+ * caller), RMH_RAY_ORDER_TILES = the model's W x H grid (buffer index row * W + column) in
+ * 4 x 16 tiles, as image sensors are cast (for a scanner's ray grid).  Defaults, set by
+ * rmh_sensor_create: models of at least 4 rows and 16 columns start in tiles with the
+ * classic walk (rmh_sensor_set_walk), other lists sorted with the speculative walk.
  * Results are identical in every order; a tuning choice of the autotuner.  Rewrites the
  * sensor's device arrays in place (batches and graphs that use the sensor stay valid);
  * synchronous; no launch using the sensor may be in flight.  Other values are refused with
@@ -237,9 +239,10 @@ int rmh_sensor_get_ray_order(const rmh_sensor* sensor, int* order);
  * rmh_simulate_visit_stats (no counterpart in the reference, whose ray casts are one
  * library call per ray): speculative != 0 = a lane that reaches a leaf holds it while the
  * wave walks on, and the wave tests leaves once an eighth of its lanes hold one (the
- * default; fastest for scattered rays such as ICP scans), 0 = the classic walk, in which
- * the wave walks until every lane holds a leaf (fastest for a scanner's coherent ray
- * grid).  Results are identical either way; a tuning choice of the autotuner.  A flag read
+ * default of lists of fewer than 4 rows or 16 columns; fastest for scattered rays such as
+ * ICP scans), 0 = the classic walk, in which the wave walks until every lane holds a leaf
+ * (the default of larger grids; fastest for a scanner's coherent ray grid).  Results are
+ * identical either way; a tuning choice of the autotuner.  A flag read
  * at launch: no device work.  Batches and the pose gradient walk ray lists speculatively
  * whatever the sensor holds.  Image sensors always walk the classic way and report 0. */
 int rmh_sensor_set_walk(rmh_sensor* sensor, int speculative);

@@ -365,6 +365,18 @@ int rmh_sensor_create(int kind, const void* model, rmh_sensor** out) {
     sensor_free(s);
     return hip_fail(e, "rmh_sensor_create");
   }
+  // A ray list shaped as a scanner's grid (at least one 4 x 16 tile) is cast like an image
+  // by default: in the grid's tiles with the classic walk (an Ouster-style 128 x 1024 OnDn
+  // list on the terrain: 9.3 ms instead of 15.4 sorted and speculative).  Lists of one or a
+  // few rows keep the direction order and the speculative walk that scattered rays need.
+  if ((s->kind == RMH_MODEL_O1DN || s->kind == RMH_MODEL_ONDN) && s->height >= rmh::kGridTileRows &&
+      s->width >= rmh::kGridTileCols) {
+    if (int rc = rmh_sensor_set_ray_order(s, RMH_RAY_ORDER_TILES)) {
+      sensor_free(s);
+      return rc;
+    }
+    s->speculative = 0;
+  }
   *out = s;
   return RMH_OK;
 }

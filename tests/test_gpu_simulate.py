@@ -196,7 +196,8 @@ def test_walk_switch_and_sensor_tuner(room):
     P = torch.as_tensor(Tbm[:5], device="cuda")
     for m in models:
         sim = simulator_for(m, hm)
-        assert sim.walk_speculative()  # the default for ray lists
+        grid = m.getHeight() >= 4 and m.getWidth() >= 16  # a scanner grid: cast like an image
+        assert sim.walk_speculative() == (not grid) and sim.ray_order() == ("tiles" if grid else "sorted")
         ref = sim.simulate(P, attrs=T.ATTRIBUTES).to_numpy()
         label = f"{type(m).__name__} {m.getWidth()}x{m.getHeight()}"
         for order in ("sorted", "list", "tiles"):
