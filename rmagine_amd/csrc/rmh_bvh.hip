@@ -405,18 +405,24 @@ __global__ __launch_bounds__(kBlock) void k_leaf_boxes(uint32_t nf, const uint32
 // per-subset choices are those of the serial treelet_optimize().
 __device__ void treelet_wave(const TreeArrays& t, int root, float c_node, float c_tri, float* s_area, float* s_copt,
                              uint8_t* s_part) {
+  // the treelet (formed by lane 0) and its leaf boxes (one lane each) live in LDS: held per
+  // lane, the arrays were private memory that all 64 lanes wrote and read
+  __shared__ uint32_t s_leaf[kTreeletLeaves];
+  __shared__ int s_inner[kTreeletLeaves - 1];
+  __shared__ float s_lb[kTreeletLeaves][6];
+  __shared__ int s_nl;
   const int lane = threadIdx.x;
-  uint32_t leaf[kTreeletLeaves];
-  int inner[kTreeletLeaves - 1];
-  const int nl = treelet_form(t, root, leaf, inner);
-  if (nl < 3) return;
-  float lb[kTreeletLeaves][6];
-  for (int j = 0; j < nl; j++) ref_box(t, leaf[j], lb[j]);
+  if (lane == 0) s_nl = treelet_form(t, root, s_leaf, s_inner);
+  __syncthreads();
+  const int nl = s_nl;
+  if (nl < 3) return;  // the same for every lane
+  if (lane < nl) ref_box(t, s_leaf[lane], s_lb[lane]);
+  __syncthreads();
   const uint32_t full = (1u << nl) - 1u;
   for (uint32_t S = lane; S <= full; S += 64)
-    if (S >= 1) s_area[S] = subset_area(lb, nl, S);
+    if (S >= 1) s_area[S] = subset_area(s_lb, nl, S);
   if (lane < nl) {
-    s_copt[1u << lane] = ref_cost(t, leaf[lane], c_tri);
+    s_copt[1u << lane] = ref_cost(t, s_leaf[lane], c_tri);
     s_part[1u << lane] = 0;
   }
   __syncthreads();
@@ -431,7 +437,7 @@ __device__ void treelet_wave(const TreeArrays& t, int root, float c_node, float 
     __syncthreads();
   }
   const bool better = s_copt[full] < t.cost[root] * (1.0f - 1e-6f);
-  if (better && lane == 0) treelet_rebuild(t, root, leaf, inner, nl, s_part, c_node, c_tri);
+  if (better && lane == 0) treelet_rebuild(t, root, s_leaf, s_inner, nl, s_part, c_node, c_tri);
   __syncthreads();
 }
 
