@@ -93,14 +93,19 @@ def test_batch_lds_stack_depths_and_validation():
 def test_batch_one_or_two_launches():
     """One persistent launch for every entry (the default) and the split into an image
     kernel and a ray-list kernel write the same bytes; other launch counts are refused."""
-    V, F, P = scenes.room_workload(n_sensors=12)
+    V, F, P = scenes.room_workload(n_sensors=15)
     hm = HipMap(V, F)
     rng = np.random.default_rng(8)
     d = rng.normal(size=(1500, 3))
     o1 = T.O1DnModel(1500, 1, T.Interval(0.1, 60.0), (0.0, 0.0, 0.05),
                      (d / np.linalg.norm(d, axis=1, keepdims=True)).astype(np.float32))
     pin = T.PinholeModel(96, 64, T.Interval(0.05, 80.0), (50.0, 50.0), (47.5, 31.5))
-    models = (o1, scenes.vlp16_workload_model(), pin, o1)  # list entries before image entries
+    # a scanner-grid list (its sensor walks the classic way in its own simulate, the batch
+    # speculatively: the bytes must agree) beside scattered lists
+    ph, th = np.meshgrid(np.radians(np.linspace(-15.0, 15.0, 16)), np.radians(np.arange(90) * 4.0), indexing="ij")
+    gd = np.stack([np.cos(ph) * np.cos(th), np.cos(ph) * np.sin(th), np.sin(ph)], -1).reshape(-1, 3).astype(np.float32)
+    grid = T.O1DnModel(90, 16, T.Interval(0.1, 60.0), (0.0, 0.0, 0.05), gd)
+    models = (o1, scenes.vlp16_workload_model(), pin, o1, grid)  # list entries before image entries
     entries = [{"model": m, "Tbm": torch.as_tensor(P[3 * i:3 * i + 3], device="cuda"), "attrs": T.ATTRIBUTES}
                for i, m in enumerate(models)]
     batch = BatchSimulatorHip(hm, entries)
