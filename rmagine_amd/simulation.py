@@ -316,10 +316,11 @@ class SimulatorHip:
         return None if self._sensor is None else self._sensor.model
 
     def set_ray_order(self, sorted_=True) -> None:
-        """O1Dn/OnDn: cast the rays in direction-sorted order (True or "sorted", the
-        default), in list order (False or "list"), or as the model's W x H grid in 4 x 16
-        tiles ("tiles", as image sensors are cast).  The results are identical; only the
-        speed differs.  Image models ignore it."""
+        """O1Dn/OnDn: cast the rays in direction-sorted order (True or "sorted"), in list
+        order (False or "list"), or as the model's W x H grid in 4 x 16 tiles ("tiles", as
+        image sensors are cast).  Default: "tiles" for models of at least 4 rows and 16
+        columns, else "sorted".  The results are identical; only the speed differs.  Image
+        models ignore it."""
         if self._sensor is None:
             raise RuntimeError("set_ray_order: no sensor model set")
         self._sensor.set_ray_order(sorted_)
@@ -331,10 +332,11 @@ class SimulatorHip:
         return self._sensor is not None and self._sensor.ray_order_sorted()
 
     def set_walk(self, speculative: bool = True) -> None:
-        """O1Dn/OnDn: walk the BVH speculatively (default: lanes that reach a leaf hold it
-        while the wave walks on; fastest for scattered rays) or the classic way (the wave
-        walks until every lane holds a leaf; fastest for a scanner's coherent ray grid).  The
-        results are identical; only the speed differs.  Image models ignore it."""
+        """O1Dn/OnDn: walk the BVH speculatively (lanes that reach a leaf hold it while the
+        wave walks on; fastest for scattered rays, the default of lists of fewer than 4 rows
+        or 16 columns) or the classic way (the wave walks until every lane holds a leaf;
+        fastest for a scanner's coherent ray grid, the default of larger grids).  The results
+        are identical; only the speed differs.  Image models ignore it."""
         if self._sensor is None:
             raise RuntimeError("set_walk: no sensor model set")
         self._sensor.set_walk(speculative)
