@@ -474,7 +474,8 @@ def extra_next_rows(do_cpu: bool, cpu_seconds: float = 3.0) -> dict:
       with the sensor's defaults (for a grid of this size: its 4 x 16 tiles, classic walk)
       and after autotune.tune_sensor timed every ray order and walk;
     * closest-point queries (rmh_closest_points): 4,194,304 points within 2 m of the surface,
-      unbounded search: 12 B read + 40 B written per query;
+      unbounded search: 12 B read + 40 B written per query; in random order and again sorted
+      by the Morton code of their 0.5 m cells (as a scan's points come);
     * BVH refit (rmh_map_refit) after moving every vertex, synchronous, one launch per tree
       depth (the schedule made at the first, untimed refit): per triangle 48 B of triangle and
       16 B of normal written, 36 B of vertices read; per node 64 B written;
@@ -586,6 +587,23 @@ def extra_next_rows(do_cpu: bool, cpu_seconds: float = 3.0) -> dict:
     ms = _timed_ms(lambda: hm.closest_points(qd))
     closest = {"queries": nq, "ms": round(ms, 3), "mqueries_s": round(nq / ms / 1e3, 1),
                "roofline": roofline("k_closest", 52 * nq, ms, note="algorithmic bytes: 12 B query read, 40 B written")}
+    # the same queries in Morton order of their cells (sorted by the caller, outside the timed
+    # region): neighbouring lanes then walk neighbouring parts of the tree, as the points of
+    # one scan do; results are the same per query
+    cell = np.floor((q - lo) / 0.5).astype(np.int64)
+    key = np.zeros(nq, np.int64)
+    for b in range(16):
+        for k in range(3):
+            key |= ((cell[:, k] >> b) & 1) << (3 * b + k)
+    perm = np.argsort(key, kind="stable")
+    qs = torch.as_tensor(np.ascontiguousarray(q[perm]), device=dev)
+    cps = hm.closest_points(qs)
+    ms_s = _timed_ms(lambda: hm.closest_points(qs))
+    pd = torch.as_tensor(perm, device=dev)
+    closest["morton_sorted_queries"] = {
+        "ms": round(ms_s, 3), "mqueries_s": round(nq / ms_s / 1e3, 1),
+        "results_equal": bool(torch.equal(cps["dist"], cp["dist"][pd]) and torch.equal(cps["face_ids"], cp["face_ids"][pd]))}
+    del qs, cps, pd
     if do_cpu:
         from oracle import closest as OC
         k = 2
