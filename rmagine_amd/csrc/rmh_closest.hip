@@ -4,20 +4,21 @@
 // and its point-query callback (EmbreeMesh.cpp:22-131: closest point of a triangle,
 // result {d, p, n, primID, geomID}, updated while d < the query radius).
 //
-// One lane per query.  Depth-first BVH walk, nearer child first; a subtree is skipped when
-// the squared distance from the query to its (outward padded) box exceeds the best squared
-// distance so far.  The walk's stack is the ray kernels' (rmh_trace.hpp): the top rows in
-// LDS, deeper entries in private memory; held in registers, the 32 entries and their
-// distances took the kernel to 190 VGPRs (two waves per SIMD).  An unbounded search pushes
-// a subtree at almost every level of its first descent, so the stack is deeper than a ray's:
-// 16 LDS rows (8 KB per wave; the kernel's 90 VGPRs allow five waves per SIMD, which the
-// 160 KB of LDS of a CU then just holds).  Box distances and the triangle's closest point (Ericson, Real-Time
-// Collision Detection, 5.1.5: Voronoi regions of the vertices, edges and face) are
-// evaluated in FP64, so the result is the FP64 nearest triangle; equal distances go to the
-// lower global face index.  Outputs per query (each pointer may be NULL): distance
-// (FLT_MAX when nothing lies within max_distance), closest point, unit face normal (not
-// flipped), face id within its mesh, geom id, object id (0xFFFFFFFF for none; point and
-// normal NaN).
+// One lane per query.  A bound first (one descent to the nearer child at every level and
+// that leaf's triangles), then a depth-first BVH walk, nearer child first; a subtree is
+// skipped when a lower bound of the squared distance from the query to its (outward padded)
+// box, computed in FP32 (box_d2), exceeds the best squared distance so far.  The walk's
+// stack is the ray kernels' (rmh_trace.hpp): the top rows in LDS, deeper entries in private
+// memory; held in registers, the 32 entries and their distances took the kernel to 190
+// VGPRs (two waves per SIMD).  An unbounded search pushes a subtree at almost every level of
+// its first descent, so the stack is deeper than a ray's: 16 LDS rows (8 KB per wave; the
+// kernel's registers allow five waves per SIMD, which the 160 KB of LDS of a CU then just
+// holds).  The triangle's closest point (Ericson, Real-Time Collision Detection, 5.1.5:
+// Voronoi regions of the vertices, edges and face) and its distance are evaluated in FP64,
+// so the result is the FP64 nearest triangle; equal distances go to the lower global face
+// index.  Outputs per query (each pointer may be NULL): distance (FLT_MAX when nothing lies
+// within max_distance), closest point, unit face normal (not flipped), face id within its
+// mesh, geom id, object id (0xFFFFFFFF for none; point and normal NaN).
 #include "rmh_trace.hpp"
 
 #include <cfloat>
@@ -55,12 +56,19 @@ __device__ D3 closest_on_triangle(const D3& p, const D3& a, const D3& b, const D
   return madd(madd(a, ab, vb * den), ac, vc * den);
 }
 
-// Squared distance from p to the box [lo, hi].
-__device__ __forceinline__ double box_d2(const D3& p, float lox, float hix, float loy, float hiy, float loz, float hiz) {
-  const double dx = fmax(fmax((double)lox - p.x, p.x - (double)hix), 0.0);
-  const double dy = fmax(fmax((double)loy - p.y, p.y - (double)hiy), 0.0);
-  const double dz = fmax(fmax((double)loz - p.z, p.z - (double)hiz), 0.0);
-  return dx * dx + dy * dy + dz * dz;
+// A lower bound of the squared distance from the query (qx, qy, qz: FP32, as given) to the
+// box [lo, hi], in FP32: each difference of two floats is rounded once (relative error at
+// most 2^-24), the squares and the sum add three more roundings, so shrinking the result by
+// 2^-20 bounds it from below; an overflow to infinity is clamped to FLT_MAX (still below any
+// FP64 bound the walk can hold for such a box).  Pruning with a lower bound never drops a
+// box that holds the nearest point; the triangle distances stay FP64.
+__device__ __forceinline__ double box_d2(float qx, float qy, float qz, float lox, float hix, float loy, float hiy,
+                                         float loz, float hiz) {
+  const float dx = fmaxf(fmaxf(lox - qx, qx - hix), 0.0f);
+  const float dy = fmaxf(fmaxf(loy - qy, qy - hiy), 0.0f);
+  const float dz = fmaxf(fmaxf(loz - qz, qz - hiz), 0.0f);
+  const float d2 = (dx * dx + dy * dy + dz * dz) * (1.0f - 0x1p-20f);
+  return static_cast<double>(fminf(d2, FLT_MAX));
 }
 
 struct CPArgs {
@@ -114,8 +122,8 @@ __global__ __launch_bounds__(64, 4) void k_closest(const CPArgs a) {
     int g = a.map.root;
     while (g >= 0) {
       const Node2 nd = a.map.nodes[g];
-      const double b0 = box_d2(q, nd.a.x, nd.a.y, nd.a.z, nd.a.w, nd.c.x, nd.c.y);
-      const double b1 = box_d2(q, nd.b.x, nd.b.y, nd.b.z, nd.b.w, nd.c.z, nd.c.w);
+      const double b0 = box_d2(qv.x, qv.y, qv.z, nd.a.x, nd.a.y, nd.a.z, nd.a.w, nd.c.x, nd.c.y);
+      const double b1 = box_d2(qv.x, qv.y, qv.z, nd.b.x, nd.b.y, nd.b.z, nd.b.w, nd.c.z, nd.c.w);
       g = b0 <= b1 ? nd.d.x : nd.d.y;
     }
     leaf_test(g);
@@ -124,8 +132,8 @@ __global__ __launch_bounds__(64, 4) void k_closest(const CPArgs a) {
   while (ref != kDone) {
     if (ref >= 0) {
       const Node2 nd = a.map.nodes[ref];
-      const double b0 = box_d2(q, nd.a.x, nd.a.y, nd.a.z, nd.a.w, nd.c.x, nd.c.y);
-      const double b1 = box_d2(q, nd.b.x, nd.b.y, nd.b.z, nd.b.w, nd.c.z, nd.c.w);
+      const double b0 = box_d2(qv.x, qv.y, qv.z, nd.a.x, nd.a.y, nd.a.z, nd.a.w, nd.c.x, nd.c.y);
+      const double b1 = box_d2(qv.x, qv.y, qv.z, nd.b.x, nd.b.y, nd.b.z, nd.b.w, nd.c.z, nd.c.w);
       const bool h0 = b0 <= best, h1 = b1 <= best;
       if (h0 && h1) {
         const bool first0 = b0 <= b1;
