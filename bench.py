@@ -474,8 +474,9 @@ def extra_next_rows(do_cpu: bool, cpu_seconds: float = 3.0) -> dict:
       with the sensor's defaults (for a grid of this size: its 4 x 16 tiles, classic walk)
       and after autotune.tune_sensor timed every ray order and walk;
     * closest-point queries (rmh_closest_points): 4,194,304 points within 2 m of the surface,
-      unbounded search: 12 B read + 40 B written per query; in random order and again sorted
-      by the Morton code of their 0.5 m cells (as a scan's points come);
+      unbounded search: 12 B read + 40 B written per query; in random order (which the call
+      sorts by the Morton code of their cells and walks in that order) and again sorted by
+      the caller by the Morton code of their 0.5 m cells;
     * BVH refit (rmh_map_refit) after moving every vertex, synchronous, one launch per tree
       depth (the schedule made at the first, untimed refit): per triangle 48 B of triangle and
       16 B of normal written, 36 B of vertices read; per node 64 B written;

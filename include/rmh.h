@@ -175,7 +175,12 @@ int rmh_map_set_layout(rmh_map* map, int wide, void* stream);
  * strictly within max_distance.  Replaces EmbreeMap::closestPoint
  * (src/rmagine_embree/src/map/EmbreeMap.cpp:33-37, EmbreeScene.cpp:292-315, callback
  * EmbreeMesh.cpp:22-131) for a batch.  Every output pointer (device) may be NULL.  Misses:
- * dist FLT_MAX, ids 0xFFFFFFFF, point and normal NaN.  Equal distances: lower global face. */
+ * dist FLT_MAX, ids 0xFFFFFFFF, point and normal NaN.  Equal distances: lower global face.
+ * Batches of 65,536 queries or more are sorted on the device by the Morton code of their
+ * positions and walked in that order when their order is spatially incoherent (random
+ * points; not the rows of a scan); the results do not depend on the order.  The sort's
+ * scratch (17 B per query) comes from a stream-ordered pool of the library that keeps up to
+ * 256 MiB per device between calls; under stream capture the batch is walked as given. */
 typedef struct {
   float* dist;
   rmh_vec3* points;

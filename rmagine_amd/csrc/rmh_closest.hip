@@ -19,9 +19,25 @@
 // index.  Outputs per query (each pointer may be NULL): distance (FLT_MAX when nothing lies
 // within max_distance), closest point, unit face normal (not flipped), face id within its
 // mesh, geom id, object id (0xFFFFFFFF for none; point and normal NaN).
+//
+// Lanes of a wave that walk unrelated parts of the tree fetch its nodes separately: random
+// query sets fetch about 3x the node bytes of the same points in spatial order.  Batches of
+// at least kSortQueries queries are therefore sorted by the Morton code of their positions in
+// the map's bounding box (30-bit codes, on the device, in stream-ordered scratch), and walked
+// in that order when a sample of their waves spans kSpreadRatio times less space in it than
+// in the caller's order (the walk then reads queries and writes results scattered, which
+// costs a walk in an already coherent order more than it gains).  Each lane reads and writes
+// its own query's slots, so the results do not depend on the order.  Under stream capture,
+// or when the scratch cannot be had, the batch is walked in the caller's order.  The scratch
+// (17 B per query) comes from a pool of the library's own that keeps kPoolKeep bytes per
+// device between calls.
 #include "rmh_trace.hpp"
 
+#include <rocprim/device/device_radix_sort.hpp>
+
+#include <algorithm>
 #include <cfloat>
+#include <mutex>
 
 namespace {
 
@@ -74,18 +90,23 @@ __device__ __forceinline__ double box_d2(float qx, float qy, float qz, float lox
 struct CPArgs {
   MapData map;
   const rmh_vec3* q;
+  const uint32_t* order;  // lane i takes query order[i] when the spread test below holds
+  const float* spread;    // summed wave extents of a sample: [0] caller's order, [1] sorted
   uint32_t n;
   double best0;  // max_distance^2: a result needs d^2 < best0
   rmh_closest_outputs out;
 };
 
 constexpr int kClosestLds = 16;
+constexpr float kSpreadRatio = 4.0f;
 
 template <int STACK>
 __global__ __launch_bounds__(64, 4) void k_closest(const CPArgs a) {
   __shared__ int2 stk[kClosestLds][64];
-  const uint32_t i = blockIdx.x * 64 + threadIdx.x;
-  if (i >= a.n) return;
+  const uint32_t lane_q = blockIdx.x * 64 + threadIdx.x;
+  if (lane_q >= a.n) return;
+  const bool reorder = a.order && a.spread[1] * kSpreadRatio < a.spread[0];
+  const uint32_t i = reorder ? a.order[lane_q] : lane_q;
   const rmh_vec3 qv = a.q[i];
   const D3 q = d3(qv.x, qv.y, qv.z);
   double best = a.best0;
@@ -176,6 +197,151 @@ __global__ __launch_bounds__(64, 4) void k_closest(const CPArgs a) {
   }
 }
 
+constexpr uint32_t kSortQueries = 1u << 16;
+
+// Spreads the low 10 bits of v to every third bit.
+__device__ __forceinline__ uint32_t spread3(uint32_t v) {
+  v &= 0x3FFu;
+  v = (v | (v << 16)) & 0x030000FFu;
+  v = (v | (v << 8)) & 0x0300F00Fu;
+  v = (v | (v << 4)) & 0x030C30C3u;
+  v = (v | (v << 2)) & 0x09249249u;
+  return v;
+}
+
+// 30-bit Morton code of each query's cell in a grid of cubic cells, 1024 along the longest
+// side of the root's box (queries outside it clamp to the border cells; cubic cells keep a
+// wave's queries in a compact region of a flat map, not a thin slab), and the identity
+// permutation beside it.
+__global__ __launch_bounds__(256) void k_query_keys(const Node2* nodes, int root, const rmh_vec3* q, uint32_t n,
+                                                   uint32_t* keys, uint32_t* idx, float* spread) {
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i == 0) { spread[0] = 0.0f; spread[1] = 0.0f; }
+  if (i >= n) return;
+  const Node2 r = nodes[root];
+  const float lx = fminf(r.a.x, r.b.x), hx = fmaxf(r.a.y, r.b.y);
+  const float ly = fminf(r.a.z, r.b.z), hy = fmaxf(r.a.w, r.b.w);
+  const float lz = fminf(r.c.x, r.c.z), hz = fmaxf(r.c.y, r.c.w);
+  const float scale = 1024.0f / fmaxf(fmaxf(fmaxf(hx - lx, hy - ly), hz - lz), 1e-30f);
+  const rmh_vec3 p = q[i];
+  auto cell = [scale](float v, float lo) {
+    return static_cast<uint32_t>(fminf(fmaxf((v - lo) * scale, 0.0f), 1023.0f));  // NaN -> 0
+  };
+  keys[i] = spread3(cell(p.x, lx)) | (spread3(cell(p.y, ly)) << 1) | (spread3(cell(p.z, lz)) << 2);
+  idx[i] = i;
+}
+
+constexpr uint32_t kSpreadWaves = 4096;
+
+// Extent (x + y + z) of the queries of sampled waves, in the caller's order and in the
+// sorted one, summed into spread[0] and spread[1].  Wave k of the grid takes the 64 queries
+// from (k * stride) * 64; lanes past n repeat query n - 1, which belongs to the same wave.
+__global__ __launch_bounds__(256) void k_order_spread(const rmh_vec3* q, const uint32_t* order, uint32_t n,
+                                                     uint32_t stride, float* spread) {
+  __shared__ float part[2][4];
+  const uint32_t wave = blockIdx.x * 4 + threadIdx.x / 64, lane = threadIdx.x & 63u;
+  const uint64_t first = static_cast<uint64_t>(wave) * stride * 64;
+  float ext[2] = {0.0f, 0.0f};
+  if (first < n) {
+    const uint32_t j = first + lane < n ? static_cast<uint32_t>(first + lane) : n - 1;
+    const rmh_vec3 pts[2] = {q[j], q[order[j]]};
+    for (int o = 0; o < 2; o++) {
+      float lo[3] = {pts[o].x, pts[o].y, pts[o].z}, hi[3] = {pts[o].x, pts[o].y, pts[o].z};
+      for (int m = 32; m >= 1; m >>= 1)
+        for (int c = 0; c < 3; c++) {
+          lo[c] = fminf(lo[c], __shfl_xor(lo[c], m, 64));
+          hi[c] = fmaxf(hi[c], __shfl_xor(hi[c], m, 64));
+        }
+      ext[o] = (hi[0] - lo[0]) + (hi[1] - lo[1]) + (hi[2] - lo[2]);
+      if (!(ext[o] <= FLT_MAX)) ext[o] = FLT_MAX;  // NaN or overflow
+    }
+  }
+  if (lane == 0) { part[0][threadIdx.x / 64] = ext[0]; part[1][threadIdx.x / 64] = ext[1]; }
+  __syncthreads();
+  if (threadIdx.x < 2) {
+    const float v = part[threadIdx.x][0] + part[threadIdx.x][1] + part[threadIdx.x][2] + part[threadIdx.x][3];
+    atomicAdd(&spread[threadIdx.x], v);
+  }
+}
+
+// Stream-ordered pool of the sorting scratch, one per device, created at first use.  It keeps
+// up to kPoolKeep bytes after a synchronisation (the default pool would return them to the
+// driver, and map them again at the next call: 0.3 ms per call for 4M queries).
+constexpr uint64_t kPoolKeep = 256ull << 20;
+constexpr int kMaxPoolDevices = 64;
+
+hipMemPool_t scratch_pool(int device) {
+  static std::mutex mu;
+  static hipMemPool_t pools[kMaxPoolDevices] = {};
+  if (device < 0 || device >= kMaxPoolDevices) return nullptr;
+  std::lock_guard<std::mutex> lock(mu);
+  if (!pools[device]) {
+    hipMemPoolProps props = {};
+    props.allocType = hipMemAllocationTypePinned;
+    props.location.type = hipMemLocationTypeDevice;
+    props.location.id = device;
+    hipMemPool_t p = nullptr;
+    if (hipMemPoolCreate(&p, &props) != hipSuccess) { (void)hipGetLastError(); return nullptr; }
+    uint64_t keep = kPoolKeep;
+    (void)hipMemPoolSetAttribute(p, hipMemPoolAttrReleaseThreshold, &keep);
+    (void)hipGetLastError();
+    pools[device] = p;
+  }
+  return pools[device];
+}
+
+// Morton order of the queries in `scratch` (stream-ordered; freed by the caller after the
+// walk) and the spread sums that decide whether the walk takes it, or NULL when the batch is
+// walked in the caller's order.
+uint32_t* query_order(const rmh_map* map, const rmh_vec3* q, uint32_t n, hipStream_t s, void** scratch,
+                      const float** spread_out) {
+  *scratch = nullptr;
+  if (n < kSortQueries || map->root < 0 || map->n_faces == 0) return nullptr;
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(s, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) {
+    (void)hipGetLastError();
+    return nullptr;
+  }
+  size_t tmp_bytes = 0;
+  if (rocprim::radix_sort_pairs(nullptr, tmp_bytes, static_cast<uint32_t*>(nullptr), static_cast<uint32_t*>(nullptr),
+                                static_cast<uint32_t*>(nullptr), static_cast<uint32_t*>(nullptr), n, 0u, 30u,
+                                s) != hipSuccess) {
+    (void)hipGetLastError();
+    return nullptr;
+  }
+  const size_t arr = (static_cast<size_t>(n) * sizeof(uint32_t) + 255) & ~static_cast<size_t>(255);
+  hipMemPool_t pool = scratch_pool(map->device);
+  if (!pool || hipMallocFromPoolAsync(scratch, 4 * arr + 256 + tmp_bytes, pool, s) != hipSuccess) {
+    (void)hipGetLastError();
+    *scratch = nullptr;
+    return nullptr;
+  }
+  char* base = static_cast<char*>(*scratch);
+  uint32_t* keys = reinterpret_cast<uint32_t*>(base);
+  uint32_t* keys2 = reinterpret_cast<uint32_t*>(base + arr);
+  uint32_t* idx = reinterpret_cast<uint32_t*>(base + 2 * arr);
+  uint32_t* order = reinterpret_cast<uint32_t*>(base + 3 * arr);
+  float* spread = reinterpret_cast<float*>(base + 4 * arr);
+  const uint32_t waves = (n + 63) / 64, sampled = std::min(waves, kSpreadWaves);
+  const uint32_t stride = waves / sampled;  // >= 1; sampled * stride <= waves
+  k_query_keys<<<(n + 255) / 256, 256, 0, s>>>(map->nodes, map->root, q, n, keys, idx, spread);
+  bool ok = hipGetLastError() == hipSuccess &&
+            rocprim::radix_sort_pairs(base + 4 * arr + 256, tmp_bytes, keys, keys2, idx, order, n, 0u, 30u, s) ==
+                hipSuccess;
+  if (ok) {
+    k_order_spread<<<(sampled + 3) / 4, 256, 0, s>>>(q, order, n, stride, spread);
+    ok = hipGetLastError() == hipSuccess;
+  }
+  if (!ok) {
+    (void)hipGetLastError();
+    (void)hipFreeAsync(*scratch, s);
+    *scratch = nullptr;
+    return nullptr;
+  }
+  *spread_out = spread;
+  return order;
+}
+
 }  // namespace
 
 extern "C" int rmh_closest_points(const rmh_map* map, const rmh_vec3* queries, uint32_t n, float max_distance,
@@ -198,10 +364,15 @@ extern "C" int rmh_closest_points(const rmh_map* map, const rmh_vec3* queries, u
   a.out = *out;
   const uint32_t blocks = (n + 63) / 64;
   hipStream_t s = static_cast<hipStream_t>(stream);
+  void* scratch = nullptr;
+  a.spread = nullptr;
+  a.order = query_order(map, queries, n, s, &scratch, &a.spread);
   const uint32_t depth = map->info.max_depth;
   if (depth <= 32) k_closest<32><<<blocks, 64, 0, s>>>(a);
   else if (depth <= 64) k_closest<64><<<blocks, 64, 0, s>>>(a);
   else k_closest<128><<<blocks, 64, 0, s>>>(a);
-  RMH_HIP_TRY(hipGetLastError());
+  const hipError_t e = hipGetLastError();
+  if (scratch) RMH_HIP_TRY(hipFreeAsync(scratch, s));
+  RMH_HIP_TRY(e);
   return RMH_OK;
 }

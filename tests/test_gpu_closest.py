@@ -83,3 +83,29 @@ def test_imported_sphere_surface_and_empty():
     empty = HipMap(np.zeros((0, 3), np.float32), np.zeros((0, 3), np.uint32))
     r = empty.closest_points(torch.as_tensor(Q[:5], device="cuda"))
     assert (r["dist"].cpu().numpy() == C.FLT_MAX).all()
+
+
+def test_large_batch_in_morton_order_equals_caller_order():
+    """Batches of 65,536 queries or more are walked in the Morton order of their cells within
+    the map's box; every output equals that of the same queries in batches below that size,
+    which are walked in the caller's order (queries far outside the box included)."""
+    V, F, Tbm = scenes.room_workload(n_sensors=1)
+    hm = HipMap(V, F)
+    rng = np.random.default_rng(11)
+    Q = rng.uniform([-14, -14, -3], [14, 14, 7], size=(150_000, 3)).astype(np.float32)
+    Q[:100] *= 50
+    Qd = torch.as_tensor(Q, device="cuda")
+    for max_distance in (C.FLT_MAX, 0.3):
+        big = hm.closest_points(Qd, max_distance)
+        parts = [hm.closest_points(Qd[i:i + 50_000], max_distance) for i in range(0, len(Q), 50_000)]
+        torch.cuda.synchronize()
+        for k in big:
+            np.testing.assert_array_equal(big[k].cpu().numpy(), torch.cat([p[k] for p in parts]).cpu().numpy(),
+                                          err_msg=f"{k} max {max_distance}")
+        sel = rng.choice(len(Q), 300, replace=False)
+        d, p, f = C.closest_points(V, F, Q[sel], max_distance)
+        gd = big["dist"].cpu().numpy()[sel]
+        np.testing.assert_array_equal(gd == C.FLT_MAX, d == C.FLT_MAX)
+        ok = d != C.FLT_MAX
+        np.testing.assert_allclose(gd[ok], d[ok], rtol=2e-7, atol=1e-7)
+        print(f"max {max_distance}: {len(Q)} queries equal in both orders, {ok.sum()} of 300 sampled hits match the oracle")
