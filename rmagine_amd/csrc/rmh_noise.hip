@@ -14,7 +14,9 @@
 //   uniform dust  d = min(r, max_range); with probability 1 - (1 - p_hit)^d a dust particle
 //                 at u * d is seen and returned with probability p_return^(u d): r = u d
 // n is a standard normal draw (Box-Muller, u1 in (0, 1], u2 in [0, 1)), u uniform; the
-// transcendental parts run in FP64 and round once to FP32.
+// transcendental parts run in FP64 and round once to FP32.  r^1 is r, and the dust powers
+// of per-call bases are exp(y log base) with log base computed once on the host (pow's
+// x^0 = 1 kept): differences from pow far below the FP32 rounding of the result.
 #include "rmh_internal.hpp"
 
 #include <cmath>
@@ -47,28 +49,40 @@ __device__ __forceinline__ double unit_open1(uint32_t x) { return static_cast<do
 struct NoiseArgs {
   float* r;
   uint64_t n;
+  double log_miss, log_return;  // dust: log(1 - p_hit), log(p_return)
   uint32_t kind, seed, epoch;
   float a, b, c, max_range;
 };
 
+// base^y from log(base) for base >= 0, as pow(base, y) has it for y = 0 (1 for every base);
+// other bases (negative, NaN) go to pow
+__device__ __forceinline__ double pow_from_log(double base, double log_base, double y) {
+  if (y == 0.0) return 1.0;
+  if (!(base >= 0.0)) return pow(base, y);
+  return exp(y * log_base);
+}
+
+// One instantiation per model, so each carries only its own transcendentals (and registers).
+template <uint32_t KIND>
 __global__ __launch_bounds__(256) void k_noise(const NoiseArgs a) {
   const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
   for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < a.n; i += stride) {
     const float range = a.r[i];
-    const U4 w = philox4x32_10(U4{static_cast<uint32_t>(i), static_cast<uint32_t>(i >> 32), a.epoch, a.kind},
+    const U4 w = philox4x32_10(U4{static_cast<uint32_t>(i), static_cast<uint32_t>(i >> 32), a.epoch, KIND},
                                a.seed, 0x5EEDu);
-    if (a.kind == RMH_NOISE_UNIFORM_DUST) {
+    if constexpr (KIND == RMH_NOISE_UNIFORM_DUST) {
       const float d = range > a.max_range ? a.max_range : range;
-      const double p_hit = 1.0 - pow(1.0 - static_cast<double>(a.a), static_cast<double>(d));
+      const double p_hit = 1.0 - pow_from_log(1.0 - static_cast<double>(a.a), a.log_miss, static_cast<double>(d));
       if (unit_open1(w.x) < p_hit) {
         const float nr = static_cast<float>(unit_open1(w.y) * static_cast<double>(d));
-        const double p_ret = pow(static_cast<double>(a.b), static_cast<double>(nr));
+        const double p_ret = pow_from_log(static_cast<double>(a.b), a.log_return, static_cast<double>(nr));
         if (unit_open1(w.z) < p_ret) a.r[i] = nr;
       }
     } else if (range <= a.max_range) {
       const double n = sqrt(-2.0 * log(unit_open0(w.x))) * cos(6.283185307179586 * unit_open1(w.y));
       double sd = static_cast<double>(a.b);
-      if (a.kind == RMH_NOISE_REL_GAUSSIAN) sd *= pow(static_cast<double>(range), static_cast<double>(a.c));
+      if constexpr (KIND == RMH_NOISE_REL_GAUSSIAN)
+        sd *= a.c == 1.0f ? static_cast<double>(range) : pow(static_cast<double>(range), static_cast<double>(a.c));
       a.r[i] = static_cast<float>(static_cast<double>(range) + n * sd + static_cast<double>(a.a));
     }
   }
@@ -94,9 +108,14 @@ extern "C" int rmh_noise_apply(const rmh_noise_params* params, float* ranges, ui
   a.b = params->p1;
   a.c = params->p2;
   a.max_range = params->max_range;
+  a.log_miss = std::log(1.0 - static_cast<double>(params->p0));
+  a.log_return = std::log(static_cast<double>(params->p1));
   const uint64_t want = (n + 255) / 256;
   const uint32_t blocks = static_cast<uint32_t>(want < 65536 ? want : 65536);
-  k_noise<<<blocks, 256, 0, static_cast<hipStream_t>(stream)>>>(a);
+  const hipStream_t s = static_cast<hipStream_t>(stream);
+  if (a.kind == RMH_NOISE_GAUSSIAN) k_noise<RMH_NOISE_GAUSSIAN><<<blocks, 256, 0, s>>>(a);
+  else if (a.kind == RMH_NOISE_REL_GAUSSIAN) k_noise<RMH_NOISE_REL_GAUSSIAN><<<blocks, 256, 0, s>>>(a);
+  else k_noise<RMH_NOISE_UNIFORM_DUST><<<blocks, 256, 0, s>>>(a);
   RMH_HIP_TRY(hipGetLastError());
   return RMH_OK;
 }

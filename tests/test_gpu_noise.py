@@ -21,6 +21,7 @@ def _ranges(n, seed=0):
 @pytest.mark.parametrize("cls,args,kind", [
     (noise.GaussianNoiseHip, (0.01, 0.05), 0),
     (noise.RelGaussianNoiseHip, (0.0, 0.02, 0.7), 1),
+    (noise.RelGaussianNoiseHip, (0.005, 0.01, 1.0), 1),
     (noise.UniformDustNoiseHip, (0.01, 0.95), 2),
 ])
 def test_matches_oracle_over_calls(cls, args, kind):
