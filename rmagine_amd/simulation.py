@@ -153,7 +153,10 @@ class HipMap:
         max_distance: the batched form of EmbreeMap::closestPoint
         (src/rmagine_embree/src/map/EmbreeMap.cpp:33-37).  Returns dist (N,), points (N,3),
         normals (N,3), face_ids, geom_ids, object_ids (N,) device tensors; misses have dist =
-        FLT_MAX, ids 0xFFFFFFFF (as int32 -1) and NaN point/normal."""
+        FLT_MAX, ids 0xFFFFFFFF (as int32 -1) and NaN point/normal.  Batches of 65,536
+        queries or more are sorted on the device by the Morton code of their positions and
+        walked in that order when their own order is spatially incoherent; the results are
+        per query and do not depend on the order."""
         if not (isinstance(queries, torch.Tensor) and queries.dtype == torch.float32 and queries.is_cuda
                 and queries.is_contiguous() and queries.dim() == 2 and queries.shape[1] == 3):
             raise ValueError("queries must be a contiguous (N,3) float32 CUDA tensor")
